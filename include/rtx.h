@@ -1,0 +1,220 @@
+/*
+ * rtx.h — C ABI of the MI355X ray-traversal library (librtx.so).
+ *
+ * The library replaces the functional BVH traversal that Vulkan-Sim runs per
+ * lane inside its simulated RT unit:
+ *   VulkanRayTracing::traceRay              src/cuda-sim/vulkan_ray_tracing.cc:2309-3076
+ *   VulkanRayTracing::traceRayWithTreelets  src/cuda-sim/vulkan_ray_tracing.cc:1522-2307
+ *   VulkanRayTracing::createTreelets        src/cuda-sim/vulkan_ray_tracing.cc:823-1470
+ *   VulkanRayTracing::buildNodeToRootMap    src/cuda-sim/vulkan_ray_tracing.cc:475-489
+ * declared in src/cuda-sim/vulkan_ray_tracing.h:329-356.
+ *
+ * The reference is called once per ray with host pointers into a Mesa-owned
+ * acceleration structure and returns its result through simulator globals.
+ * This ABI is batched: the acceleration structure is one byte arena (TLAS and
+ * every BLAS, node layout of src/cuda-sim/vulkan_acceleration_structure_util.h,
+ * relative offsets unchanged) copied to or borrowed in device memory, and every
+ * node is named by its byte offset in that arena.  That offset replaces the
+ * simulated device address the reference stores in MemoryTransactionRecord
+ * (src/abstract_hardware_model.h:315-321).
+ *
+ * All entry points return rtx_status and never abort (the reference asserts).
+ * Pointers named *_dev are device pointers (from rtx_malloc or any HIP
+ * allocation on the context's device); pointers named *_host are host memory.
+ * Work is ordered on the context's stream; rtx_synchronize waits for it.
+ * A context is not thread-safe; distinct contexts are independent.
+ */
+#ifndef RTX_H
+#define RTX_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum rtx_status {
+  RTX_OK = 0,
+  RTX_ERR_INVALID_ARGUMENT = 1,
+  RTX_ERR_BAD_NODE_TYPE = 2,           /* ChildType not internal/instance/quad/procedural */
+  RTX_ERR_OFFSET_OUT_OF_ARENA = 3,     /* a node or child offset points outside the arena */
+  RTX_ERR_NODE_LARGER_THAN_BUDGET = 4, /* reference asserts at vulkan_ray_tracing.cc:893,962,976 */
+  RTX_ERR_TRACE_OVERFLOW = 5,          /* record buffer smaller than the records produced */
+  RTX_ERR_STACK_OVERFLOW = 6,          /* per-ray traversal stack exceeded its capacity */
+  RTX_ERR_HIP = 7,
+  RTX_ERR_NO_DEVICE = 8,
+  RTX_ERR_NO_SCENE = 9,
+  RTX_ERR_NO_PARTITION = 10,
+  RTX_ERR_UNSUPPORTED = 11,
+  RTX_ERR_OUT_OF_MEMORY = 12,
+  RTX_ERR_TIMEOUT = 13
+} rtx_status;
+
+/* Traversal order. PLAIN follows traceRay, TREELET follows traceRayWithTreelets. */
+enum { RTX_MODE_PLAIN = 0, RTX_MODE_TREELET = 1 };
+
+/* Treelet staging (prefetch into LDS).  Never changes hits or traces. */
+enum {
+  RTX_PREFETCH_NONE = 0,       /* every node read goes to global memory */
+  RTX_PREFETCH_ALWAYS = 1,     /* heuristic 0: stage the wave's most popular treelet */
+  RTX_PREFETCH_POPULARITY = 2, /* heuristic 1: stage it only if its share >= threshold */
+  RTX_PREFETCH_PARTIAL = 3,    /* heuristic 2: stage the first round(n*share) nodes */
+  RTX_PREFETCH_LEARNED = 4     /* MLP predicts the next treelet; heuristic 0 is the fallback */
+};
+/* Heuristic numbering follows src/gpgpu-sim/shader.cc:3437-3559. */
+
+/* SPIR-V ray flags used by the traversal (SpvRayFlags*KHRMask). */
+enum {
+  RTX_RAY_FLAG_OPAQUE = 0x1,
+  RTX_RAY_FLAG_TERMINATE_ON_FIRST_HIT = 0x4,
+  RTX_RAY_FLAG_SKIP_CLOSEST_HIT_SHADER = 0x8
+};
+
+/* TransactionType, src/abstract_hardware_model.h:201-211. */
+enum {
+  RTX_TX_BVH_STRUCTURE = 0,
+  RTX_TX_BVH_INTERNAL_NODE = 1,
+  RTX_TX_BVH_INSTANCE_LEAF = 2,
+  RTX_TX_BVH_PRIMITIVE_LEAF_DESCRIPTOR = 3,
+  RTX_TX_BVH_QUAD_LEAF = 4,
+  RTX_TX_BVH_QUAD_LEAF_HIT = 5,
+  RTX_TX_BVH_PROCEDURAL_LEAF = 6
+};
+
+#define RTX_NO_TREELET 0xFFFFFFFFu
+
+/* One ray: the traceRay operands decoded by trace_ray_impl
+ * (src/cuda-sim/instructions.cc:7158-7227).  cull_mask, sbt_* and miss_index
+ * are carried through untouched, as in the reference (vulkan_ray_tracing.cc:2389-2395). */
+typedef struct rtx_ray {
+  float org[3];
+  float tmin;
+  float dir[3];
+  float tmax;
+  uint32_t flags;
+  uint32_t cull_mask;
+  uint32_t sbt_record_offset;
+  uint32_t sbt_record_stride;
+  uint32_t miss_index;
+  uint32_t reserved[3];
+} rtx_ray; /* 64 bytes */
+
+/* Closest-hit record: the fields of Traversal_data / Hit_data
+ * (src/cuda-sim/vulkan_rt_thread_data.h:29-58) plus per-ray counters that the
+ * reference accumulates in its global statistics (gpu-sim.cc:2168-2182). */
+typedef struct rtx_hit {
+  uint32_t hit;             /* hit_geometry */
+  uint32_t primitive_index; /* closest_leaf.PrimitiveIndex0 */
+  uint32_t geometry_index;  /* closest_leaf.LeafDescriptor.GeometryIndex */
+  uint32_t instance_index;  /* closest_instanceLeaf.InstanceID */
+  float t;                  /* world_min_thit */
+  float p[3];               /* intersection_point = origin + direction * t */
+  float bary[3];            /* Barycentric(): (v, w, u) */
+  uint32_t n_records;       /* memory transactions of this ray */
+  uint32_t n_nodes;         /* total_nodes_accessed */
+  uint32_t fetch_bytes;     /* sum of record sizes, leaf-descriptor records excluded */
+  uint32_t n_any_hits;      /* accepted hits of a non-opaque ray (any-hit shader calls) */
+  uint32_t max_level;       /* deepest tree level assigned (tree_level_map) */
+} rtx_hit; /* 64 bytes */
+
+/* MemoryTransactionRecord (src/abstract_hardware_model.h:315-321) with the
+ * node's treelet attached (addrToTreeletID, vulkan_ray_tracing.cc:468-472). */
+typedef struct rtx_record {
+  uint64_t offset;  /* arena byte offset of the node */
+  uint32_t treelet; /* dense treelet index of the node's owner, or RTX_NO_TREELET */
+  uint16_t size;    /* bytes: 64, 128 or 8 */
+  uint8_t type;     /* RTX_TX_* */
+  uint8_t reserved;
+} rtx_record; /* 16 bytes */
+
+typedef struct rtx_trace_options {
+  uint32_t mode;         /* RTX_MODE_* */
+  uint32_t prefetch;     /* RTX_PREFETCH_* */
+  float threshold;       /* POPULARITY share threshold (reference default 0.5, tested 0.75) */
+  uint32_t emit_records; /* 1: fill the record buffer (two passes: count, then write) */
+  uint32_t waves_per_cu; /* launch shape; 0 = library default */
+  uint32_t lds_stack;    /* stack entries per lane held in LDS; 0 = library default */
+  uint32_t reserved[2];
+} rtx_trace_options;
+
+/* Prefetch accounting of the last rtx_trace call, modelled on the
+ * TIMELY / NEVER_USED classes of src/gpgpu-sim/gpu-sim.cc:1750-1868. */
+typedef struct rtx_prefetch_stats {
+  uint64_t node_reads;        /* node reads issued by the traversal */
+  uint64_t lds_hits;          /* node reads served from a staged treelet (timely) */
+  uint64_t staged_treelets;   /* treelet loads into LDS */
+  uint64_t staged_bytes;      /* bytes copied from HBM into LDS */
+  uint64_t wasted_bytes;      /* staged bytes never read before eviction (never used) */
+  uint64_t predictions;       /* learned mode: predictions made */
+  uint64_t prediction_hits;   /* learned mode: predicted top-k contained the next treelet */
+  uint64_t fallbacks;         /* learned mode: heuristic used instead */
+} rtx_prefetch_stats;
+
+typedef struct rtx_treelet_summary {
+  uint32_t n_treelets;      /* "Treelet Count" (vulkan_ray_tracing.cc:1366) */
+  uint32_t budget_bytes;    /* maxBytesPerTreelet */
+  uint64_t n_list_entries;  /* total node-list entries after de-duplication */
+  uint64_t total_bytes;     /* bytes placed in treelets, each distinct root counted once */
+  uint32_t max_treelet_bytes;
+  uint32_t max_treelet_nodes;
+} rtx_treelet_summary;
+
+typedef struct rtx_ctx rtx_ctx;
+
+/* ---- context and memory ------------------------------------------------ */
+const char* rtx_status_string(rtx_status s);
+const char* rtx_version(void);
+rtx_status rtx_device_count(int* count);
+rtx_status rtx_create(int device, rtx_ctx** out);
+rtx_status rtx_destroy(rtx_ctx* ctx);
+rtx_status rtx_synchronize(rtx_ctx* ctx);
+rtx_status rtx_malloc(rtx_ctx* ctx, size_t bytes, void** out_dev);
+rtx_status rtx_free(rtx_ctx* ctx, void* dev);
+rtx_status rtx_memcpy_h2d(rtx_ctx* ctx, void* dst_dev, const void* src_host, size_t bytes);
+rtx_status rtx_memcpy_d2h(rtx_ctx* ctx, void* dst_host, const void* src_dev, size_t bytes);
+rtx_status rtx_memset(rtx_ctx* ctx, void* dev, int value, size_t bytes);
+/* Stream-ordered timer around work issued on the context's stream. */
+rtx_status rtx_timer_start(rtx_ctx* ctx);
+rtx_status rtx_timer_stop(rtx_ctx* ctx, float* ms);
+rtx_status rtx_device_info(rtx_ctx* ctx, int* n_cus, int* lds_bytes_per_cu, char* name, size_t name_len);
+
+/* ---- scene ---------------------------------------------------------------
+ * Replaces the TLAS pointer handed to traceRay (instructions.cc:7161) and the
+ * BLAS/TLAS host-to-device maps of allocTLAS/allocBLAS (vulkan_ray_tracing.cc:4891-4899).
+ * tlas_offset is the arena offset of the TLAS GEN_RT_BVH header. */
+rtx_status rtx_set_scene(rtx_ctx* ctx, const void* arena_host, size_t bytes, uint64_t tlas_offset);
+rtx_status rtx_set_scene_device(rtx_ctx* ctx, const void* arena_dev, size_t bytes, uint64_t tlas_offset);
+/* Walk the whole TLAS->BLAS DAG on the device and check node types and offsets. */
+rtx_status rtx_validate_scene(rtx_ctx* ctx, uint64_t* n_nodes_reached);
+
+/* ---- treelets ------------------------------------------------------------
+ * Device replacement of createTreelets + buildNodeToRootMap with budget
+ * maxBytesPerTreelet (gpu-sim.cc:910-912; tested 512 B, configs 8-64 KiB). */
+rtx_status rtx_build_treelets(rtx_ctx* ctx, uint32_t budget_bytes, rtx_treelet_summary* out);
+/* Copy the partition to host memory.  Any pointer may be NULL.
+ *  roots[n_treelets]          treelet root offsets, ascending (treelet_addr_to_metadata_idx order)
+ *  list_offsets[n_treelets+1] CSR offsets into list_nodes
+ *  list_nodes[n_list_entries] de-duplicated node offsets of each treelet, in insertion order
+ *  owner[arena_bytes/64]      dense treelet index owning the node at offset 64*i, or RTX_NO_TREELET */
+rtx_status rtx_get_treelets(rtx_ctx* ctx, uint64_t* roots, uint64_t* list_offsets,
+                            uint64_t* list_nodes, uint32_t* owner);
+/* Install a partition computed elsewhere (same arrays as rtx_get_treelets). */
+rtx_status rtx_set_treelets(rtx_ctx* ctx, uint32_t budget_bytes, uint32_t n_treelets,
+                            const uint64_t* roots, const uint64_t* list_offsets,
+                            const uint64_t* list_nodes, const uint32_t* owner);
+
+/* ---- traversal -----------------------------------------------------------
+ * Trace n rays.  hits_dev[n] is always written.  When opts->emit_records is 1,
+ * record_offsets_dev[n+1] receives the CSR offsets and records_dev the records
+ * (record_capacity entries); a larger total returns RTX_ERR_TRACE_OVERFLOW with
+ * record_offsets_dev still filled, so the caller can size the buffer. */
+rtx_status rtx_trace(rtx_ctx* ctx, const rtx_ray* rays_dev, uint32_t n, const rtx_trace_options* opts,
+                     rtx_hit* hits_dev, rtx_record* records_dev, uint64_t record_capacity,
+                     uint64_t* record_offsets_dev);
+rtx_status rtx_get_prefetch_stats(rtx_ctx* ctx, rtx_prefetch_stats* out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* RTX_H */

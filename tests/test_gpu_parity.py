@@ -1,0 +1,77 @@
+"""GPU traversal and partition vs the CPU oracle, bit-exact (integer, index and FP32 bit patterns)."""
+import numpy as np
+import pytest
+
+from conftest import assert_hits_equal, assert_records_equal
+
+pytestmark = pytest.mark.gpu
+
+
+def _scene_on(gpu, scene):
+    gpu.set_scene(scene.arena, scene.tlas_offset)
+
+
+@pytest.mark.parametrize("lds_stack", [4, 8, 16])
+def test_plain_c1_hits_and_records(gpu, oracle, c1, lds_stack):
+    scene, rays = c1
+    _scene_on(gpu, scene)
+    h_ref, o_ref, r_ref = oracle.trace(scene.arena, scene.tlas_offset, rays, mode=0)
+    h, o, r = gpu.trace(rays, mode=0, records=True, lds_stack=lds_stack)
+    assert_hits_equal(h, h_ref, "plain C1")
+    assert_records_equal(o, r, o_ref, r_ref, "plain C1")
+    assert h["hit"].sum() > 0.5 * len(rays)
+
+
+@pytest.mark.parametrize("budget", [512, 8192, 65536])
+def test_partition_c1(gpu, oracle, c1, budget):
+    scene, _ = c1
+    _scene_on(gpu, scene)
+    s = gpu.build_treelets(budget)
+    p = oracle.Partition(scene.arena, scene.tlas_offset, budget)
+    t = gpu.get_treelets()
+    assert s["n_treelets"] == p.n_treelets
+    assert np.array_equal(t["roots"], p.roots)
+    assert np.array_equal(t["list_offsets"], p.list_offsets)
+    assert np.array_equal(t["list_nodes"], p.list_nodes)
+    assert np.array_equal(t["owner"], p.owner)
+    assert s["total_bytes"] == p.total_bytes
+
+
+@pytest.mark.parametrize("budget", [512, 8192, 65536])
+def test_treelet_c1_hits_and_records(gpu, oracle, c1, budget):
+    scene, rays = c1
+    _scene_on(gpu, scene)
+    gpu.build_treelets(budget)
+    p = oracle.Partition(scene.arena, scene.tlas_offset, budget)
+    h_ref, o_ref, r_ref = oracle.trace(scene.arena, scene.tlas_offset, rays, mode=1, partition=p)
+    h, o, r = gpu.trace(rays, mode=1, records=True)
+    assert_hits_equal(h, h_ref, f"treelet C1 budget {budget}")
+    assert_records_equal(o, r, o_ref, r_ref, f"treelet C1 budget {budget}")
+
+
+def test_plain_records_carry_treelets(gpu, oracle, c1):
+    scene, rays = c1
+    _scene_on(gpu, scene)
+    gpu.build_treelets(8192)
+    p = oracle.Partition(scene.arena, scene.tlas_offset, 8192)
+    h_ref, o_ref, r_ref = oracle.trace(scene.arena, scene.tlas_offset, rays, mode=0, partition=p)
+    h, o, r = gpu.trace(rays, mode=0, records=True)
+    assert_hits_equal(h, h_ref, "plain with partition")
+    assert_records_equal(o, r, o_ref, r_ref, "plain with partition")
+
+
+@pytest.mark.parametrize("mode", [0, 1])
+def test_instanced_scenes(gpu, oracle, instanced, instanced_shared, mode):
+    for scene, rays in (instanced, instanced_shared):
+        _scene_on(gpu, scene)
+        p = None
+        if mode == 1:
+            gpu.build_treelets(4096)
+            p = oracle.Partition(scene.arena, scene.tlas_offset, 4096)
+            t = gpu.get_treelets()
+            assert np.array_equal(t["owner"], p.owner)
+            assert np.array_equal(t["list_nodes"], p.list_nodes)
+        h_ref, o_ref, r_ref = oracle.trace(scene.arena, scene.tlas_offset, rays, mode=mode, partition=p)
+        h, o, r = gpu.trace(rays, mode=mode, records=True)
+        assert_hits_equal(h, h_ref, f"instanced mode {mode}")
+        assert_records_equal(o, r, o_ref, r_ref, f"instanced mode {mode}")

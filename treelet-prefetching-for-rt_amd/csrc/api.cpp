@@ -1,0 +1,439 @@
+// api.cpp — host runtime behind include/rtx.h.
+//
+// A context owns one HIP stream on one device, the scene arena (copied or
+// borrowed), the treelet partition and the scratch the kernels need.  Every
+// entry point validates its arguments and returns an rtx_status; nothing
+// aborts.  rtx_trace is synchronous: it returns once the rays are traced so
+// that kernel-side errors (stack overflow, malformed nodes) reach the caller.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdio>
+#include <cstring>
+#include <new>
+#include <vector>
+
+#include "kernels.h"
+#include "partition.h"
+#include "rtx.h"
+
+using namespace rtxk;
+
+struct rtx_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  int n_cus = 0;
+  int lds_per_cu = 0;
+  char name[128] = {0};
+  // scene
+  const uint8_t* arena = nullptr;
+  uint8_t* arena_owned = nullptr;
+  uint64_t arena_bytes = 0;
+  uint64_t tlas = 0;
+  // partition
+  Partition part;
+  // scratch
+  uint64_t* spill = nullptr;
+  size_t spill_entries = 0;
+  uint32_t* counters = nullptr;  // [0] ray counter, [1] error word
+  unsigned long long* stats = nullptr;
+  uint64_t* scan_tmp = nullptr;
+  size_t scan_tmp_entries = 0;
+  rtx_prefetch_stats last_stats{};
+};
+
+namespace {
+
+rtx_status hip_status(hipError_t e) { return e == hipSuccess ? RTX_OK : RTX_ERR_HIP; }
+
+#define HIP_TRY(expr)                         \
+  do {                                        \
+    hipError_t _e = (expr);                   \
+    if (_e != hipSuccess) {                   \
+      last_hip_error = _e;                    \
+      return RTX_ERR_HIP;                     \
+    }                                         \
+  } while (0)
+
+thread_local hipError_t last_hip_error = hipSuccess;
+
+rtx_status set_device(rtx_ctx* c) {
+  HIP_TRY(hipSetDevice(c->device));
+  return RTX_OK;
+}
+
+rtx_status kernel_error_status(uint32_t bits) {
+  if (bits == 0) return RTX_OK;
+  if (bits & kErrRange) return RTX_ERR_OFFSET_OUT_OF_ARENA;
+  if (bits & kErrType) return RTX_ERR_BAD_NODE_TYPE;
+  if (bits & kErrStack) return RTX_ERR_STACK_OVERFLOW;
+  if (bits & kErrRecords) return RTX_ERR_TRACE_OVERFLOW;
+  if (bits & kErrBudget) return RTX_ERR_NODE_LARGER_THAN_BUDGET;
+  if (bits & kErrInstSlot) return RTX_ERR_UNSUPPORTED;
+  if (bits & kErrSteps) return RTX_ERR_BAD_NODE_TYPE;
+  return RTX_ERR_HIP;
+}
+
+void free_scene(rtx_ctx* c) {
+  if (c->arena_owned) hipFree(c->arena_owned);
+  c->arena_owned = nullptr;
+  c->arena = nullptr;
+  c->arena_bytes = 0;
+}
+
+rtx_status ensure_spill(rtx_ctx* c, size_t entries) {
+  if (entries <= c->spill_entries) return RTX_OK;
+  if (c->spill) HIP_TRY(hipFree(c->spill));
+  c->spill = nullptr;
+  c->spill_entries = 0;
+  HIP_TRY(hipMalloc(&c->spill, entries * sizeof(uint64_t)));
+  c->spill_entries = entries;
+  return RTX_OK;
+}
+
+rtx_status ensure_scan_tmp(rtx_ctx* c, size_t entries) {
+  if (entries <= c->scan_tmp_entries) return RTX_OK;
+  if (c->scan_tmp) HIP_TRY(hipFree(c->scan_tmp));
+  c->scan_tmp = nullptr;
+  c->scan_tmp_entries = 0;
+  HIP_TRY(hipMalloc(&c->scan_tmp, entries * sizeof(uint64_t)));
+  c->scan_tmp_entries = entries;
+  return RTX_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* rtx_status_string(rtx_status s) {
+  switch (s) {
+    case RTX_OK: return "ok";
+    case RTX_ERR_INVALID_ARGUMENT: return "invalid argument";
+    case RTX_ERR_BAD_NODE_TYPE: return "malformed node (bad child type, non-zero PrimitiveIndex1Delta or cyclic BVH)";
+    case RTX_ERR_OFFSET_OUT_OF_ARENA: return "node offset outside the arena";
+    case RTX_ERR_NODE_LARGER_THAN_BUDGET: return "treelet budget smaller than a node";
+    case RTX_ERR_TRACE_OVERFLOW: return "record buffer too small";
+    case RTX_ERR_STACK_OVERFLOW: return "traversal stack overflow";
+    case RTX_ERR_HIP: return "HIP runtime error";
+    case RTX_ERR_NO_DEVICE: return "no HIP device";
+    case RTX_ERR_NO_SCENE: return "no scene set";
+    case RTX_ERR_NO_PARTITION: return "no treelet partition built";
+    case RTX_ERR_UNSUPPORTED: return "unsupported input";
+    case RTX_ERR_OUT_OF_MEMORY: return "out of memory";
+    case RTX_ERR_TIMEOUT: return "timeout";
+  }
+  return "unknown status";
+}
+
+const char* rtx_version(void) { return "rtx-mi355x 0.1 (gfx950)"; }
+
+rtx_status rtx_device_count(int* count) {
+  if (!count) return RTX_ERR_INVALID_ARGUMENT;
+  *count = 0;
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess) {
+    (void)hipGetLastError();
+    return RTX_OK;
+  }
+  *count = n;
+  return RTX_OK;
+}
+
+rtx_status rtx_create(int device, rtx_ctx** out) {
+  if (!out) return RTX_ERR_INVALID_ARGUMENT;
+  *out = nullptr;
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess || n == 0) {
+    (void)hipGetLastError();
+    return RTX_ERR_NO_DEVICE;
+  }
+  if (device < 0 || device >= n) return RTX_ERR_INVALID_ARGUMENT;
+  rtx_ctx* c = new (std::nothrow) rtx_ctx();
+  if (!c) return RTX_ERR_OUT_OF_MEMORY;
+  c->device = device;
+  auto fail = [&](rtx_status st) {
+    rtx_destroy(c);
+    return st;
+  };
+  if (hipSetDevice(device) != hipSuccess) return fail(RTX_ERR_HIP);
+  hipDeviceProp_t prop;
+  if (hipGetDeviceProperties(&prop, device) != hipSuccess) return fail(RTX_ERR_HIP);
+  c->n_cus = prop.multiProcessorCount;
+  c->lds_per_cu = (int)prop.maxSharedMemoryPerMultiProcessor;
+  snprintf(c->name, sizeof(c->name), "%s", prop.gcnArchName);
+  if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) return fail(RTX_ERR_HIP);
+  if (hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess) return fail(RTX_ERR_HIP);
+  if (hipMalloc(&c->counters, 64 * sizeof(uint32_t)) != hipSuccess) return fail(RTX_ERR_OUT_OF_MEMORY);
+  if (hipMalloc(&c->stats, 16 * sizeof(unsigned long long)) != hipSuccess) return fail(RTX_ERR_OUT_OF_MEMORY);
+  *out = c;
+  return RTX_OK;
+}
+
+rtx_status rtx_destroy(rtx_ctx* c) {
+  if (!c) return RTX_ERR_INVALID_ARGUMENT;
+  hipSetDevice(c->device);
+  if (c->stream) hipStreamSynchronize(c->stream);
+  free_scene(c);
+  partition_free(c->part);
+  if (c->spill) hipFree(c->spill);
+  if (c->counters) hipFree(c->counters);
+  if (c->stats) hipFree(c->stats);
+  if (c->scan_tmp) hipFree(c->scan_tmp);
+  if (c->ev0) hipEventDestroy(c->ev0);
+  if (c->ev1) hipEventDestroy(c->ev1);
+  if (c->stream) hipStreamDestroy(c->stream);
+  delete c;
+  return RTX_OK;
+}
+
+rtx_status rtx_synchronize(rtx_ctx* c) {
+  if (!c) return RTX_ERR_INVALID_ARGUMENT;
+  HIP_TRY(hipSetDevice(c->device));
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  return RTX_OK;
+}
+
+rtx_status rtx_malloc(rtx_ctx* c, size_t bytes, void** out) {
+  if (!c || !out || bytes == 0) return RTX_ERR_INVALID_ARGUMENT;
+  *out = nullptr;
+  HIP_TRY(hipSetDevice(c->device));
+  if (hipMalloc(out, bytes) != hipSuccess) {
+    (void)hipGetLastError();
+    *out = nullptr;
+    return RTX_ERR_OUT_OF_MEMORY;
+  }
+  return RTX_OK;
+}
+
+rtx_status rtx_free(rtx_ctx* c, void* p) {
+  if (!c) return RTX_ERR_INVALID_ARGUMENT;
+  if (!p) return RTX_OK;
+  HIP_TRY(hipSetDevice(c->device));
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  HIP_TRY(hipFree(p));
+  return RTX_OK;
+}
+
+rtx_status rtx_memcpy_h2d(rtx_ctx* c, void* dst, const void* src, size_t bytes) {
+  if (!c || (!dst && bytes) || (!src && bytes)) return RTX_ERR_INVALID_ARGUMENT;
+  if (!bytes) return RTX_OK;
+  HIP_TRY(hipSetDevice(c->device));
+  HIP_TRY(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, c->stream));
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  return RTX_OK;
+}
+
+rtx_status rtx_memcpy_d2h(rtx_ctx* c, void* dst, const void* src, size_t bytes) {
+  if (!c || (!dst && bytes) || (!src && bytes)) return RTX_ERR_INVALID_ARGUMENT;
+  if (!bytes) return RTX_OK;
+  HIP_TRY(hipSetDevice(c->device));
+  HIP_TRY(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, c->stream));
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  return RTX_OK;
+}
+
+rtx_status rtx_memset(rtx_ctx* c, void* dst, int value, size_t bytes) {
+  if (!c || (!dst && bytes)) return RTX_ERR_INVALID_ARGUMENT;
+  if (!bytes) return RTX_OK;
+  HIP_TRY(hipSetDevice(c->device));
+  HIP_TRY(hipMemsetAsync(dst, value, bytes, c->stream));
+  return RTX_OK;
+}
+
+rtx_status rtx_timer_start(rtx_ctx* c) {
+  if (!c) return RTX_ERR_INVALID_ARGUMENT;
+  HIP_TRY(hipSetDevice(c->device));
+  HIP_TRY(hipEventRecord(c->ev0, c->stream));
+  return RTX_OK;
+}
+
+rtx_status rtx_timer_stop(rtx_ctx* c, float* ms) {
+  if (!c || !ms) return RTX_ERR_INVALID_ARGUMENT;
+  HIP_TRY(hipSetDevice(c->device));
+  HIP_TRY(hipEventRecord(c->ev1, c->stream));
+  HIP_TRY(hipEventSynchronize(c->ev1));
+  HIP_TRY(hipEventElapsedTime(ms, c->ev0, c->ev1));
+  return RTX_OK;
+}
+
+rtx_status rtx_device_info(rtx_ctx* c, int* n_cus, int* lds, char* name, size_t len) {
+  if (!c) return RTX_ERR_INVALID_ARGUMENT;
+  if (n_cus) *n_cus = c->n_cus;
+  if (lds) *lds = c->lds_per_cu;
+  if (name && len) snprintf(name, len, "%s", c->name);
+  return RTX_OK;
+}
+
+static rtx_status check_scene_args(size_t bytes, uint64_t tlas) {
+  if (bytes == 0 || (bytes % 64) != 0 || (tlas % 64) != 0 || tlas + 64 > bytes) return RTX_ERR_INVALID_ARGUMENT;
+  if (bytes / 64 >= 0xffffffffull) return RTX_ERR_UNSUPPORTED;  // node slots are 32-bit
+  return RTX_OK;
+}
+
+rtx_status rtx_set_scene(rtx_ctx* c, const void* host, size_t bytes, uint64_t tlas) {
+  if (!c || !host) return RTX_ERR_INVALID_ARGUMENT;
+  rtx_status st = check_scene_args(bytes, tlas);
+  if (st != RTX_OK) return st;
+  HIP_TRY(hipSetDevice(c->device));
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  free_scene(c);
+  partition_free(c->part);
+  uint8_t* d = nullptr;
+  if (hipMalloc(&d, bytes) != hipSuccess) {
+    (void)hipGetLastError();
+    return RTX_ERR_OUT_OF_MEMORY;
+  }
+  HIP_TRY(hipMemcpy(d, host, bytes, hipMemcpyHostToDevice));
+  c->arena_owned = d;
+  c->arena = d;
+  c->arena_bytes = bytes;
+  c->tlas = tlas;
+  return RTX_OK;
+}
+
+rtx_status rtx_set_scene_device(rtx_ctx* c, const void* dev, size_t bytes, uint64_t tlas) {
+  if (!c || !dev) return RTX_ERR_INVALID_ARGUMENT;
+  rtx_status st = check_scene_args(bytes, tlas);
+  if (st != RTX_OK) return st;
+  if (((uintptr_t)dev & 63) != 0) return RTX_ERR_INVALID_ARGUMENT;
+  HIP_TRY(hipSetDevice(c->device));
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  free_scene(c);
+  partition_free(c->part);
+  c->arena = (const uint8_t*)dev;
+  c->arena_bytes = bytes;
+  c->tlas = tlas;
+  return RTX_OK;
+}
+
+rtx_status rtx_validate_scene(rtx_ctx* c, uint64_t* n_nodes) {
+  if (!c) return RTX_ERR_INVALID_ARGUMENT;
+  if (!c->arena) return RTX_ERR_NO_SCENE;
+  HIP_TRY(hipSetDevice(c->device));
+  return partition_validate(c->arena, c->arena_bytes, c->tlas, c->stream, n_nodes);
+}
+
+rtx_status rtx_build_treelets(rtx_ctx* c, uint32_t budget, rtx_treelet_summary* out) {
+  if (!c) return RTX_ERR_INVALID_ARGUMENT;
+  if (!c->arena) return RTX_ERR_NO_SCENE;
+  if (budget < 128 || budget % 64 != 0 || budget > (1u << 20)) return RTX_ERR_INVALID_ARGUMENT;
+  HIP_TRY(hipSetDevice(c->device));
+  rtx_status st = partition_build(c->arena, c->arena_bytes, c->tlas, budget, c->stream, c->part);
+  if (st != RTX_OK) return st;
+  if (out) *out = c->part.summary;
+  return RTX_OK;
+}
+
+rtx_status rtx_get_treelets(rtx_ctx* c, uint64_t* roots, uint64_t* list_offsets, uint64_t* list_nodes,
+                            uint32_t* owner) {
+  if (!c) return RTX_ERR_INVALID_ARGUMENT;
+  if (!c->part.valid) return RTX_ERR_NO_PARTITION;
+  HIP_TRY(hipSetDevice(c->device));
+  return partition_copy_out(c->part, c->arena_bytes, c->stream, roots, list_offsets, list_nodes, owner);
+}
+
+rtx_status rtx_set_treelets(rtx_ctx* c, uint32_t budget, uint32_t n, const uint64_t* roots,
+                            const uint64_t* list_offsets, const uint64_t* list_nodes, const uint32_t* owner) {
+  if (!c || !roots || !list_offsets || !list_nodes || !owner || n == 0) return RTX_ERR_INVALID_ARGUMENT;
+  if (!c->arena) return RTX_ERR_NO_SCENE;
+  HIP_TRY(hipSetDevice(c->device));
+  return partition_install(c->arena, c->arena_bytes, c->tlas, budget, n, roots, list_offsets, list_nodes, owner,
+                           c->stream, c->part);
+}
+
+rtx_status rtx_trace(rtx_ctx* c, const rtx_ray* rays, uint32_t n, const rtx_trace_options* opts, rtx_hit* hits,
+                     rtx_record* records, uint64_t capacity, uint64_t* offsets) {
+  if (!c || !opts || (n > 0 && (!rays || !hits))) return RTX_ERR_INVALID_ARGUMENT;
+  if (opts->mode != RTX_MODE_PLAIN && opts->mode != RTX_MODE_TREELET) return RTX_ERR_INVALID_ARGUMENT;
+  if (opts->prefetch > RTX_PREFETCH_LEARNED) return RTX_ERR_INVALID_ARGUMENT;
+  if (opts->emit_records && !offsets) return RTX_ERR_INVALID_ARGUMENT;
+  if (!c->arena) return RTX_ERR_NO_SCENE;
+  if ((opts->mode == RTX_MODE_TREELET || opts->prefetch != RTX_PREFETCH_NONE) && !c->part.valid)
+    return RTX_ERR_NO_PARTITION;
+  if (((uintptr_t)rays & 15) || ((uintptr_t)hits & 15) || ((uintptr_t)records & 15)) return RTX_ERR_INVALID_ARGUMENT;
+  HIP_TRY(hipSetDevice(c->device));
+  memset(&c->last_stats, 0, sizeof(c->last_stats));
+  if (n == 0) {
+    if (opts->emit_records) HIP_TRY(hipMemsetAsync(offsets, 0, sizeof(uint64_t), c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    return RTX_OK;
+  }
+  const uint32_t lds_stack = opts->lds_stack ? opts->lds_stack : 8;
+  if (lds_stack != 4 && lds_stack != 8 && lds_stack != 16) return RTX_ERR_INVALID_ARGUMENT;
+  const uint32_t waves_per_cu = opts->waves_per_cu ? opts->waves_per_cu : 16;
+  const uint32_t blocks_per_cu = std::max(1u, waves_per_cu / 4);
+  uint32_t grid = (uint32_t)c->n_cus * blocks_per_cu;
+  grid = std::min<uint32_t>(grid, (n + kBlock - 1) / kBlock);
+  grid = std::max(grid, 1u);
+  const uint32_t spill_cap = 128;
+  const uint32_t ns = opts->mode == RTX_MODE_TREELET ? 2 : 1;
+  rtx_status st = ensure_spill(c, (size_t)grid * kBlock * spill_cap * ns);
+  if (st != RTX_OK) return st;
+
+  TraceParams p{};
+  p.arena = c->arena;
+  p.arena_bytes = c->arena_bytes;
+  p.tlas = c->tlas;
+  p.rays = rays;
+  p.n = n;
+  p.refill = 32;
+  p.hits = hits;
+  p.records = nullptr;
+  p.record_capacity = 0;
+  p.record_offsets = nullptr;
+  p.owner = c->part.valid ? c->part.owner : nullptr;
+  p.first_treelet = c->part.valid ? c->part.first_treelet : 0;
+  p.spill = c->spill;
+  p.spill_cap = spill_cap;
+  p.ray_counter = c->counters;
+  p.error = c->counters + 1;
+  p.stats = c->stats;
+  p.prefetch = opts->prefetch;
+  p.threshold = opts->threshold;
+
+  HIP_TRY(hipMemsetAsync(c->counters, 0, 2 * sizeof(uint32_t), c->stream));
+  HIP_TRY(hipMemsetAsync(c->stats, 0, 16 * sizeof(unsigned long long), c->stream));
+  HIP_TRY(launch_trace(p, opts->mode, false, lds_stack, grid, c->stream));
+  if (opts->emit_records) {
+    const size_t tiles = (n + 1023) / 1024;
+    st = ensure_scan_tmp(c, tiles + 1);
+    if (st != RTX_OK) return st;
+    HIP_TRY(launch_scan_records(hits, n, offsets, c->scan_tmp, c->stream));
+    uint64_t total = 0;
+    uint32_t err = 0;
+    HIP_TRY(hipMemcpyAsync(&total, offsets + n, sizeof(uint64_t), hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipMemcpyAsync(&err, c->counters + 1, sizeof(uint32_t), hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    if (err) return kernel_error_status(err);
+    if (total > capacity || (total > 0 && !records)) return RTX_ERR_TRACE_OVERFLOW;
+    p.records = records;
+    p.record_capacity = capacity;
+    p.record_offsets = offsets;
+    HIP_TRY(hipMemsetAsync(c->counters, 0, 2 * sizeof(uint32_t), c->stream));
+    HIP_TRY(hipMemsetAsync(c->stats, 0, 16 * sizeof(unsigned long long), c->stream));
+    HIP_TRY(launch_trace(p, opts->mode, true, lds_stack, grid, c->stream));
+  }
+  uint32_t err = 0;
+  unsigned long long stats[16];
+  HIP_TRY(hipMemcpyAsync(&err, c->counters + 1, sizeof(uint32_t), hipMemcpyDeviceToHost, c->stream));
+  HIP_TRY(hipMemcpyAsync(stats, c->stats, sizeof(stats), hipMemcpyDeviceToHost, c->stream));
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  c->last_stats.node_reads = stats[0];
+  c->last_stats.lds_hits = stats[1];
+  c->last_stats.staged_treelets = stats[2];
+  c->last_stats.staged_bytes = stats[3];
+  c->last_stats.wasted_bytes = stats[4];
+  c->last_stats.predictions = stats[5];
+  c->last_stats.prediction_hits = stats[6];
+  c->last_stats.fallbacks = stats[7];
+  return kernel_error_status(err);
+}
+
+rtx_status rtx_get_prefetch_stats(rtx_ctx* c, rtx_prefetch_stats* out) {
+  if (!c || !out) return RTX_ERR_INVALID_ARGUMENT;
+  *out = c->last_stats;
+  return RTX_OK;
+}
+
+}  // extern "C"

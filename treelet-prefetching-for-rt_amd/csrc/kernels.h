@@ -1,0 +1,61 @@
+// kernels.h — launch parameters shared by the host runtime and the kernels.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "rtx.h"
+
+namespace rtxk {
+
+// Error bits raised by kernels (mapped to rtx_status on the host).
+enum : uint32_t {
+  kErrStack = 1u,      // per-lane traversal stack overflow
+  kErrType = 2u,       // malformed node type
+  kErrRange = 4u,      // node outside the arena
+  kErrRecords = 8u,    // record buffer overflow
+  kErrInstSlot = 16u,  // instance leaf beyond the 256 MiB the stack entry can name
+  kErrSteps = 32u,     // per-ray step limit (cyclic or corrupt BVH)
+  kErrBudget = 64u,    // treelet node larger than the budget
+  kErrQueue = 128u     // partitioner work buffer overflow
+};
+
+constexpr uint32_t kBlock = 256;        // threads per block (4 waves of 64)
+constexpr uint32_t kMaxStepsPerRay = 1u << 24;
+
+struct TraceParams {
+  const uint8_t* arena;
+  uint64_t arena_bytes;
+  uint64_t tlas;
+  const rtx_ray* rays;
+  uint32_t n;
+  uint32_t refill;  // refill the wave when at least this many lanes are idle
+  rtx_hit* hits;
+  rtx_record* records;
+  uint64_t record_capacity;
+  const uint64_t* record_offsets;
+  const uint32_t* owner;  // node slot -> dense treelet index (nullable in plain mode)
+  uint32_t first_treelet; // dense index of the treelet rooted at the TLAS header
+  uint64_t* spill;        // global stack spill, spill_cap entries per stack per lane
+  uint32_t spill_cap;
+  uint32_t* ray_counter;
+  uint32_t* error;
+  unsigned long long* stats;  // rtx_prefetch_stats counters
+  // treelet staging
+  uint32_t prefetch;                 // RTX_PREFETCH_*
+  float threshold;
+  const uint32_t* stage_slot;        // node slot -> position (64 B units) inside its owner's image
+  const uint64_t* image_offset;      // dense treelet -> byte offset of its image in `images`
+  const uint32_t* image_bytes;       // dense treelet -> image size
+  const uint8_t* images;             // treelet-contiguous copies of the arena
+  uint32_t stage_bytes;              // LDS bytes per wave for a staged treelet
+};
+
+struct PartitionParams;
+
+// Launchers (defined in the .hip files).
+hipError_t launch_trace(const TraceParams& p, uint32_t mode, bool records, uint32_t lds_stack, uint32_t grid,
+                        hipStream_t s);
+hipError_t launch_scan_records(const rtx_hit* hits, uint32_t n, uint64_t* offsets, uint64_t* block_tmp,
+                               hipStream_t s);
+
+}  // namespace rtxk

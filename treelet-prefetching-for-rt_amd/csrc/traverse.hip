@@ -1,0 +1,542 @@
+// traverse.hip — per-ray BVH traversal kernels for gfx950.
+//
+// One ray per lane.  Waves are persistent: whenever at least `refill` lanes of
+// a wave are idle, the wave takes that many ray indices from a global counter
+// with one atomic, so uneven ray costs do not leave lanes idle.
+//
+// Both traversal orders of the reference are flattened into "pop one entry,
+// process it" steps:
+//  * PLAIN (traceRay, vulkan_ray_tracing.cc:2309-3076): the nested top/bottom
+//    loops visit the first hit internal child immediately and push the other
+//    hit children in slot order; that is the same sequence as pushing all hit
+//    children in slot order with the first internal child pushed last.  Only
+//    one instance is open at a time, so the instance context lives in
+//    registers.
+//  * TREELET (traceRayWithTreelets, :1522-2307): two LIFO lists.  A child goes
+//    to the "current" list while the current treelet is still the one rooted
+//    at the TLAS header and the child's owner is that treelet; after the
+//    first move from the "other" list the reference compares against a host
+//    address (:1752), so every later child goes to "other".  Entries carry the
+//    instance leaf that opened them; the object-space ray is recomputed from
+//    that leaf when it differs from the cached one (same inputs, same floats).
+//
+// Stacks: a ring of K entries per lane in LDS, spilling the oldest entries to
+// a per-lane region in global memory (exact LIFO order).
+#include "device_common.h"
+#include "kernels.h"
+
+using namespace rtxd;
+
+namespace rtxk {
+namespace {
+
+constexpr uint32_t kFlTop = 1u, kFlLeaf = 2u;
+constexpr uint32_t kNoSlot = 0xffffffffu;
+
+__device__ __forceinline__ uint64_t make_entry(uint32_t slot, uint32_t flags, uint32_t level, uint32_t inst) {
+  uint32_t lv = level > 255u ? 255u : level;
+  return (uint64_t)slot | ((uint64_t)(flags | (lv << 2) | (inst << 10)) << 32);
+}
+__device__ __forceinline__ uint32_t e_slot(uint64_t e) { return (uint32_t)e; }
+__device__ __forceinline__ uint32_t e_flags(uint64_t e) { return (uint32_t)(e >> 32) & 3u; }
+__device__ __forceinline__ uint32_t e_level(uint64_t e) { return ((uint32_t)(e >> 32) >> 2) & 255u; }
+__device__ __forceinline__ uint32_t e_inst(uint64_t e) { return (uint32_t)(e >> 42); }
+
+struct Stack {
+  uint32_t bot, cnt, g;
+};
+
+template <int K>
+__device__ __forceinline__ bool st_push(Stack& s, uint64_t e, uint64_t* lds, uint64_t* g, uint32_t gcap) {
+  if (s.cnt == (uint32_t)K) {
+    if (s.g >= gcap) return false;
+    g[s.g++] = lds[s.bot * kBlock];
+    s.bot = (s.bot + 1) & (K - 1);
+    s.cnt--;
+  }
+  lds[((s.bot + s.cnt) & (K - 1)) * kBlock] = e;
+  s.cnt++;
+  return true;
+}
+
+template <int K>
+__device__ __forceinline__ uint64_t st_pop(Stack& s, uint64_t* lds, uint64_t* g) {
+  if (s.cnt == 0) return g[--s.g];
+  s.cnt--;
+  return lds[((s.bot + s.cnt) & (K - 1)) * kBlock];
+}
+
+__device__ __forceinline__ bool st_empty(const Stack& s) { return s.cnt == 0 && s.g == 0; }
+
+__device__ __forceinline__ uint32_t lanes_below(uint64_t mask) {
+  return __builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0u));
+}
+
+template <int MODE, bool REC, int K>
+__global__ __launch_bounds__(kBlock) void trace_kernel(TraceParams p) {
+  constexpr int NS = MODE == 1 ? 2 : 1;
+  __shared__ uint64_t lds_stack[NS * K * kBlock];
+  const uint32_t tid = threadIdx.x;
+  const uint32_t lane = tid & 63u;
+  uint64_t* ldsA = lds_stack + tid;
+  uint64_t* ldsB = lds_stack + (NS - 1) * K * kBlock + tid;
+  const uint64_t lane_slot = (uint64_t)blockIdx.x * kBlock + tid;
+  uint64_t* gA = p.spill + lane_slot * (uint64_t)p.spill_cap * NS;
+  uint64_t* gB = gA + (NS - 1) * (uint64_t)p.spill_cap;
+  const uint8_t* __restrict__ A = p.arena;
+  const uint64_t bytes = p.arena_bytes;
+
+  Stack sa{0, 0, 0}, sb{0, 0, 0};
+  bool active = false, exhausted = false;
+  uint32_t ray_id = 0;
+  uint32_t err = 0;
+
+  F3 ro{}, rd{}, ridir{};
+  float rtmin = 0.f, rtmax = 0.f;
+  uint32_t rflags = 0;
+  float min_thit = 0.f, min_thit_obj = 0.f;
+  uint32_t closest_leaf = kNoSlot, closest_inst = kNoSlot;
+  F3 cobj_o{}, cobj_d{};
+  ObjRay cur{};
+  uint32_t cur_inst = kNoSlot;
+  uint32_t n_rec = 0, n_nodes = 0, fetch_bytes = 0, n_any = 0, max_level = 0, steps = 0;
+  uint64_t rec_base = 0;
+  bool phase2 = false;
+
+  auto emit = [&](uint64_t off, uint32_t size, uint32_t type) {
+    if (type != RTX_TX_BVH_PRIMITIVE_LEAF_DESCRIPTOR) fetch_bytes += size;
+    if (REC) {
+      uint64_t idx = rec_base + n_rec;
+      if (idx < p.record_capacity) {
+        uint32_t tl = p.owner ? p.owner[off >> 6] : RTX_NO_TREELET;
+        uint4 v;
+        v.x = (uint32_t)off;
+        v.y = (uint32_t)(off >> 32);
+        v.z = tl;
+        v.w = size | (type << 16);
+        reinterpret_cast<uint4*>(p.records)[idx] = v;
+      } else {
+        err |= kErrRecords;
+      }
+    }
+    n_rec++;
+  };
+
+  auto push_entry = [&](uint64_t e, bool to_a) {
+    bool ok;
+    if (MODE == 0 || to_a)
+      ok = st_push<K>(sa, e, ldsA, gA, p.spill_cap);
+    else
+      ok = st_push<K>(sb, e, ldsB, gB, p.spill_cap);
+    if (!ok) err |= kErrStack;
+  };
+
+  // Treelet order: does `slot` belong to the current ("first") treelet?
+  auto goes_current = [&](uint32_t slot) -> bool {
+    if (MODE == 0) return true;
+    if (phase2) return false;
+    return p.owner[slot] == p.first_treelet;
+  };
+
+  auto load_instance = [&](uint32_t inst) {
+    const uint64_t ioff = (uint64_t)inst << 6;
+    if (ioff + 128 > bytes) {
+      err |= kErrRange;
+      return;
+    }
+    const uint4* q = reinterpret_cast<const uint4*>(A + ioff + 16);
+    uint4 a = q[0], b = q[1], c = q[2];
+    float m[12] = {__uint_as_float(a.x), __uint_as_float(a.y), __uint_as_float(a.z), __uint_as_float(a.w),
+                   __uint_as_float(b.x), __uint_as_float(b.y), __uint_as_float(b.z), __uint_as_float(b.w),
+                   __uint_as_float(c.x), __uint_as_float(c.y), __uint_as_float(c.z), __uint_as_float(c.w)};
+    cur = transform_ray(m, ro, rd, rtmin, rtmax);
+    cur_inst = inst;
+  };
+
+  while (true) {
+    // ---- refill idle lanes from the global ray counter -------------------
+    const uint64_t idle = __ballot(!active);
+    const uint32_t nidle = (uint32_t)__popcll(idle);
+    if (!exhausted && nidle > 0 && (nidle == 64u || nidle >= p.refill)) {
+      const uint32_t leader = (uint32_t)__ffsll((long long)idle) - 1u;
+      uint32_t base = 0;
+      if (lane == leader) base = atomicAdd(p.ray_counter, nidle);
+      base = __shfl(base, (int)leader, 64);
+      if ((uint64_t)base + nidle >= p.n) exhausted = true;
+      if (!active) {
+        const uint32_t r = base + lanes_below(idle);
+        if (r < p.n) {
+          ray_id = r;
+          active = true;
+          const uint4* rq = reinterpret_cast<const uint4*>(p.rays + r);
+          uint4 r0 = rq[0], r1 = rq[1], r2 = rq[2];
+          ro = mk3(__uint_as_float(r0.x), __uint_as_float(r0.y), __uint_as_float(r0.z));
+          rtmin = __uint_as_float(r0.w);
+          rd = mk3(__uint_as_float(r1.x), __uint_as_float(r1.y), __uint_as_float(r1.z));
+          rtmax = __uint_as_float(r1.w);
+          rflags = r2.x;
+          ridir = calc_idir(rd);
+          min_thit = rtmax;
+          min_thit_obj = 0.f;
+          closest_leaf = kNoSlot;
+          closest_inst = kNoSlot;
+          cur_inst = kNoSlot;
+          n_rec = n_nodes = fetch_bytes = n_any = steps = 0;
+          max_level = 1;  // tree_level_map[topRootAddr] = 1
+          rec_base = REC ? p.record_offsets[r] : 0;
+          sa = Stack{0, 0, 0};
+          sb = Stack{0, 0, 0};
+          phase2 = false;
+          err = 0;
+          // TLAS header record and bounds test (vulkan_ray_tracing.cc:2444-2484)
+          const uint64_t t = p.tlas;
+          if (t + 64 > bytes) {
+            err |= kErrRange;
+          } else {
+            emit(t, 64, RTX_TX_BVH_STRUCTURE);
+            const uint4* hq = reinterpret_cast<const uint4*>(A + t);
+            uint4 h0 = hq[0], h1 = hq[1];
+            const uint64_t root = t + ((uint64_t)h0.x | ((uint64_t)h0.y << 32));
+            F3 lo = mk3(__uint_as_float(h0.z), __uint_as_float(h0.w), __uint_as_float(h1.x));
+            F3 hi = mk3(__uint_as_float(h1.y), __uint_as_float(h1.z), __uint_as_float(h1.w));
+            float th;
+            if (ray_box(lo, hi, ridir, ro, rtmin, rtmax, th)) {
+              if ((root & 63) || root + 64 > bytes) {
+                err |= kErrRange;
+              } else {
+                const uint32_t rs = (uint32_t)(root >> 6);
+                push_entry(make_entry(rs, kFlTop, 1, 0), goes_current(rs));
+              }
+            }
+          }
+        }
+      }
+    }
+    if (__ballot(active) == 0) {
+      if (exhausted) break;
+      continue;
+    }
+    if (!active) continue;
+
+    // ---- one traversal step ------------------------------------------------
+    bool done;
+    if (MODE == 0) {
+      done = st_empty(sa);
+    } else {
+      done = st_empty(sa) && st_empty(sb);
+    }
+    if (!done && err == 0) {
+      uint64_t e;
+      if (MODE == 0) {
+        e = st_pop<K>(sa, ldsA, gA);
+      } else if (!st_empty(sa)) {
+        e = st_pop<K>(sa, ldsA, gA);
+      } else {
+        phase2 = true;  // current_treelet_root := host address of the moved node (:1748-1754)
+        e = st_pop<K>(sb, ldsB, gB);
+      }
+      const uint32_t slot = e_slot(e);
+      const uint32_t flags = e_flags(e);
+      const uint32_t level = e_level(e);
+      const uint64_t off = (uint64_t)slot << 6;
+      if (++steps > kMaxStepsPerRay) err |= kErrSteps;
+      if (MODE == 1 && !(flags & kFlTop) && err == 0 && e_inst(e) != cur_inst) load_instance(e_inst(e));
+
+      if (err != 0) {
+        // fall through to completion
+      } else if (!(flags & kFlLeaf)) {
+        // ---- internal node (top :2499-2599 / bottom :2686-2786) ----------
+        if (off + 64 > bytes) {
+          err |= kErrRange;
+        } else {
+          const Words16 nd = load64B(A + off);
+          emit(off, 64, RTX_TX_BVH_INTERNAL_NODE);
+          n_nodes++;
+          const bool top = (flags & kFlTop) != 0;
+          const int ex = (int)(int8_t)byte_of(nd, 18);
+          const int ey = (int)(int8_t)byte_of(nd, 19);
+          const int ez = (int)(int8_t)byte_of(nd, 20);
+          F3 o, idir;
+          float tmin, tmax, cull;
+          if (top) {
+            o = ro;
+            idir = ridir;
+            tmin = rtmin;
+            tmax = rtmax;
+            cull = min_thit;
+          } else {
+            o = cur.o;
+            idir = cur.idir;
+            tmin = cur.tmin;
+            tmax = cur.tmax;
+            cull = min_thit * cur.tmult;
+          }
+          uint64_t child = off + (uint64_t)((int64_t)(int32_t)nd.w[3] * 64);
+          uint64_t coff[6];
+          uint32_t ctype[6];
+          uint32_t hitmask = 0;
+#pragma unroll
+          for (int i = 0; i < 6; ++i) {
+            const uint32_t info = byte_of(nd, 22 + i) & 0x3fu;
+            const uint32_t size = info & 3u;
+            ctype[i] = info >> 2;
+            coff[i] = child;
+            child += (uint64_t)size * 64;
+            if (size > 0) {
+              F3 lo, hi;
+              child_box(nd, i, ex, ey, ez, lo, hi);
+              float th;
+              bool h = ray_box(lo, hi, idir, o, tmin, tmax, th);
+              if (h && th >= cull) h = false;
+              if (h) hitmask |= 1u << i;
+            }
+          }
+          if (hitmask) {
+            const uint32_t lvl = level + 1;
+            if (lvl > max_level) max_level = lvl;
+            const uint32_t inst = top ? 0u : e_inst(e);
+            const uint32_t base_flags = top ? kFlTop : 0u;
+            int deferred = -1;
+#pragma unroll
+            for (int i = 0; i < 6; ++i) {
+              if (!(hitmask & (1u << i))) continue;
+              const uint64_t c = coff[i];
+              const bool leaf = ctype[i] != 0;
+              if (top && leaf && ctype[i] != 1u) err |= kErrType;
+              if ((c & 63) || c + 64 > bytes) {
+                err |= kErrRange;
+                continue;
+              }
+              const uint32_t cs = (uint32_t)(c >> 6);
+              if (MODE == 0 && !leaf && deferred < 0) {
+                deferred = i;
+                continue;
+              }
+              push_entry(make_entry(cs, base_flags | (leaf ? kFlLeaf : 0u), lvl, inst), goes_current(cs));
+            }
+            if (MODE == 0 && deferred >= 0) {
+              const uint32_t cs = (uint32_t)(coff[deferred] >> 6);
+              push_entry(make_entry(cs, base_flags, lvl, inst), true);
+            }
+          }
+        }
+      } else if (flags & kFlTop) {
+        // ---- instance leaf (:2602-2668 / :1872-1955) ----------------------
+        if (off + 128 > bytes) {
+          err |= kErrRange;
+        } else {
+          const uint4* q = reinterpret_cast<const uint4*>(A + off + 16);
+          uint4 a = q[0], b = q[1], c = q[2], d = q[3];
+          float m[12] = {__uint_as_float(a.x), __uint_as_float(a.y), __uint_as_float(a.z), __uint_as_float(a.w),
+                         __uint_as_float(b.x), __uint_as_float(b.y), __uint_as_float(b.z), __uint_as_float(b.w),
+                         __uint_as_float(c.x), __uint_as_float(c.y), __uint_as_float(c.z), __uint_as_float(c.w)};
+          const uint64_t bvh = (uint64_t)d.x | ((uint64_t)d.y << 32);
+          emit(off, 128, RTX_TX_BVH_INSTANCE_LEAF);
+          n_nodes++;
+          const uint64_t blas = off + bvh;
+          if ((blas & 63) || blas + 64 > bytes) {
+            err |= kErrRange;
+          } else {
+            const uint2 ro2 = *reinterpret_cast<const uint2*>(A + blas);
+            emit(blas, 64, RTX_TX_BVH_STRUCTURE);
+            cur = transform_ray(m, ro, rd, rtmin, rtmax);
+            cur_inst = slot;
+            if (slot >= (1u << 22)) err |= kErrInstSlot;
+            const uint64_t root = blas + ((uint64_t)ro2.x | ((uint64_t)ro2.y << 32));
+            if ((root & 63) || root + 64 > bytes) {
+              err |= kErrRange;
+            } else {
+              const uint32_t rs = (uint32_t)(root >> 6);
+              if (level > max_level) max_level = level;
+              push_entry(make_entry(rs, 0u, level, slot), goes_current(rs));
+            }
+          }
+        }
+      } else {
+        // ---- primitive leaf (:2789-2985 / :2073-2203) ---------------------
+        if (off + 64 > bytes) {
+          err |= kErrRange;
+        } else {
+          emit(off, 8, RTX_TX_BVH_PRIMITIVE_LEAF_DESCRIPTOR);
+          const Words16 q = load64B(A + off);
+          if (((q.w[1] >> 29) & 1u) == 0u) {
+            if ((q.w[3] & 0x1ffffu) != 0u) err |= kErrType;
+            const F3 v0 = mk3(f_of(q, 4), f_of(q, 5), f_of(q, 6));
+            const F3 v1 = mk3(f_of(q, 7), f_of(q, 8), f_of(q, 9));
+            const F3 v2 = mk3(f_of(q, 10), f_of(q, 11), f_of(q, 12));
+            float th = 0.f;
+            const bool h = mt_test(v0, v1, v2, cur.o, cur.d, th);
+            const float wt = th / cur.tmult;
+            bool accept;
+            if (MODE == 0) {
+              accept = h && rtmin <= wt && wt <= rtmax;
+              if (accept && (rflags & RTX_RAY_FLAG_OPAQUE) && wt < min_thit) min_thit = th / cur.tmult;
+            } else {
+              accept = h && rtmin <= wt && wt <= rtmax && wt < min_thit;
+              if (accept) min_thit = th / cur.tmult;
+            }
+            if (accept) {
+              min_thit_obj = th;
+              closest_leaf = slot;
+              closest_inst = cur_inst;
+              cobj_o = cur.o;
+              cobj_d = cur.d;
+              emit(off, 64, RTX_TX_BVH_QUAD_LEAF_HIT);
+              n_nodes++;
+              if (MODE == 0 && !(rflags & RTX_RAY_FLAG_OPAQUE)) n_any++;
+              if (rflags & RTX_RAY_FLAG_TERMINATE_ON_FIRST_HIT) {
+                sa = Stack{0, 0, 0};
+                sb = Stack{0, 0, 0};
+              }
+            } else {
+              emit(off, 64, RTX_TX_BVH_QUAD_LEAF);
+              n_nodes++;
+            }
+          } else {
+            emit(off, 64, RTX_TX_BVH_PROCEDURAL_LEAF);
+            n_nodes++;
+          }
+        }
+      }
+      if (err == 0) continue;
+    }
+
+    // ---- ray complete: write the hit record (:2990-3038) --------------------
+    {
+      uint4 o0 = make_uint4(0, 0, 0, 0), o1 = make_uint4(0, 0, 0, 0), o2 = make_uint4(0, 0, 0, 0);
+      if (err == 0 && min_thit < rtmax && closest_leaf != kNoSlot) {
+        const Words16 q = load64B(A + ((uint64_t)closest_leaf << 6));
+        const uint32_t inst_id = *reinterpret_cast<const uint32_t*>(A + ((uint64_t)closest_inst << 6) + 72);
+        const F3 v0 = mk3(f_of(q, 4), f_of(q, 5), f_of(q, 6));
+        const F3 v1 = mk3(f_of(q, 7), f_of(q, 8), f_of(q, 9));
+        const F3 v2 = mk3(f_of(q, 10), f_of(q, 11), f_of(q, 12));
+        const F3 ip = add3(ro, mk3(rd.x * min_thit, rd.y * min_thit, rd.z * min_thit));
+        const F3 op = add3(cobj_o, mk3(cobj_d.x * min_thit_obj, cobj_d.y * min_thit_obj, cobj_d.z * min_thit_obj));
+        const F3 bc = barycentric(op, v0, v1, v2);
+        o0 = make_uint4(1u, q.w[2], q.w[1] & 0x0fffffffu, inst_id);
+        o1 = make_uint4(__float_as_uint(min_thit), __float_as_uint(ip.x), __float_as_uint(ip.y),
+                        __float_as_uint(ip.z));
+        o2 = make_uint4(__float_as_uint(bc.x), __float_as_uint(bc.y), __float_as_uint(bc.z), n_rec);
+      } else {
+        o2.w = n_rec;
+      }
+      uint4* hq = reinterpret_cast<uint4*>(p.hits + ray_id);
+      hq[0] = o0;
+      hq[1] = o1;
+      hq[2] = o2;
+      hq[3] = make_uint4(n_nodes, fetch_bytes, n_any, max_level);
+      if (err) atomicOr(p.error, err);
+      active = false;
+    }
+  }
+}
+
+// ---- record offsets: exclusive scan of hits[i].n_records --------------------
+constexpr uint32_t kScanBlock = 256;
+constexpr uint32_t kScanItems = 4;
+constexpr uint32_t kScanTile = kScanBlock * kScanItems;
+
+__device__ __forceinline__ uint64_t block_exclusive_scan(uint64_t v, uint64_t* smem, uint64_t& total) {
+  const uint32_t tid = threadIdx.x, lane = tid & 63u, wave = tid >> 6;
+  uint64_t x = v;
+#pragma unroll
+  for (int d = 1; d < 64; d <<= 1) {
+    uint64_t y = __shfl_up(x, d, 64);
+    if (lane >= (uint32_t)d) x += y;
+  }
+  if (lane == 63) smem[wave] = x;
+  __syncthreads();
+  uint64_t wsum = 0;
+  for (uint32_t w = 0; w < wave; ++w) wsum += smem[w];
+  total = smem[0] + smem[1] + smem[2] + smem[3];
+  __syncthreads();
+  return wsum + x - v;
+}
+
+__global__ __launch_bounds__(kScanBlock) void scan_tiles(const rtx_hit* hits, uint32_t n, uint64_t* out,
+                                                         uint64_t* tile_sums) {
+  __shared__ uint64_t smem[4];
+  const uint64_t base = (uint64_t)blockIdx.x * kScanTile + threadIdx.x * kScanItems;
+  uint64_t v[kScanItems];
+  uint64_t s = 0;
+#pragma unroll
+  for (uint32_t k = 0; k < kScanItems; ++k) {
+    v[k] = base + k < n ? hits[base + k].n_records : 0;
+    s += v[k];
+  }
+  uint64_t total;
+  uint64_t ex = block_exclusive_scan(s, smem, total);
+#pragma unroll
+  for (uint32_t k = 0; k < kScanItems; ++k) {
+    if (base + k < n) out[base + k] = ex;
+    ex += v[k];
+  }
+  if (threadIdx.x == 0) tile_sums[blockIdx.x] = total;
+}
+
+__global__ __launch_bounds__(kScanBlock) void scan_sums(uint64_t* tile_sums, uint32_t n_tiles, uint64_t* out,
+                                                        uint32_t n) {
+  __shared__ uint64_t smem[4];
+  uint64_t carry = 0;
+  for (uint32_t b = 0; b < n_tiles; b += kScanBlock) {
+    const uint32_t i = b + threadIdx.x;
+    uint64_t v = i < n_tiles ? tile_sums[i] : 0;
+    uint64_t total;
+    uint64_t ex = block_exclusive_scan(v, smem, total);
+    if (i < n_tiles) tile_sums[i] = carry + ex;
+    carry += total;
+  }
+  if (threadIdx.x == 0) out[n] = carry;
+}
+
+__global__ __launch_bounds__(kScanBlock) void scan_add(uint64_t* out, uint32_t n, const uint64_t* tile_sums) {
+  const uint64_t base = (uint64_t)blockIdx.x * kScanTile + threadIdx.x * kScanItems;
+  const uint64_t add = tile_sums[blockIdx.x];
+#pragma unroll
+  for (uint32_t k = 0; k < kScanItems; ++k)
+    if (base + k < n) out[base + k] += add;
+}
+
+template <int MODE, bool REC, int K>
+hipError_t launch_one(const TraceParams& p, uint32_t grid, hipStream_t s) {
+  hipLaunchKernelGGL((trace_kernel<MODE, REC, K>), dim3(grid), dim3(kBlock), 0, s, p);
+  return hipGetLastError();
+}
+
+template <int MODE, bool REC>
+hipError_t launch_k(const TraceParams& p, uint32_t lds_stack, uint32_t grid, hipStream_t s) {
+  switch (lds_stack) {
+    case 4:
+      return launch_one<MODE, REC, 4>(p, grid, s);
+    case 8:
+      return launch_one<MODE, REC, 8>(p, grid, s);
+    case 16:
+      return launch_one<MODE, REC, 16>(p, grid, s);
+    default:
+      return hipErrorInvalidValue;
+  }
+}
+
+}  // namespace
+
+hipError_t launch_trace(const TraceParams& p, uint32_t mode, bool records, uint32_t lds_stack, uint32_t grid,
+                        hipStream_t s) {
+  if (mode == RTX_MODE_PLAIN)
+    return records ? launch_k<0, true>(p, lds_stack, grid, s) : launch_k<0, false>(p, lds_stack, grid, s);
+  return records ? launch_k<1, true>(p, lds_stack, grid, s) : launch_k<1, false>(p, lds_stack, grid, s);
+}
+
+hipError_t launch_scan_records(const rtx_hit* hits, uint32_t n, uint64_t* offsets, uint64_t* tile_sums,
+                               hipStream_t s) {
+  const uint32_t tiles = (n + kScanTile - 1) / kScanTile;
+  if (tiles > 0) {
+    hipLaunchKernelGGL(scan_tiles, dim3(tiles), dim3(kScanBlock), 0, s, hits, n, offsets, tile_sums);
+  }
+  hipLaunchKernelGGL(scan_sums, dim3(1), dim3(kScanBlock), 0, s, tile_sums, tiles, offsets, n);
+  if (tiles > 0) {
+    hipLaunchKernelGGL(scan_add, dim3(tiles), dim3(kScanBlock), 0, s, offsets, n, tile_sums);
+  }
+  return hipGetLastError();
+}
+
+}  // namespace rtxk

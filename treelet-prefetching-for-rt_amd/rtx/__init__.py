@@ -1,0 +1,430 @@
+"""Python binding of librtx.so (include/rtx.h, include/rtx_scene.h).
+
+Plumbing for tests and benchmarks: numpy arrays in host memory, device
+buffers allocated through the library itself (rtx_malloc), ctypes calls.
+The library must be built (``python treelet-prefetching-for-rt_amd/rtx/_build.py``
+or ``__graft_entry__.build()``); importing this module without it raises.
+
+Names follow the reference path they replace:
+  Context.trace(..., mode=PLAIN)    VulkanRayTracing::traceRay              (vulkan_ray_tracing.cc:2309)
+  Context.trace(..., mode=TREELET)  VulkanRayTracing::traceRayWithTreelets  (vulkan_ray_tracing.cc:1522)
+  Context.build_treelets(budget)    VulkanRayTracing::createTreelets        (vulkan_ray_tracing.cc:823)
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "librtx.so")
+
+if not os.path.exists(LIB_PATH):
+    raise ImportError(f"{LIB_PATH} is not built; run __graft_entry__.build() first")
+_lib = ctypes.CDLL(LIB_PATH)
+
+# ---- constants (include/rtx.h) ----------------------------------------------
+OK = 0
+STATUS_NAMES = {
+    0: "RTX_OK", 1: "RTX_ERR_INVALID_ARGUMENT", 2: "RTX_ERR_BAD_NODE_TYPE", 3: "RTX_ERR_OFFSET_OUT_OF_ARENA",
+    4: "RTX_ERR_NODE_LARGER_THAN_BUDGET", 5: "RTX_ERR_TRACE_OVERFLOW", 6: "RTX_ERR_STACK_OVERFLOW",
+    7: "RTX_ERR_HIP", 8: "RTX_ERR_NO_DEVICE", 9: "RTX_ERR_NO_SCENE", 10: "RTX_ERR_NO_PARTITION",
+    11: "RTX_ERR_UNSUPPORTED", 12: "RTX_ERR_OUT_OF_MEMORY", 13: "RTX_ERR_TIMEOUT",
+}
+MODE_PLAIN, MODE_TREELET = 0, 1
+PREFETCH_NONE, PREFETCH_ALWAYS, PREFETCH_POPULARITY, PREFETCH_PARTIAL, PREFETCH_LEARNED = 0, 1, 2, 3, 4
+RAY_FLAG_OPAQUE, RAY_FLAG_TERMINATE_ON_FIRST_HIT, RAY_FLAG_SKIP_CLOSEST_HIT_SHADER = 0x1, 0x4, 0x8
+TX_BVH_STRUCTURE, TX_BVH_INTERNAL_NODE, TX_BVH_INSTANCE_LEAF = 0, 1, 2
+TX_BVH_PRIMITIVE_LEAF_DESCRIPTOR, TX_BVH_QUAD_LEAF, TX_BVH_QUAD_LEAF_HIT, TX_BVH_PROCEDURAL_LEAF = 3, 4, 5, 6
+NO_TREELET = 0xFFFFFFFF
+WORKLOAD_SPHERE_AND_QUAD, WORKLOAD_INSTANCED_SPHERES = 0, 1
+
+RAY_DTYPE = np.dtype([("org", "<f4", 3), ("tmin", "<f4"), ("dir", "<f4", 3), ("tmax", "<f4"),
+                      ("flags", "<u4"), ("cull_mask", "<u4"), ("sbt_record_offset", "<u4"),
+                      ("sbt_record_stride", "<u4"), ("miss_index", "<u4"), ("reserved", "<u4", 3)])
+HIT_DTYPE = np.dtype([("hit", "<u4"), ("primitive_index", "<u4"), ("geometry_index", "<u4"),
+                      ("instance_index", "<u4"), ("t", "<f4"), ("p", "<f4", 3), ("bary", "<f4", 3),
+                      ("n_records", "<u4"), ("n_nodes", "<u4"), ("fetch_bytes", "<u4"), ("n_any_hits", "<u4"),
+                      ("max_level", "<u4")])
+RECORD_DTYPE = np.dtype([("offset", "<u8"), ("treelet", "<u4"), ("size", "<u2"), ("type", "u1"),
+                         ("reserved", "u1")])
+assert RAY_DTYPE.itemsize == 64 and HIT_DTYPE.itemsize == 64 and RECORD_DTYPE.itemsize == 16
+
+
+class RtxError(RuntimeError):
+    def __init__(self, status, what=""):
+        self.status = status
+        super().__init__(f"{what}: {STATUS_NAMES.get(status, status)}")
+
+
+class _TraceOptions(ctypes.Structure):
+    _fields_ = [("mode", ctypes.c_uint32), ("prefetch", ctypes.c_uint32), ("threshold", ctypes.c_float),
+                ("emit_records", ctypes.c_uint32), ("waves_per_cu", ctypes.c_uint32),
+                ("lds_stack", ctypes.c_uint32), ("reserved", ctypes.c_uint32 * 2)]
+
+
+class _PrefetchStats(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_uint64) for n in ("node_reads", "lds_hits", "staged_treelets", "staged_bytes",
+                                               "wasted_bytes", "predictions", "prediction_hits", "fallbacks")]
+
+
+class _TreeletSummary(ctypes.Structure):
+    _fields_ = [("n_treelets", ctypes.c_uint32), ("budget_bytes", ctypes.c_uint32),
+                ("n_list_entries", ctypes.c_uint64), ("total_bytes", ctypes.c_uint64),
+                ("max_treelet_bytes", ctypes.c_uint32), ("max_treelet_nodes", ctypes.c_uint32)]
+
+
+class _WorkloadDesc(ctypes.Structure):
+    _fields_ = [("kind", ctypes.c_uint32), ("n_instances", ctypes.c_uint32), ("sphere_slices", ctypes.c_uint32),
+                ("sphere_stacks", ctypes.c_uint32), ("shared_blas", ctypes.c_uint32), ("width", ctypes.c_uint32),
+                ("height", ctypes.c_uint32), ("seed", ctypes.c_uint64), ("box_size", ctypes.c_float),
+                ("scale_min", ctypes.c_float), ("scale_max", ctypes.c_float), ("displacement", ctypes.c_float),
+                ("ray_flags", ctypes.c_uint32), ("reserved", ctypes.c_uint32)]
+
+
+_vp = ctypes.c_void_p
+_u32, _u64, _i32, _f32 = ctypes.c_uint32, ctypes.c_uint64, ctypes.c_int, ctypes.c_float
+_P = ctypes.POINTER
+
+
+def _sig(name, restype, *args):
+    f = getattr(_lib, name)
+    f.restype = restype
+    f.argtypes = list(args)
+    return f
+
+
+_sig("rtx_status_string", ctypes.c_char_p, _i32)
+_sig("rtx_version", ctypes.c_char_p)
+_sig("rtx_device_count", _i32, _P(_i32))
+_sig("rtx_create", _i32, _i32, _P(_vp))
+_sig("rtx_destroy", _i32, _vp)
+_sig("rtx_synchronize", _i32, _vp)
+_sig("rtx_malloc", _i32, _vp, ctypes.c_size_t, _P(_vp))
+_sig("rtx_free", _i32, _vp, _vp)
+_sig("rtx_memcpy_h2d", _i32, _vp, _vp, _vp, ctypes.c_size_t)
+_sig("rtx_memcpy_d2h", _i32, _vp, _vp, _vp, ctypes.c_size_t)
+_sig("rtx_memset", _i32, _vp, _vp, _i32, ctypes.c_size_t)
+_sig("rtx_timer_start", _i32, _vp)
+_sig("rtx_timer_stop", _i32, _vp, _P(_f32))
+_sig("rtx_device_info", _i32, _vp, _P(_i32), _P(_i32), ctypes.c_char_p, ctypes.c_size_t)
+_sig("rtx_set_scene", _i32, _vp, _vp, ctypes.c_size_t, _u64)
+_sig("rtx_set_scene_device", _i32, _vp, _vp, ctypes.c_size_t, _u64)
+_sig("rtx_validate_scene", _i32, _vp, _P(_u64))
+_sig("rtx_build_treelets", _i32, _vp, _u32, _P(_TreeletSummary))
+_sig("rtx_get_treelets", _i32, _vp, _vp, _vp, _vp, _vp)
+_sig("rtx_set_treelets", _i32, _vp, _u32, _u32, _vp, _vp, _vp, _vp)
+_sig("rtx_trace", _i32, _vp, _vp, _u32, _P(_TraceOptions), _vp, _vp, _u64, _vp)
+_sig("rtx_get_prefetch_stats", _i32, _vp, _P(_PrefetchStats))
+_sig("rtx_builder_create", _i32, _P(_vp))
+_sig("rtx_builder_destroy", _i32, _vp)
+_sig("rtx_builder_add_blas", _i32, _vp, _vp, _u32, _u32, _P(_u32))
+_sig("rtx_builder_add_instance", _i32, _vp, _u32, _vp, _u32, _u32)
+_sig("rtx_builder_finish", _i32, _vp, _u32, _P(_u64), _P(_u64))
+_sig("rtx_builder_copy_arena", _i32, _vp, _vp, ctypes.c_size_t)
+_sig("rtx_builder_arena_info", _i32, _vp, _P(_u64), _P(_u64))
+_sig("rtx_builder_counts", _i32, _vp, _P(_u64), _P(_u64), _P(_u64), _P(_u64))
+_sig("rtx_builder_blas_offset", _i32, _vp, _u32, _P(_u64))
+_sig("rtx_mesh_displaced_sphere", _i32, _u32, _u32, _f32, _f32, _u64, _vp)
+_sig("rtx_workload_scene", _i32, _P(_WorkloadDesc), _u32, _P(_vp))
+_sig("rtx_workload_primary_rays", _i32, _P(_WorkloadDesc), _vp)
+_sig("rtx_workload_secondary_rays", _i32, _vp, _vp, _vp, _u32, _u32, _u64, _vp, _vp, _vp, _P(_u32))
+
+#: every symbol declared in include/rtx.h and include/rtx_scene.h
+EXPORTED_SYMBOLS = [
+    "rtx_status_string", "rtx_version", "rtx_device_count", "rtx_create", "rtx_destroy", "rtx_synchronize",
+    "rtx_malloc", "rtx_free", "rtx_memcpy_h2d", "rtx_memcpy_d2h", "rtx_memset", "rtx_timer_start",
+    "rtx_timer_stop", "rtx_device_info", "rtx_set_scene", "rtx_set_scene_device", "rtx_validate_scene",
+    "rtx_build_treelets", "rtx_get_treelets", "rtx_set_treelets", "rtx_trace", "rtx_get_prefetch_stats",
+    "rtx_builder_create", "rtx_builder_destroy", "rtx_builder_add_blas", "rtx_builder_add_instance",
+    "rtx_builder_finish", "rtx_builder_copy_arena", "rtx_builder_arena_info", "rtx_builder_counts", "rtx_builder_blas_offset",
+    "rtx_mesh_displaced_sphere", "rtx_workload_scene", "rtx_workload_primary_rays", "rtx_workload_secondary_rays",
+]
+
+
+def _check(st, what):
+    if st != OK:
+        raise RtxError(st, what)
+
+
+def _ptr(a):
+    return ctypes.c_void_p(a.ctypes.data) if a is not None else None
+
+
+def version():
+    return _lib.rtx_version().decode()
+
+
+def device_count():
+    n = ctypes.c_int(0)
+    _check(_lib.rtx_device_count(ctypes.byref(n)), "rtx_device_count")
+    return n.value
+
+
+# ---- scenes ------------------------------------------------------------------
+class Scene:
+    """A finished acceleration-structure arena plus the builder that made it."""
+
+    def __init__(self, handle):
+        self._h = ctypes.c_void_p(handle)
+        nbytes, tlas = ctypes.c_uint64(0), ctypes.c_uint64(0)
+        tri, blas, inst, internal = (ctypes.c_uint64(0) for _ in range(4))
+        _check(_lib.rtx_builder_counts(self._h, ctypes.byref(tri), ctypes.byref(blas), ctypes.byref(inst),
+                                       ctypes.byref(internal)), "rtx_builder_counts")
+        self.n_triangles, self.n_blas, self.n_instances, self.n_internal = tri.value, blas.value, inst.value, internal.value
+        self._arena = None
+        self.tlas_offset = 0
+
+    @property
+    def arena(self):
+        return self._arena
+
+    def _capture_finished(self):
+        nb, tl = ctypes.c_uint64(0), ctypes.c_uint64(0)
+        _check(_lib.rtx_builder_arena_info(self._h, ctypes.byref(nb), ctypes.byref(tl)), "rtx_builder_arena_info")
+        buf = np.empty(nb.value, dtype=np.uint8)
+        _check(_lib.rtx_builder_copy_arena(self._h, _ptr(buf), nb.value), "rtx_builder_copy_arena")
+        self._arena = buf
+        self.tlas_offset = tl.value
+
+    def blas_offset(self, blas_id):
+        o = ctypes.c_uint64(0)
+        _check(_lib.rtx_builder_blas_offset(self._h, blas_id, ctypes.byref(o)), "rtx_builder_blas_offset")
+        return o.value
+
+    def secondary_rays(self, rays, hits, kind=0, seed=1, light=(0.0, 0.0, 0.0)):
+        rays = np.ascontiguousarray(rays, dtype=RAY_DTYPE)
+        hits = np.ascontiguousarray(hits, dtype=HIT_DTYPE)
+        out = np.zeros(len(rays), dtype=RAY_DTYPE)
+        src = np.zeros(len(rays), dtype=np.uint32)
+        n_out = ctypes.c_uint32(0)
+        lp = np.asarray(light, dtype=np.float32)
+        _check(_lib.rtx_workload_secondary_rays(self._h, _ptr(rays), _ptr(hits), len(rays), kind, seed, _ptr(lp),
+                                                _ptr(out), _ptr(src), ctypes.byref(n_out)),
+               "rtx_workload_secondary_rays")
+        return out[:n_out.value].copy(), src[:n_out.value].copy()
+
+    def close(self):
+        if self._h:
+            _lib.rtx_builder_destroy(self._h)
+            self._h = ctypes.c_void_p(None)
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class SceneBuilder:
+    """rtx_builder: add triangle BLASes and instances, then finish()."""
+
+    def __init__(self):
+        h = ctypes.c_void_p()
+        _check(_lib.rtx_builder_create(ctypes.byref(h)), "rtx_builder_create")
+        self._h = h
+
+    def add_blas(self, tris, geometry_index=0):
+        t = np.ascontiguousarray(tris, dtype=np.float32).reshape(-1, 9)
+        bid = ctypes.c_uint32(0)
+        _check(_lib.rtx_builder_add_blas(self._h, _ptr(t), len(t), geometry_index, ctypes.byref(bid)),
+               "rtx_builder_add_blas")
+        return bid.value
+
+    def add_instance(self, blas_id, object_to_world=None, instance_id=0, hit_group=0):
+        m = np.eye(3, 4, dtype=np.float32) if object_to_world is None else np.asarray(object_to_world, np.float32)
+        m = np.ascontiguousarray(m.reshape(12))
+        _check(_lib.rtx_builder_add_instance(self._h, blas_id, _ptr(m), instance_id, hit_group),
+               "rtx_builder_add_instance")
+
+    def finish(self, threads=0):
+        nb, tl = ctypes.c_uint64(0), ctypes.c_uint64(0)
+        _check(_lib.rtx_builder_finish(self._h, threads, ctypes.byref(nb), ctypes.byref(tl)), "rtx_builder_finish")
+        s = Scene(self._h.value)
+        s._capture_finished()
+        self._h = ctypes.c_void_p(None)
+        return s
+
+
+def displaced_sphere(slices, stacks, radius=1.0, disp=0.05, seed=1):
+    out = np.empty((slices * stacks * 2, 9), dtype=np.float32)
+    _check(_lib.rtx_mesh_displaced_sphere(slices, stacks, radius, disp, seed, _ptr(out)), "rtx_mesh_displaced_sphere")
+    return out
+
+
+def workload_desc(kind=WORKLOAD_SPHERE_AND_QUAD, n_instances=2, slices=128, stacks=64, shared_blas=0, width=128,
+                  height=128, seed=1, box_size=100.0, scale_min=0.5, scale_max=2.0, displacement=0.05,
+                  ray_flags=RAY_FLAG_OPAQUE):
+    return _WorkloadDesc(kind, n_instances, slices, stacks, shared_blas, width, height, seed, box_size, scale_min,
+                         scale_max, displacement, ray_flags, 0)
+
+
+def workload_scene(desc, threads=0):
+    h = ctypes.c_void_p()
+    _check(_lib.rtx_workload_scene(ctypes.byref(desc), threads, ctypes.byref(h)), "rtx_workload_scene")
+    s = Scene(h.value)
+    s._capture_finished()
+    return s
+
+
+def primary_rays(desc):
+    rays = np.zeros(desc.width * desc.height, dtype=RAY_DTYPE)
+    _check(_lib.rtx_workload_primary_rays(ctypes.byref(desc), _ptr(rays)), "rtx_workload_primary_rays")
+    return rays
+
+
+# ---- device context -------------------------------------------------------------
+class DeviceBuffer:
+    def __init__(self, ctx, nbytes):
+        self.ctx = ctx
+        self.nbytes = max(int(nbytes), 1)
+        p = ctypes.c_void_p()
+        _check(_lib.rtx_malloc(ctx._h, self.nbytes, ctypes.byref(p)), "rtx_malloc")
+        self.ptr = p
+
+    def upload(self, arr):
+        a = np.ascontiguousarray(arr)
+        assert a.nbytes <= self.nbytes
+        _check(_lib.rtx_memcpy_h2d(self.ctx._h, self.ptr, _ptr(a), a.nbytes), "rtx_memcpy_h2d")
+
+    def download(self, dtype, count):
+        out = np.empty(count, dtype=dtype)
+        if out.nbytes:
+            _check(_lib.rtx_memcpy_d2h(self.ctx._h, _ptr(out), self.ptr, out.nbytes), "rtx_memcpy_d2h")
+        return out
+
+    def free(self):
+        if self.ptr:
+            _lib.rtx_free(self.ctx._h, self.ptr)
+            self.ptr = ctypes.c_void_p(None)
+
+    def __del__(self):
+        try:
+            if self.ctx._h:
+                self.free()
+        except Exception:
+            pass
+
+
+class Context:
+    """One rtx_ctx on a device (VulkanRayTracing's static state, per context)."""
+
+    def __init__(self, device=0):
+        h = ctypes.c_void_p()
+        _check(_lib.rtx_create(device, ctypes.byref(h)), "rtx_create")
+        self._h = h
+        self.summary = None
+
+    def close(self):
+        if self._h:
+            _lib.rtx_destroy(self._h)
+            self._h = ctypes.c_void_p(None)
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def device_info(self):
+        n, lds = ctypes.c_int(0), ctypes.c_int(0)
+        name = ctypes.create_string_buffer(128)
+        _check(_lib.rtx_device_info(self._h, ctypes.byref(n), ctypes.byref(lds), name, 128), "rtx_device_info")
+        return {"n_cus": n.value, "lds_per_cu": lds.value, "name": name.value.decode()}
+
+    def buffer(self, nbytes):
+        return DeviceBuffer(self, nbytes)
+
+    def to_device(self, arr):
+        b = DeviceBuffer(self, np.asarray(arr).nbytes)
+        b.upload(arr)
+        return b
+
+    def set_scene(self, arena, tlas_offset=0):
+        a = np.ascontiguousarray(arena, dtype=np.uint8)
+        _check(_lib.rtx_set_scene(self._h, _ptr(a), a.nbytes, tlas_offset), "rtx_set_scene")
+        self.arena_bytes = a.nbytes
+        self.summary = None
+
+    def validate_scene(self):
+        n = ctypes.c_uint64(0)
+        _check(_lib.rtx_validate_scene(self._h, ctypes.byref(n)), "rtx_validate_scene")
+        return n.value
+
+    def build_treelets(self, budget_bytes):
+        s = _TreeletSummary()
+        _check(_lib.rtx_build_treelets(self._h, budget_bytes, ctypes.byref(s)), "rtx_build_treelets")
+        self.summary = {f: getattr(s, f) for f, _ in _TreeletSummary._fields_}
+        return self.summary
+
+    def get_treelets(self):
+        if self.summary is None:
+            raise RtxError(10, "get_treelets")
+        n = self.summary["n_treelets"]
+        roots = np.empty(n, np.uint64)
+        lo = np.empty(n + 1, np.uint64)
+        ln = np.empty(max(self.summary["n_list_entries"], 1), np.uint64)
+        owner = np.empty(self.arena_bytes // 64, np.uint32)
+        _check(_lib.rtx_get_treelets(self._h, _ptr(roots), _ptr(lo), _ptr(ln), _ptr(owner)), "rtx_get_treelets")
+        return {"roots": roots, "list_offsets": lo, "list_nodes": ln[: self.summary["n_list_entries"]],
+                "owner": owner}
+
+    def set_treelets(self, budget, roots, list_offsets, list_nodes, owner):
+        r = np.ascontiguousarray(roots, np.uint64)
+        lo = np.ascontiguousarray(list_offsets, np.uint64)
+        ln = np.ascontiguousarray(list_nodes, np.uint64)
+        ow = np.ascontiguousarray(owner, np.uint32)
+        if len(ln) == 0:
+            ln = np.zeros(1, np.uint64)
+        _check(_lib.rtx_set_treelets(self._h, budget, len(r), _ptr(r), _ptr(lo), _ptr(ln), _ptr(ow)),
+               "rtx_set_treelets")
+        self.summary = {"n_treelets": len(r), "budget_bytes": budget, "n_list_entries": int(lo[-1])}
+
+    def trace_device(self, rays_buf, n, hits_buf, mode=MODE_PLAIN, prefetch=PREFETCH_NONE, threshold=0.5,
+                     records_buf=None, record_capacity=0, offsets_buf=None, waves_per_cu=0, lds_stack=0):
+        """rtx_trace on device buffers; returns the rtx_status (0 = ok)."""
+        o = _TraceOptions(mode, prefetch, threshold, 1 if offsets_buf is not None else 0, waves_per_cu, lds_stack,
+                          (ctypes.c_uint32 * 2)(0, 0))
+        return _lib.rtx_trace(self._h, rays_buf.ptr, n, ctypes.byref(o), hits_buf.ptr,
+                              records_buf.ptr if records_buf is not None else None, record_capacity,
+                              offsets_buf.ptr if offsets_buf is not None else None)
+
+    def trace(self, rays, mode=MODE_PLAIN, prefetch=PREFETCH_NONE, threshold=0.5, records=False, **kw):
+        """Host arrays in and out.  Returns hits, or (hits, offsets, records) when records=True."""
+        rays = np.ascontiguousarray(rays, dtype=RAY_DTYPE)
+        n = len(rays)
+        rb = self.to_device(rays)
+        hb = self.buffer(n * HIT_DTYPE.itemsize)
+        if not records:
+            _check(self.trace_device(rb, n, hb, mode, prefetch, threshold, **kw), "rtx_trace")
+            return hb.download(HIT_DTYPE, n)
+        ob = self.buffer((n + 1) * 8)
+        st = self.trace_device(rb, n, hb, mode, prefetch, threshold, None, 0, ob, **kw)
+        if st not in (OK, 5):
+            raise RtxError(st, "rtx_trace")
+        offsets = ob.download(np.uint64, n + 1)
+        total = int(offsets[-1])
+        recb = self.buffer(max(total, 1) * RECORD_DTYPE.itemsize)
+        _check(self.trace_device(rb, n, hb, mode, prefetch, threshold, recb, total, ob, **kw), "rtx_trace")
+        return hb.download(HIT_DTYPE, n), ob.download(np.uint64, n + 1), recb.download(RECORD_DTYPE, total)
+
+    def prefetch_stats(self):
+        s = _PrefetchStats()
+        _check(_lib.rtx_get_prefetch_stats(self._h, ctypes.byref(s)), "rtx_get_prefetch_stats")
+        return {f: getattr(s, f) for f, _ in _PrefetchStats._fields_}
+
+    def timer_start(self):
+        _check(_lib.rtx_timer_start(self._h), "rtx_timer_start")
+
+    def timer_stop(self):
+        ms = ctypes.c_float(0)
+        _check(_lib.rtx_timer_stop(self._h, ctypes.byref(ms)), "rtx_timer_stop")
+        return ms.value
+
+    def synchronize(self):
+        _check(_lib.rtx_synchronize(self._h), "rtx_synchronize")
