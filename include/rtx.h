@@ -213,6 +213,9 @@ rtx_status rtx_trace(rtx_ctx* ctx, const rtx_ray* rays_dev, uint32_t n, const rt
                      rtx_hit* hits_dev, rtx_record* records_dev, uint64_t record_capacity,
                      uint64_t* record_offsets_dev);
 rtx_status rtx_get_prefetch_stats(rtx_ctx* ctx, rtx_prefetch_stats* out);
+/* Duration of the traversal kernel of the last rtx_trace call (the hit pass),
+ * measured with HIP events recorded on the context's stream right around it. */
+rtx_status rtx_last_kernel_ms(rtx_ctx* ctx, float* ms);
 
 #ifdef __cplusplus
 }

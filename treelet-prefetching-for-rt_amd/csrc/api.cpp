@@ -23,6 +23,8 @@ struct rtx_ctx {
   int device = 0;
   hipStream_t stream = nullptr;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  hipEvent_t evk0 = nullptr, evk1 = nullptr;  // around the last traversal kernel
+  bool kernel_timed = false;
   int n_cus = 0;
   int lds_per_cu = 0;
   char name[128] = {0};
@@ -164,6 +166,7 @@ rtx_status rtx_create(int device, rtx_ctx** out) {
   snprintf(c->name, sizeof(c->name), "%s", prop.gcnArchName);
   if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) return fail(RTX_ERR_HIP);
   if (hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess) return fail(RTX_ERR_HIP);
+  if (hipEventCreate(&c->evk0) != hipSuccess || hipEventCreate(&c->evk1) != hipSuccess) return fail(RTX_ERR_HIP);
   if (hipMalloc(&c->counters, 64 * sizeof(uint32_t)) != hipSuccess) return fail(RTX_ERR_OUT_OF_MEMORY);
   if (hipMalloc(&c->stats, 16 * sizeof(unsigned long long)) != hipSuccess) return fail(RTX_ERR_OUT_OF_MEMORY);
   *out = c;
@@ -182,6 +185,8 @@ rtx_status rtx_destroy(rtx_ctx* c) {
   if (c->scan_tmp) hipFree(c->scan_tmp);
   if (c->ev0) hipEventDestroy(c->ev0);
   if (c->ev1) hipEventDestroy(c->ev1);
+  if (c->evk0) hipEventDestroy(c->evk0);
+  if (c->evk1) hipEventDestroy(c->evk1);
   if (c->stream) hipStreamDestroy(c->stream);
   delete c;
   return RTX_OK;
@@ -394,7 +399,11 @@ rtx_status rtx_trace(rtx_ctx* c, const rtx_ray* rays, uint32_t n, const rtx_trac
 
   HIP_TRY(hipMemsetAsync(c->counters, 0, 2 * sizeof(uint32_t), c->stream));
   HIP_TRY(hipMemsetAsync(c->stats, 0, 16 * sizeof(unsigned long long), c->stream));
+  c->kernel_timed = false;
+  HIP_TRY(hipEventRecord(c->evk0, c->stream));
   HIP_TRY(launch_trace(p, opts->mode, false, lds_stack, grid, c->stream));
+  HIP_TRY(hipEventRecord(c->evk1, c->stream));
+  c->kernel_timed = true;
   if (opts->emit_records) {
     const size_t tiles = (n + 1023) / 1024;
     st = ensure_scan_tmp(c, tiles + 1);
@@ -428,6 +437,15 @@ rtx_status rtx_trace(rtx_ctx* c, const rtx_ray* rays, uint32_t n, const rtx_trac
   c->last_stats.prediction_hits = stats[6];
   c->last_stats.fallbacks = stats[7];
   return kernel_error_status(err);
+}
+
+rtx_status rtx_last_kernel_ms(rtx_ctx* c, float* ms) {
+  if (!c || !ms) return RTX_ERR_INVALID_ARGUMENT;
+  if (!c->kernel_timed) return RTX_ERR_INVALID_ARGUMENT;
+  HIP_TRY(hipSetDevice(c->device));
+  HIP_TRY(hipEventSynchronize(c->evk1));
+  HIP_TRY(hipEventElapsedTime(ms, c->evk0, c->evk1));
+  return RTX_OK;
 }
 
 rtx_status rtx_get_prefetch_stats(rtx_ctx* c, rtx_prefetch_stats* out) {

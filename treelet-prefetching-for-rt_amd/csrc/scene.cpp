@@ -78,8 +78,9 @@ Camera workload_camera(const rtx_workload_desc* d) {
   if (d->kind == RTX_WORKLOAD_SPHERE_AND_QUAD) {
     c = {{0.0, 0.6, 3.6}, {0.0, -0.15, 0.0}, {0.0, 1.0, 0.0}, 45.0};
   } else {
+    // In front of the box, close enough that the box fills the view.
     double h = 0.5 * d->box_size;
-    c = {{h, h, -1.0 * d->box_size}, {h, h, h}, {0.0, 1.0, 0.0}, 60.0};
+    c = {{h, h, -0.4 * d->box_size}, {h, h, h}, {0.0, 1.0, 0.0}, 80.0};
   }
   return c;
 }

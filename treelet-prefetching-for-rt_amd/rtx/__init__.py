@@ -115,6 +115,7 @@ _sig("rtx_get_treelets", _i32, _vp, _vp, _vp, _vp, _vp)
 _sig("rtx_set_treelets", _i32, _vp, _u32, _u32, _vp, _vp, _vp, _vp)
 _sig("rtx_trace", _i32, _vp, _vp, _u32, _P(_TraceOptions), _vp, _vp, _u64, _vp)
 _sig("rtx_get_prefetch_stats", _i32, _vp, _P(_PrefetchStats))
+_sig("rtx_last_kernel_ms", _i32, _vp, _P(_f32))
 _sig("rtx_builder_create", _i32, _P(_vp))
 _sig("rtx_builder_destroy", _i32, _vp)
 _sig("rtx_builder_add_blas", _i32, _vp, _vp, _u32, _u32, _P(_u32))
@@ -135,6 +136,7 @@ EXPORTED_SYMBOLS = [
     "rtx_malloc", "rtx_free", "rtx_memcpy_h2d", "rtx_memcpy_d2h", "rtx_memset", "rtx_timer_start",
     "rtx_timer_stop", "rtx_device_info", "rtx_set_scene", "rtx_set_scene_device", "rtx_validate_scene",
     "rtx_build_treelets", "rtx_get_treelets", "rtx_set_treelets", "rtx_trace", "rtx_get_prefetch_stats",
+    "rtx_last_kernel_ms",
     "rtx_builder_create", "rtx_builder_destroy", "rtx_builder_add_blas", "rtx_builder_add_instance",
     "rtx_builder_finish", "rtx_builder_copy_arena", "rtx_builder_arena_info", "rtx_builder_counts", "rtx_builder_blas_offset",
     "rtx_mesh_displaced_sphere", "rtx_workload_scene", "rtx_workload_primary_rays", "rtx_workload_secondary_rays",
@@ -417,6 +419,11 @@ class Context:
         s = _PrefetchStats()
         _check(_lib.rtx_get_prefetch_stats(self._h, ctypes.byref(s)), "rtx_get_prefetch_stats")
         return {f: getattr(s, f) for f, _ in _PrefetchStats._fields_}
+
+    def last_kernel_ms(self):
+        ms = ctypes.c_float(0)
+        _check(_lib.rtx_last_kernel_ms(self._h, ctypes.byref(ms)), "rtx_last_kernel_ms")
+        return ms.value
 
     def timer_start(self):
         _check(_lib.rtx_timer_start(self._h), "rtx_timer_start")
