@@ -160,27 +160,30 @@ def validate(arena, tlas):
 
 
 def ray_box_test(lo, hi, idir, org, tmin, tmax):
+    a = [_f3(lo), _f3(hi), _f3(idir), _f3(org)]
     th = ctypes.c_float(0)
-    r = _lib.orc_ray_box_test(_p(_f3(lo)), _p(_f3(hi)), _p(_f3(idir)), _p(_f3(org)), tmin, tmax, ctypes.byref(th))
+    r = _lib.orc_ray_box_test(*(_p(x) for x in a), tmin, tmax, ctypes.byref(th))
     return bool(r), th.value
 
 
 def calculate_idir(d):
+    dd = _f3(d)
     out = np.zeros(3, np.float32)
-    _lib.orc_calculate_idir(_p(_f3(d)), _p(out))
+    _lib.orc_calculate_idir(_p(dd), _p(out))
     return out
 
 
 def mt_ray_triangle_test(p0, p1, p2, org, d):
+    a = [_f3(p0), _f3(p1), _f3(p2), _f3(org), _f3(d)]
     th = ctypes.c_float(0)
-    r = _lib.orc_mt_ray_triangle_test(_p(_f3(p0)), _p(_f3(p1)), _p(_f3(p2)), _p(_f3(org)), _p(_f3(d)),
-                                      ctypes.byref(th))
+    r = _lib.orc_mt_ray_triangle_test(*(_p(x) for x in a), ctypes.byref(th))
     return bool(r), th.value
 
 
 def barycentric(p, a, b, c):
+    args = [_f3(p), _f3(a), _f3(b), _f3(c)]
     out = np.zeros(3, np.float32)
-    _lib.orc_barycentric(_p(_f3(p)), _p(_f3(a)), _p(_f3(b)), _p(_f3(c)), _p(out))
+    _lib.orc_barycentric(*(_p(x) for x in args), _p(out))
     return out
 
 
@@ -193,8 +196,9 @@ def set_child_bounds(node64, child):
 
 def make_transformed_ray(rows12, org, d, tmin, tmax):
     rows = np.ascontiguousarray(rows12, dtype=np.float32)
+    o3, d3 = _f3(org), _f3(d)
     oo, od = np.zeros(3, np.float32), np.zeros(3, np.float32)
     a, b, m = ctypes.c_float(0), ctypes.c_float(0), ctypes.c_float(0)
-    _lib.orc_make_transformed_ray(_p(rows), _p(_f3(org)), _p(_f3(d)), tmin, tmax, _p(oo), _p(od), ctypes.byref(a),
+    _lib.orc_make_transformed_ray(_p(rows), _p(o3), _p(d3), tmin, tmax, _p(oo), _p(od), ctypes.byref(a),
                                   ctypes.byref(b), ctypes.byref(m))
     return oo, od, a.value, b.value, m.value

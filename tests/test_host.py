@@ -1,0 +1,168 @@
+"""Host-side checks that need no GPU: the C ABI library loads and exports every
+declared symbol, the builder emits a valid arena, argument errors are reported."""
+import ctypes
+import os
+import re
+import struct
+
+import numpy as np
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def _declared(header):
+    text = open(os.path.join(ROOT, "include", header)).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(rtx_[a-z0-9_]+)\s*\(", text)))
+
+
+@pytest.mark.parametrize("header", ["rtx.h", "rtx_scene.h"])
+def test_library_exports_every_declared_symbol(rtx, header):
+    lib = ctypes.CDLL(rtx.LIB_PATH)
+    names = _declared(header)
+    assert len(names) > 5
+    missing = [n for n in names if not hasattr(lib, n)]
+    assert not missing, missing
+    assert set(names) <= set(rtx.EXPORTED_SYMBOLS)
+
+
+def test_status_strings(rtx):
+    for s in range(14):
+        assert rtx._lib.rtx_status_string(s)
+    assert rtx.version().startswith("rtx-mi355x")
+
+
+def test_create_without_device_reports_no_device(rtx):
+    if rtx.device_count() > 0:
+        pytest.skip("a HIP device is present")
+    with pytest.raises(rtx.RtxError) as e:
+        rtx.Context(0)
+    assert e.value.status == 8
+
+
+def _leaves(A, hdr):
+    root = hdr + struct.unpack_from("<Q", A, hdr)[0]
+    out, stack, internal = [], [root], 0
+    while stack:
+        n = stack.pop()
+        internal += 1
+        c = n + 64 * struct.unpack_from("<i", A, n + 12)[0]
+        for i in range(6):
+            info = int(A[n + 22 + i]) & 0x3F
+            size, typ = info & 3, info >> 2
+            if not size:
+                continue
+            if typ == 0:
+                stack.append(c)
+            else:
+                out.append(c)
+            c += 64 * size
+    return out, internal
+
+
+def test_builder_blas_reaches_every_triangle_once(rtx, oracle):
+    tris = rtx.displaced_sphere(24, 12, seed=11)
+    b = rtx.SceneBuilder()
+    bid = b.add_blas(tris, geometry_index=5)
+    b.add_instance(bid, instance_id=42)
+    s = b.finish()
+    A = s.arena
+    assert oracle.validate(A, s.tlas_offset) > len(tris)
+    hdr = s.blas_offset(0)
+    leaves, _ = _leaves(A, hdr)
+    prims = sorted(struct.unpack_from("<I", A, l + 8)[0] for l in leaves)
+    assert prims == list(range(len(tris)))
+    for l in leaves:
+        p = struct.unpack_from("<I", A, l + 8)[0]
+        v = np.frombuffer(A[l + 16:l + 52].tobytes(), np.float32).reshape(3, 3)
+        assert np.array_equal(v, tris[p].reshape(3, 3))
+        d1 = struct.unpack_from("<I", A, l + 4)[0]
+        assert d1 & 0x0FFFFFFF == 5 and (d1 >> 29) & 1 == 0
+
+
+def test_builder_child_boxes_are_conservative(rtx, oracle):
+    tris = rtx.displaced_sphere(32, 16, seed=5)
+    b = rtx.SceneBuilder()
+    b.add_instance(b.add_blas(tris))
+    s = b.finish()
+    A = s.arena
+    hdr = s.blas_offset(0)
+
+    def bounds(n):
+        """Asserts every decoded child box holds the triangles below it; returns their bounds."""
+        c = n + 64 * struct.unpack_from("<i", A, n + 12)[0]
+        lo, hi = np.full(3, np.inf, np.float32), np.full(3, -np.inf, np.float32)
+        for i in range(6):
+            info = int(A[n + 22 + i]) & 0x3F
+            size, typ = info & 3, info >> 2
+            if not size:
+                continue
+            blo, bhi = oracle.set_child_bounds(A[n:n + 64], i)
+            if typ == 0:
+                clo, chi = bounds(c)
+            else:
+                v = np.frombuffer(A[c + 16:c + 52].tobytes(), np.float32).reshape(3, 3)
+                clo, chi = v.min(0), v.max(0)
+            assert np.all(blo <= clo) and np.all(bhi >= chi)
+            lo, hi = np.minimum(lo, clo), np.maximum(hi, chi)
+            c += 64 * size
+        return lo, hi
+
+    bounds(hdr + struct.unpack_from("<Q", A, hdr)[0])
+
+
+def test_builder_instance_leaf_layout(rtx):
+    b = rtx.SceneBuilder()
+    bid = b.add_blas(rtx.displaced_sphere(8, 4))
+    m = np.array([[2, 0, 0, 10], [0, 2, 0, -4], [0, 0, 2, 1]], np.float32)
+    b.add_instance(bid, m, instance_id=9, hit_group=3)
+    s = b.finish()
+    A = s.arena
+    root = struct.unpack_from("<Q", A, 0)[0]
+    leaf = root + 64 * struct.unpack_from("<i", A, root + 12)[0]
+    assert int(A[root + 22]) & 3 == 2 and (int(A[root + 22]) & 0x3F) >> 2 == 1
+    w2o = np.frombuffer(A[leaf + 16:leaf + 64].tobytes(), np.float32).reshape(4, 3)
+    o2w = np.frombuffer(A[leaf + 80:leaf + 128].tobytes(), np.float32).reshape(4, 3)
+    assert np.allclose(w2o[:3], np.eye(3) * 0.5) and np.allclose(w2o[3], [-5, 2, -0.5])
+    assert np.allclose(o2w[:3], np.eye(3) * 2) and np.allclose(o2w[3], [10, -4, 1])
+    assert leaf + struct.unpack_from("<Q", A, leaf + 64)[0] == s.blas_offset(bid)
+    assert struct.unpack_from("<I", A, leaf + 72)[0] == 9
+    assert struct.unpack_from("<I", A, leaf + 4)[0] & 0xFFFFFF == 3
+
+
+def test_builder_rejects_bad_input(rtx):
+    b = rtx.SceneBuilder()
+    bad = np.zeros((1, 9), np.float32)
+    bad[0, 4] = np.nan
+    with pytest.raises(rtx.RtxError):
+        b.add_blas(bad)
+    with pytest.raises(rtx.RtxError):
+        b.add_instance(3)  # no such BLAS
+    bid = b.add_blas(rtx.displaced_sphere(4, 2))
+    with pytest.raises(rtx.RtxError):
+        b.add_instance(bid, np.zeros((3, 4), np.float32))  # singular transform
+
+
+def test_workloads_are_deterministic(rtx):
+    d = rtx.workload_desc(rtx.WORKLOAD_INSTANCED_SPHERES, n_instances=8, slices=8, stacks=4, width=16, height=16,
+                          seed=4, box_size=10.0)
+    a, b = rtx.workload_scene(d), rtx.workload_scene(d)
+    assert np.array_equal(a.arena, b.arena)
+    assert rtx.primary_rays(d).tobytes() == rtx.primary_rays(d).tobytes()
+    c = rtx.workload_scene(rtx.workload_desc(rtx.WORKLOAD_INSTANCED_SPHERES, n_instances=8, slices=8, stacks=4,
+                                             width=16, height=16, seed=5, box_size=10.0))
+    assert not np.array_equal(a.arena, c.arena)
+
+
+def test_secondary_rays(rtx, oracle, c1):
+    scene, rays = c1
+    h = oracle.trace(scene.arena, scene.tlas_offset, rays, records=False)
+    diff, src = scene.secondary_rays(rays, h, kind=0, seed=3)
+    assert len(diff) == h["hit"].sum()
+    assert np.all(h["hit"][src] == 1)
+    n = np.linalg.norm(diff["dir"], axis=1)
+    assert np.allclose(n, 1, atol=1e-5)
+    shadow, src2 = scene.secondary_rays(rays, h, kind=1, seed=3, light=(0, 10, 0))
+    assert np.all(shadow["flags"] == (1 | 4 | 8))
+    assert np.all(shadow["tmax"] > 0)
