@@ -27,10 +27,14 @@ import numpy as np  # noqa: E402
 HBM_PEAK_GBS = 8000.0
 METRIC = "Mrays/s plain vs treelet vs learned, 1M-tri diffuse; node-fetch GB/s % of HBM peak"
 
-# name -> (traversal order, prefetch policy)
+# name -> (traversal order, prefetch policy); policies: 1 always, 2 popularity, 3 partial
 MODES = {
     "plain": (0, 0),
     "treelet": (1, 0),
+    "plain+stage": (0, 1),
+    "treelet+stage": (1, 1),
+    "treelet+stage_pop": (1, 2),
+    "treelet+stage_partial": (1, 3),
 }
 HEADLINE = "plain"
 
@@ -71,6 +75,8 @@ def main():
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--budget", type=int, default=8192, help="treelet byte budget")
+    ap.add_argument("--stage", type=int, default=8192, help="LDS bytes per wave for a staged treelet")
+    ap.add_argument("--threshold", type=float, default=0.5, help="popularity share for the popularity policy")
     ap.add_argument("--mode", default=HEADLINE, choices=sorted(MODES))
     ap.add_argument("--modes", default=",".join(MODES), help="modes to measure")
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -108,15 +114,16 @@ def main():
     results = {}
     for name in [m for m in args.modes.split(",") if m in MODES]:
         order, prefetch = MODES[name]
+        kw = dict(mode=order, prefetch=prefetch, threshold=args.threshold, stage_bytes=args.stage if prefetch else 0)
         for _ in range(args.warmup):
-            st = ctx.trace_device(rays_d, n, hits_d, mode=order, prefetch=prefetch)
+            st = ctx.trace_device(rays_d, n, hits_d, **kw)
             if st != 0:
                 raise rtx.RtxError(st, f"rtx_trace {name}")
         kms = []
         ctx.synchronize()
         t0 = time.perf_counter()
         for _ in range(args.steps):
-            st = ctx.trace_device(rays_d, n, hits_d, mode=order, prefetch=prefetch)
+            st = ctx.trace_device(rays_d, n, hits_d, **kw)
             if st != 0:
                 raise rtx.RtxError(st, f"rtx_trace {name}")
             kms.append(ctx.last_kernel_ms())

@@ -76,7 +76,8 @@ enum {
 
 /* Build the scene of a workload into a fresh builder (finished). */
 rtx_status rtx_workload_scene(const rtx_workload_desc* d, uint32_t threads, rtx_builder** out);
-/* Pinhole primary rays through pixel centres, row-major, one per pixel. */
+/* Pinhole primary rays through pixel centres, one per pixel, ordered by 8x8
+ * pixel tiles (tiles row-major, pixels row-major inside a tile). */
 rtx_status rtx_workload_primary_rays(const rtx_workload_desc* d, rtx_ray* rays_out);
 /* One secondary ray per hit: kind 0 = cosine-weighted diffuse bounce, 1 = shadow ray
  * toward light_pos (Opaque|TerminateOnFirstHit|SkipClosestHitShader, tmax = distance).

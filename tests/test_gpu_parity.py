@@ -61,6 +61,26 @@ def test_plain_records_carry_treelets(gpu, oracle, c1):
 
 
 @pytest.mark.parametrize("mode", [0, 1])
+@pytest.mark.parametrize("prefetch,stage", [(1, 4096), (1, 8192), (2, 8192), (3, 16384), (1, 65536)])
+def test_staging_never_changes_results(gpu, oracle, c1, mode, prefetch, stage):
+    """Treelet staging into LDS only moves where node bytes are read from."""
+    scene, rays = c1
+    _scene_on(gpu, scene)
+    gpu.build_treelets(8192)
+    p = oracle.Partition(scene.arena, scene.tlas_offset, 8192)
+    h_ref, o_ref, r_ref = oracle.trace(scene.arena, scene.tlas_offset, rays, mode=mode, partition=p)
+    h, o, r = gpu.trace(rays, mode=mode, prefetch=prefetch, threshold=0.5, records=True, stage_bytes=stage)
+    assert_hits_equal(h, h_ref, f"staged mode {mode} prefetch {prefetch} stage {stage}")
+    assert_records_equal(o, r, o_ref, r_ref, f"staged mode {mode} prefetch {prefetch} stage {stage}")
+    s = gpu.prefetch_stats()
+    # one node read per internal node, instance leaf and primitive leaf visited
+    assert s["node_reads"] == int(h["n_nodes"].sum())
+    assert s["lds_hits"] <= s["node_reads"]
+    assert s["wasted_bytes"] <= s["staged_bytes"]
+    assert s["staged_treelets"] > 0 and s["lds_hits"] > 0
+
+
+@pytest.mark.parametrize("mode", [0, 1])
 def test_instanced_scenes(gpu, oracle, instanced, instanced_shared, mode):
     for scene, rays in (instanced, instanced_shared):
         _scene_on(gpu, scene)

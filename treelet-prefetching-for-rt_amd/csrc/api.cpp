@@ -364,16 +364,29 @@ rtx_status rtx_trace(rtx_ctx* c, const rtx_ray* rays, uint32_t n, const rtx_trac
     HIP_TRY(hipStreamSynchronize(c->stream));
     return RTX_OK;
   }
-  const uint32_t lds_stack = opts->lds_stack ? opts->lds_stack : 8;
+  uint32_t lds_stack = opts->lds_stack ? opts->lds_stack : 8;
   if (lds_stack != 4 && lds_stack != 8 && lds_stack != 16) return RTX_ERR_INVALID_ARGUMENT;
+  // treelet staging: one wave per workgroup with stage_bytes of LDS for the image
+  uint32_t stage = 0;
+  if (opts->prefetch != RTX_PREFETCH_NONE) {
+    stage = opts->stage_bytes;
+    if (stage == 0) {
+      stage = 4096;
+      while (stage < c->part.budget && stage < 16384) stage *= 2;
+    }
+    if (stage < 4096 || stage > 65536 || (stage & (stage - 1))) return RTX_ERR_INVALID_ARGUMENT;
+    if (opts->prefetch == RTX_PREFETCH_LEARNED) return RTX_ERR_UNSUPPORTED;
+    lds_stack = 8;
+  }
+  const uint32_t bs = trace_block_size(stage);
   const uint32_t waves_per_cu = opts->waves_per_cu ? opts->waves_per_cu : 16;
-  const uint32_t blocks_per_cu = std::max(1u, waves_per_cu / 4);
+  const uint32_t blocks_per_cu = std::max(1u, waves_per_cu * 64 / bs);
   uint32_t grid = (uint32_t)c->n_cus * blocks_per_cu;
-  grid = std::min<uint32_t>(grid, (n + kBlock - 1) / kBlock);
+  grid = std::min<uint32_t>(grid, (n + bs - 1) / bs);
   grid = std::max(grid, 1u);
   const uint32_t spill_cap = 128;
   const uint32_t ns = opts->mode == RTX_MODE_TREELET ? 2 : 1;
-  rtx_status st = ensure_spill(c, (size_t)grid * kBlock * spill_cap * ns);
+  rtx_status st = ensure_spill(c, (size_t)grid * bs * spill_cap * ns);
   if (st != RTX_OK) return st;
 
   TraceParams p{};
@@ -382,7 +395,8 @@ rtx_status rtx_trace(rtx_ctx* c, const rtx_ray* rays, uint32_t n, const rtx_trac
   p.tlas = c->tlas;
   p.rays = rays;
   p.n = n;
-  p.refill = 32;
+  if (opts->refill > 64) return RTX_ERR_INVALID_ARGUMENT;
+  p.refill = opts->refill ? opts->refill : 32;
   p.hits = hits;
   p.records = nullptr;
   p.record_capacity = 0;
@@ -396,12 +410,18 @@ rtx_status rtx_trace(rtx_ctx* c, const rtx_ray* rays, uint32_t n, const rtx_trac
   p.stats = c->stats;
   p.prefetch = opts->prefetch;
   p.threshold = opts->threshold;
+  if (c->part.valid) {
+    p.image_slots = c->part.image_slots;
+    p.image_offset = c->part.image_offset;
+    p.image_bytes = c->part.image_bytes;
+    p.images = c->part.images;
+  }
 
   HIP_TRY(hipMemsetAsync(c->counters, 0, 2 * sizeof(uint32_t), c->stream));
   HIP_TRY(hipMemsetAsync(c->stats, 0, 16 * sizeof(unsigned long long), c->stream));
   c->kernel_timed = false;
   HIP_TRY(hipEventRecord(c->evk0, c->stream));
-  HIP_TRY(launch_trace(p, opts->mode, false, lds_stack, grid, c->stream));
+  HIP_TRY(launch_trace(p, opts->mode, false, lds_stack, stage, grid, c->stream));
   HIP_TRY(hipEventRecord(c->evk1, c->stream));
   c->kernel_timed = true;
   if (opts->emit_records) {
@@ -421,7 +441,7 @@ rtx_status rtx_trace(rtx_ctx* c, const rtx_ray* rays, uint32_t n, const rtx_trac
     p.record_offsets = offsets;
     HIP_TRY(hipMemsetAsync(c->counters, 0, 2 * sizeof(uint32_t), c->stream));
     HIP_TRY(hipMemsetAsync(c->stats, 0, 16 * sizeof(unsigned long long), c->stream));
-    HIP_TRY(launch_trace(p, opts->mode, true, lds_stack, grid, c->stream));
+    HIP_TRY(launch_trace(p, opts->mode, true, lds_stack, stage, grid, c->stream));
   }
   uint32_t err = 0;
   unsigned long long stats[16];

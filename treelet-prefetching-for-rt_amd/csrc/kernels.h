@@ -43,18 +43,16 @@ struct TraceParams {
   // treelet staging
   uint32_t prefetch;                 // RTX_PREFETCH_*
   float threshold;
-  const uint32_t* stage_slot;        // node slot -> position (64 B units) inside its owner's image
+  const uint32_t* image_slots;       // node slot held by each 64-byte unit of `images`
   const uint64_t* image_offset;      // dense treelet -> byte offset of its image in `images`
   const uint32_t* image_bytes;       // dense treelet -> image size
   const uint8_t* images;             // treelet-contiguous copies of the arena
-  uint32_t stage_bytes;              // LDS bytes per wave for a staged treelet
 };
 
-struct PartitionParams;
-
 // Launchers (defined in the .hip files).
-hipError_t launch_trace(const TraceParams& p, uint32_t mode, bool records, uint32_t lds_stack, uint32_t grid,
-                        hipStream_t s);
+uint32_t trace_block_size(uint32_t stage_bytes);
+hipError_t launch_trace(const TraceParams& p, uint32_t mode, bool records, uint32_t lds_stack, uint32_t stage_bytes,
+                        uint32_t grid, hipStream_t s);
 hipError_t launch_scan_records(const rtx_hit* hits, uint32_t n, uint64_t* offsets, uint64_t* block_tmp,
                                hipStream_t s);
 

@@ -24,6 +24,7 @@ struct Partition {
   uint64_t* image_offset = nullptr;  // [n]
   uint32_t* image_bytes = nullptr;   // [n]
   uint32_t* stage_slot = nullptr;    // [arena_bytes/64]
+  uint32_t* image_slots = nullptr;   // [images_total/64] node slot at each 64-byte unit (instance tail: none)
   uint64_t images_total = 0;
   uint32_t max_image_bytes = 0;
   rtx_treelet_summary summary{};

@@ -601,7 +601,7 @@ __global__ __launch_bounds__(256) void image_layout(uint32_t n, const uint64_t* 
 __global__ __launch_bounds__(256) void image_copy(uint32_t n, const uint64_t* lo, const uint64_t* ln,
                                                   const uint32_t* owner, const uint8_t* kind,
                                                   const uint32_t* stage_slot, const uint64_t* image_offset,
-                                                  const uint8_t* A, uint8_t* images) {
+                                                  const uint8_t* A, uint8_t* images, uint32_t* image_slots) {
   const uint32_t lane = threadIdx.x & 63u;
   const uint32_t wave = blockIdx.x * 4 + (threadIdx.x >> 6);
   for (uint32_t t = wave; t < n; t += gridDim.x * 4) {
@@ -613,6 +613,9 @@ __global__ __launch_bounds__(256) void image_copy(uint32_t n, const uint64_t* lo
       const uint4* src = reinterpret_cast<const uint4*>(A + ((uint64_t)s << 6));
       uint4* dst = reinterpret_cast<uint4*>(img + (uint64_t)stage_slot[s] * 64);
       for (uint32_t k = 0; k < 4 * units; ++k) dst[k] = src[k];
+      uint32_t* idx = image_slots + image_offset[t] / 64 + stage_slot[s];
+      idx[0] = s;
+      if (units == 2) idx[1] = 0xffffffffu;
     }
   }
 }
@@ -642,8 +645,9 @@ rtx_status build_images(const uint8_t* A, uint64_t bytes, hipStream_t s, const u
   p.max_image_bytes = 0;
   for (uint32_t v : ib) p.max_image_bytes = std::max(p.max_image_bytes, v);
   CS(dmalloc(&p.images, total));
+  CS(dmalloc(&p.image_slots, total / 64));
   hipLaunchKernelGGL(image_copy, dim3(g), dim3(256), 0, s, p.n, p.list_offsets, p.list_nodes, p.owner, kind,
-                     p.stage_slot, p.image_offset, A, p.images);
+                     p.stage_slot, p.image_offset, A, p.images, p.image_slots);
   CK(hipGetLastError());
   return RTX_OK;
 }
@@ -657,6 +661,7 @@ void partition_free(Partition& p) {
   hipFree(p.image_offset);
   hipFree(p.image_bytes);
   hipFree(p.stage_slot);
+  hipFree(p.image_slots);
   p = Partition{};
 }
 

@@ -178,14 +178,25 @@ rtx_status rtx_workload_primary_rays(const rtx_workload_desc* d, rtx_ray* rays) 
   cross3(right, fwd, up);
   const double tan_half = std::tan(0.5 * c.fov_deg * kPi / 180.0);
   const double aspect = (double)d->width / d->height;
-  for (uint32_t y = 0; y < d->height; ++y)
-    for (uint32_t x = 0; x < d->width; ++x) {
+  // Rays are ordered by 8x8 pixel tiles (row-major tiles, row-major pixels
+  // inside a tile), so the 64 lanes of a wave trace one compact tile, as the
+  // reference groups 8x4 pixels per 32-wide warp (WARP_8X4,
+  // vulkan_ray_tracing.cc:3503-3592).  Partial tiles at the right and bottom
+  // edges keep the same order over the pixels that exist.
+  const uint32_t tiles_x = (d->width + 7) / 8;
+  size_t k = 0;
+  for (uint32_t ty = 0; ty < (d->height + 7) / 8; ++ty)
+    for (uint32_t tx = 0; tx < tiles_x; ++tx)
+      for (uint32_t iy = 0; iy < 8; ++iy)
+        for (uint32_t ix = 0; ix < 8; ++ix) {
+      const uint32_t x = tx * 8 + ix, y = ty * 8 + iy;
+      if (x >= d->width || y >= d->height) continue;
       double u = (2.0 * (x + 0.5) / d->width - 1.0) * tan_half * aspect;
       double v = (1.0 - 2.0 * (y + 0.5) / d->height) * tan_half;
       double dir[3];
       for (int a = 0; a < 3; ++a) dir[a] = fwd[a] + u * right[a] + v * up[a];
       normalize3(dir);
-      rtx_ray& r = rays[(size_t)y * d->width + x];
+      rtx_ray& r = rays[k++];
       memset(&r, 0, sizeof(r));
       for (int a = 0; a < 3; ++a) {
         r.org[a] = (float)c.eye[a];

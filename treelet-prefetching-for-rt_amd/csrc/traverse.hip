@@ -22,6 +22,16 @@
 //
 // Stacks: a ring of K entries per lane in LDS, spilling the oldest entries to
 // a per-lane region in global memory (exact LIFO order).
+//
+// Treelet staging (STAGE > 0, one wave per workgroup): the wave keeps one
+// treelet image in LDS, copied from the treelet-contiguous image buffer with
+// coalesced 16-byte loads, and an LDS hash table from node slot to position in
+// the image.  A node read first probes the table; hits read LDS, misses read
+// global memory.  When most lanes miss, the wave gathers the owning treelet of
+// each lane's next node and stages the most popular one, following the
+// RT-unit policy of src/gpgpu-sim/shader.cc:3419-3559 (heuristic 0 always,
+// 1 only above a popularity share, 2 a popularity-sized prefix).  Staging
+// changes where node bytes are read from, never which nodes are visited.
 #include "device_common.h"
 #include "kernels.h"
 
@@ -32,6 +42,7 @@ namespace {
 
 constexpr uint32_t kFlTop = 1u, kFlLeaf = 2u;
 constexpr uint32_t kNoSlot = 0xffffffffu;
+constexpr uint32_t kNone = 0xffffffffu;
 
 __device__ __forceinline__ uint64_t make_entry(uint32_t slot, uint32_t flags, uint32_t level, uint32_t inst) {
   uint32_t lv = level > 255u ? 255u : level;
@@ -46,24 +57,24 @@ struct Stack {
   uint32_t bot, cnt, g;
 };
 
-template <int K>
+template <int K, uint32_t BS>
 __device__ __forceinline__ bool st_push(Stack& s, uint64_t e, uint64_t* lds, uint64_t* g, uint32_t gcap) {
   if (s.cnt == (uint32_t)K) {
     if (s.g >= gcap) return false;
-    g[s.g++] = lds[s.bot * kBlock];
+    g[s.g++] = lds[s.bot * BS];
     s.bot = (s.bot + 1) & (K - 1);
     s.cnt--;
   }
-  lds[((s.bot + s.cnt) & (K - 1)) * kBlock] = e;
+  lds[((s.bot + s.cnt) & (K - 1)) * BS] = e;
   s.cnt++;
   return true;
 }
 
-template <int K>
+template <int K, uint32_t BS>
 __device__ __forceinline__ uint64_t st_pop(Stack& s, uint64_t* lds, uint64_t* g) {
   if (s.cnt == 0) return g[--s.g];
   s.cnt--;
-  return lds[((s.bot + s.cnt) & (K - 1)) * kBlock];
+  return lds[((s.bot + s.cnt) & (K - 1)) * BS];
 }
 
 __device__ __forceinline__ bool st_empty(const Stack& s) { return s.cnt == 0 && s.g == 0; }
@@ -72,15 +83,30 @@ __device__ __forceinline__ uint32_t lanes_below(uint64_t mask) {
   return __builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0u));
 }
 
-template <int MODE, bool REC, int K>
-__global__ __launch_bounds__(kBlock) void trace_kernel(TraceParams p) {
+__device__ __forceinline__ uint32_t slot_hash(uint32_t s) { return s * 2654435761u; }
+
+__device__ __forceinline__ uint32_t wave_sum(uint32_t v) {
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+
+template <int MODE, bool REC, int K, int STAGE>
+__global__ __launch_bounds__(STAGE ? 64 : 256) void trace_kernel(TraceParams p) {
+  constexpr uint32_t BS = STAGE ? 64u : 256u;
   constexpr int NS = MODE == 1 ? 2 : 1;
-  __shared__ uint64_t lds_stack[NS * K * kBlock];
+  constexpr uint32_t UNITS = STAGE / 64;
+  constexpr uint32_t HSIZE = STAGE ? 2 * UNITS : 1;
+  __shared__ uint64_t lds_stack[NS * K * BS];
+  __shared__ uint4 stage_img[STAGE ? STAGE / 16 : 1];
+  __shared__ uint32_t hkey[HSIZE];
+  __shared__ uint16_t hpos[HSIZE];
+  __shared__ uint8_t used[STAGE ? UNITS : 1];
+
   const uint32_t tid = threadIdx.x;
   const uint32_t lane = tid & 63u;
   uint64_t* ldsA = lds_stack + tid;
-  uint64_t* ldsB = lds_stack + (NS - 1) * K * kBlock + tid;
-  const uint64_t lane_slot = (uint64_t)blockIdx.x * kBlock + tid;
+  uint64_t* ldsB = lds_stack + (NS - 1) * K * BS + tid;
+  const uint64_t lane_slot = (uint64_t)blockIdx.x * BS + tid;
   uint64_t* gA = p.spill + lane_slot * (uint64_t)p.spill_cap * NS;
   uint64_t* gB = gA + (NS - 1) * (uint64_t)p.spill_cap;
   const uint8_t* __restrict__ A = p.arena;
@@ -102,6 +128,20 @@ __global__ __launch_bounds__(kBlock) void trace_kernel(TraceParams p) {
   uint32_t n_rec = 0, n_nodes = 0, fetch_bytes = 0, n_any = 0, max_level = 0, steps = 0;
   uint64_t rec_base = 0;
   bool phase2 = false;
+
+  // The entry a lane popped this iteration and processes below.
+  bool has_e = false;
+  uint64_t e = 0;
+
+  // staging state (wave-uniform)
+  uint32_t staged = kNone, staged_units = 0, cooldown = 0;
+  uint64_t st_treelets = 0, st_bytes = 0, st_wasted = 0;
+  uint32_t reads = 0, lds_hits = 0;
+
+  if constexpr (STAGE > 0) {
+    for (uint32_t i = lane; i < HSIZE; i += 64) hkey[i] = 0;
+    __builtin_amdgcn_wave_barrier();
+  }
 
   auto emit = [&](uint64_t off, uint32_t size, uint32_t type) {
     if (type != RTX_TX_BVH_PRIMITIVE_LEAF_DESCRIPTOR) fetch_bytes += size;
@@ -125,9 +165,9 @@ __global__ __launch_bounds__(kBlock) void trace_kernel(TraceParams p) {
   auto push_entry = [&](uint64_t e, bool to_a) {
     bool ok;
     if (MODE == 0 || to_a)
-      ok = st_push<K>(sa, e, ldsA, gA, p.spill_cap);
+      ok = st_push<K, BS>(sa, e, ldsA, gA, p.spill_cap);
     else
-      ok = st_push<K>(sb, e, ldsB, gB, p.spill_cap);
+      ok = st_push<K, BS>(sb, e, ldsB, gB, p.spill_cap);
     if (!ok) err |= kErrStack;
   };
 
@@ -136,6 +176,51 @@ __global__ __launch_bounds__(kBlock) void trace_kernel(TraceParams p) {
     if (MODE == 0) return true;
     if (phase2) return false;
     return p.owner[slot] == p.first_treelet;
+  };
+
+  // Position of `slot` in the staged image, or kNone.
+  auto staged_pos = [&](uint32_t slot) -> uint32_t {
+    if constexpr (STAGE > 0) {
+      if (staged == kNone) return kNone;
+      uint32_t h = slot_hash(slot) & (HSIZE - 1);
+      for (uint32_t probe = 0; probe < HSIZE; ++probe) {
+        const uint32_t k = hkey[h];
+        if (k == slot + 1) return hpos[h];
+        if (k == 0) return kNone;
+        h = (h + 1) & (HSIZE - 1);
+      }
+    }
+    return kNone;
+  };
+
+  // Read `count` 16-byte words of a node starting at byte `first` (a multiple
+  // of 16): from the staged image when `pos` is its staged position, else
+  // from global memory.  Explicit LDS and global paths keep both loads typed.
+  auto read_words = [&](uint64_t off, uint32_t pos, uint32_t first, uint4* w, int count) {
+    if constexpr (STAGE > 0) {
+      reads++;
+      if (pos != kNone) {
+        lds_hits++;
+        used[pos] = 1;
+        const uint4* q = stage_img + pos * 4 + first / 16;
+        for (int k = 0; k < count; ++k) w[k] = q[k];
+        return;
+      }
+    }
+    const uint4* q = reinterpret_cast<const uint4*>(A + off + first);
+    for (int k = 0; k < count; ++k) w[k] = q[k];
+  };
+  auto read_node = [&](uint64_t off, uint32_t pos) -> Words16 {
+    uint4 w[4];
+    read_words(off, pos, 0, w, 4);
+    Words16 r;
+    for (int k = 0; k < 4; ++k) {
+      r.w[4 * k] = w[k].x;
+      r.w[4 * k + 1] = w[k].y;
+      r.w[4 * k + 2] = w[k].z;
+      r.w[4 * k + 3] = w[k].w;
+    }
+    return r;
   };
 
   auto load_instance = [&](uint32_t inst) {
@@ -153,8 +238,61 @@ __global__ __launch_bounds__(kBlock) void trace_kernel(TraceParams p) {
     cur_inst = inst;
   };
 
+  // Account the staged image's unused units, then forget it (wave-uniform).
+  auto evict = [&]() {
+    if constexpr (STAGE > 0) {
+      if (staged == kNone) return;
+      uint32_t u = 0;
+      for (uint32_t i = lane; i < staged_units; i += 64) u += used[i] ? 1u : 0u;
+      u = wave_sum(u);
+      st_wasted += (uint64_t)(staged_units - u) * 64;
+      staged = kNone;
+      staged_units = 0;
+    }
+  };
+
+  // Copy treelet t (its first `max_units` 64-byte units) into LDS (wave-uniform).
+  auto stage = [&](uint32_t t, uint32_t max_units) {
+    if constexpr (STAGE > 0) {
+      const uint32_t tt = __builtin_amdgcn_readfirstlane(t);
+      const uint64_t ioff = p.image_offset[tt];
+      uint32_t units = p.image_bytes[tt] / 64;
+      units = min(units, min(UNITS, max_units));
+      units = __builtin_amdgcn_readfirstlane(units);
+      if (units == 0) return;
+      evict();
+      const uint4* src = reinterpret_cast<const uint4*>(p.images + ioff);
+      for (uint32_t i = lane; i < units * 4; i += 64) stage_img[i] = src[i];
+      for (uint32_t i = lane; i < HSIZE; i += 64) hkey[i] = 0;
+      for (uint32_t i = lane; i < units; i += 64) used[i] = 0;
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+      const uint32_t* slots = p.image_slots + ioff / 64;
+      for (uint32_t u = lane; u < units; u += 64) {
+        const uint32_t s = slots[u];
+        if (s == kNone) continue;
+        uint32_t h = slot_hash(s) & (HSIZE - 1);
+        for (uint32_t probe = 0; probe < HSIZE; ++probe) {
+          if (atomicCAS(&hkey[h], 0u, s + 1) == 0u) {
+            hpos[h] = (uint16_t)u;
+            break;
+          }
+          h = (h + 1) & (HSIZE - 1);
+        }
+      }
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+      staged = tt;
+      staged_units = units;
+      st_treelets++;
+      st_bytes += (uint64_t)units * 64;
+    }
+  };
+
   while (true) {
-    // ---- refill idle lanes from the global ray counter -------------------
+    // ---- refill idle lanes from the global ray counter (wave converged) ----
     const uint64_t idle = __ballot(!active);
     const uint32_t nidle = (uint32_t)__popcll(idle);
     if (!exhausted && nidle > 0 && (nidle == 64u || nidle >= p.refill)) {
@@ -216,40 +354,115 @@ __global__ __launch_bounds__(kBlock) void trace_kernel(TraceParams p) {
       if (exhausted) break;
       continue;
     }
-    if (!active) continue;
 
-    // ---- one traversal step ------------------------------------------------
-    bool done;
-    if (MODE == 0) {
-      done = st_empty(sa);
-    } else {
-      done = st_empty(sa) && st_empty(sb);
-    }
-    if (!done && err == 0) {
-      uint64_t e;
-      if (MODE == 0) {
-        e = st_pop<K>(sa, ldsA, gA);
-      } else if (!st_empty(sa)) {
-        e = st_pop<K>(sa, ldsA, gA);
+    // ---- pop the lane's next entry, or complete the ray ---------------------
+    if (active && has_e && err != 0) has_e = false;
+    if (active && !has_e) {
+      const bool done = MODE == 0 ? st_empty(sa) : (st_empty(sa) && st_empty(sb));
+      if (!done && err == 0) {
+        if (MODE == 0) {
+          e = st_pop<K, BS>(sa, ldsA, gA);
+        } else if (!st_empty(sa)) {
+          e = st_pop<K, BS>(sa, ldsA, gA);
+        } else {
+          phase2 = true;  // current_treelet_root := host address of the moved node (:1748-1754)
+          e = st_pop<K, BS>(sb, ldsB, gB);
+        }
+        has_e = true;
+        if (++steps > kMaxStepsPerRay) err |= kErrSteps;
       } else {
-        phase2 = true;  // current_treelet_root := host address of the moved node (:1748-1754)
-        e = st_pop<K>(sb, ldsB, gB);
+        // ray complete: write the hit record (:2990-3038)
+        uint4 o0 = make_uint4(0, 0, 0, 0), o1 = make_uint4(0, 0, 0, 0), o2 = make_uint4(0, 0, 0, 0);
+        if (err == 0 && min_thit < rtmax && closest_leaf != kNoSlot) {
+          const Words16 q = load64B(A + ((uint64_t)closest_leaf << 6));
+          const uint32_t inst_id = *reinterpret_cast<const uint32_t*>(A + ((uint64_t)closest_inst << 6) + 72);
+          const F3 v0 = mk3(f_of(q, 4), f_of(q, 5), f_of(q, 6));
+          const F3 v1 = mk3(f_of(q, 7), f_of(q, 8), f_of(q, 9));
+          const F3 v2 = mk3(f_of(q, 10), f_of(q, 11), f_of(q, 12));
+          const F3 ip = add3(ro, mk3(rd.x * min_thit, rd.y * min_thit, rd.z * min_thit));
+          const F3 op =
+              add3(cobj_o, mk3(cobj_d.x * min_thit_obj, cobj_d.y * min_thit_obj, cobj_d.z * min_thit_obj));
+          const F3 bc = barycentric(op, v0, v1, v2);
+          o0 = make_uint4(1u, q.w[2], q.w[1] & 0x0fffffffu, inst_id);
+          o1 = make_uint4(__float_as_uint(min_thit), __float_as_uint(ip.x), __float_as_uint(ip.y),
+                          __float_as_uint(ip.z));
+          o2 = make_uint4(__float_as_uint(bc.x), __float_as_uint(bc.y), __float_as_uint(bc.z), n_rec);
+        } else {
+          o2.w = n_rec;
+        }
+        uint4* hq = reinterpret_cast<uint4*>(p.hits + ray_id);
+        hq[0] = o0;
+        hq[1] = o1;
+        hq[2] = o2;
+        hq[3] = make_uint4(n_nodes, fetch_bytes, n_any, max_level);
+        if (err) atomicOr(p.error, err);
+        active = false;
       }
-      const uint32_t slot = e_slot(e);
-      const uint32_t flags = e_flags(e);
-      const uint32_t level = e_level(e);
-      const uint64_t off = (uint64_t)slot << 6;
-      if (++steps > kMaxStepsPerRay) err |= kErrSteps;
-      if (MODE == 1 && !(flags & kFlTop) && err == 0 && e_inst(e) != cur_inst) load_instance(e_inst(e));
+    }
 
+    const uint32_t slot = e_slot(e);
+    const uint32_t flags = e_flags(e);
+    const uint32_t level = e_level(e);
+    const uint64_t off = (uint64_t)slot << 6;
+
+    const bool step = has_e && err == 0;
+
+    // ---- staging decision (wave converged) -----------------------------------
+    uint32_t pos = kNone;
+    if constexpr (STAGE > 0) {
+      const bool want = has_e && err == 0;
+      if (want) pos = staged_pos(slot);
+      if (p.prefetch != RTX_PREFETCH_NONE) {
+        const uint32_t nact = (uint32_t)__popcll(__ballot(want));
+        const uint32_t nmiss = (uint32_t)__popcll(__ballot(want && pos == kNone));
+        if (cooldown > 0) {
+          cooldown--;
+        } else if (nact > 0 && 2 * nmiss > nact) {
+          uint32_t t = kNone;
+          if (want && pos == kNone && off + 64 <= bytes) t = p.owner[slot];
+          uint64_t rem = __ballot(t != kNone);
+          uint32_t best = kNone, bestc = 0;
+          for (int c = 0; c < 4 && rem; ++c) {
+            const uint32_t lead = (uint32_t)__ffsll((long long)rem) - 1u;
+            const uint32_t cand = __shfl(t, (int)lead, 64);
+            const uint64_t m = __ballot(t == cand);
+            const uint32_t cnt = (uint32_t)__popcll(m);
+            if (cnt > bestc || (cnt == bestc && cand < best)) {
+              best = cand;
+              bestc = cnt;
+            }
+            rem &= ~m;
+          }
+          const float share = (float)bestc / (float)nact;
+          bool ok = best != kNone && bestc >= 2 && bestc > nact - nmiss;
+          if (p.prefetch == RTX_PREFETCH_POPULARITY && share < p.threshold) ok = false;
+          if (ok) {
+            uint32_t maxu = UNITS;
+            if (p.prefetch == RTX_PREFETCH_PARTIAL) {
+              const uint32_t units = p.image_bytes[best] / 64;
+              maxu = (uint32_t)((float)units * share + 0.5f);
+            }
+            stage(best, maxu);
+            if (want) pos = staged_pos(slot);
+          } else {
+            cooldown = 8;
+          }
+        }
+      }
+    }
+
+    // ---- process the entry ----------------------------------------------------
+    if (step) {
+      has_e = false;
+      if (MODE == 1 && !(flags & kFlTop) && e_inst(e) != cur_inst) load_instance(e_inst(e));
       if (err != 0) {
-        // fall through to completion
+        // completes at the next iteration
       } else if (!(flags & kFlLeaf)) {
         // ---- internal node (top :2499-2599 / bottom :2686-2786) ----------
         if (off + 64 > bytes) {
           err |= kErrRange;
         } else {
-          const Words16 nd = load64B(A + off);
+          const Words16 nd = read_node(off, pos);
           emit(off, 64, RTX_TX_BVH_INTERNAL_NODE);
           n_nodes++;
           const bool top = (flags & kFlTop) != 0;
@@ -325,8 +538,11 @@ __global__ __launch_bounds__(kBlock) void trace_kernel(TraceParams p) {
         if (off + 128 > bytes) {
           err |= kErrRange;
         } else {
-          const uint4* q = reinterpret_cast<const uint4*>(A + off + 16);
-          uint4 a = q[0], b = q[1], c = q[2], d = q[3];
+          // a 128-byte leaf is served from LDS only when both of its units are staged
+          const uint32_t ipos = (pos != kNone && pos + 1 < staged_units) ? pos : kNone;
+          uint4 w4[4];
+          read_words(off, ipos, 16, w4, 4);
+          const uint4 a = w4[0], b = w4[1], c = w4[2], d = w4[3];
           float m[12] = {__uint_as_float(a.x), __uint_as_float(a.y), __uint_as_float(a.z), __uint_as_float(a.w),
                          __uint_as_float(b.x), __uint_as_float(b.y), __uint_as_float(b.z), __uint_as_float(b.w),
                          __uint_as_float(c.x), __uint_as_float(c.y), __uint_as_float(c.z), __uint_as_float(c.w)};
@@ -358,7 +574,7 @@ __global__ __launch_bounds__(kBlock) void trace_kernel(TraceParams p) {
           err |= kErrRange;
         } else {
           emit(off, 8, RTX_TX_BVH_PRIMITIVE_LEAF_DESCRIPTOR);
-          const Words16 q = load64B(A + off);
+          const Words16 q = read_node(off, pos);
           if (((q.w[1] >> 29) & 1u) == 0u) {
             if ((q.w[3] & 0x1ffffu) != 0u) err |= kErrType;
             const F3 v0 = mk3(f_of(q, 4), f_of(q, 5), f_of(q, 6));
@@ -398,35 +614,18 @@ __global__ __launch_bounds__(kBlock) void trace_kernel(TraceParams p) {
           }
         }
       }
-      if (err == 0) continue;
     }
+  }
 
-    // ---- ray complete: write the hit record (:2990-3038) --------------------
-    {
-      uint4 o0 = make_uint4(0, 0, 0, 0), o1 = make_uint4(0, 0, 0, 0), o2 = make_uint4(0, 0, 0, 0);
-      if (err == 0 && min_thit < rtmax && closest_leaf != kNoSlot) {
-        const Words16 q = load64B(A + ((uint64_t)closest_leaf << 6));
-        const uint32_t inst_id = *reinterpret_cast<const uint32_t*>(A + ((uint64_t)closest_inst << 6) + 72);
-        const F3 v0 = mk3(f_of(q, 4), f_of(q, 5), f_of(q, 6));
-        const F3 v1 = mk3(f_of(q, 7), f_of(q, 8), f_of(q, 9));
-        const F3 v2 = mk3(f_of(q, 10), f_of(q, 11), f_of(q, 12));
-        const F3 ip = add3(ro, mk3(rd.x * min_thit, rd.y * min_thit, rd.z * min_thit));
-        const F3 op = add3(cobj_o, mk3(cobj_d.x * min_thit_obj, cobj_d.y * min_thit_obj, cobj_d.z * min_thit_obj));
-        const F3 bc = barycentric(op, v0, v1, v2);
-        o0 = make_uint4(1u, q.w[2], q.w[1] & 0x0fffffffu, inst_id);
-        o1 = make_uint4(__float_as_uint(min_thit), __float_as_uint(ip.x), __float_as_uint(ip.y),
-                        __float_as_uint(ip.z));
-        o2 = make_uint4(__float_as_uint(bc.x), __float_as_uint(bc.y), __float_as_uint(bc.z), n_rec);
-      } else {
-        o2.w = n_rec;
-      }
-      uint4* hq = reinterpret_cast<uint4*>(p.hits + ray_id);
-      hq[0] = o0;
-      hq[1] = o1;
-      hq[2] = o2;
-      hq[3] = make_uint4(n_nodes, fetch_bytes, n_any, max_level);
-      if (err) atomicOr(p.error, err);
-      active = false;
+  if constexpr (STAGE > 0) {
+    evict();
+    const uint32_t r = wave_sum(reads), hl = wave_sum(lds_hits);
+    if (lane == 0) {
+      atomicAdd(&p.stats[0], (unsigned long long)r);
+      atomicAdd(&p.stats[1], (unsigned long long)hl);
+      atomicAdd(&p.stats[2], (unsigned long long)st_treelets);
+      atomicAdd(&p.stats[3], (unsigned long long)st_bytes);
+      atomicAdd(&p.stats[4], (unsigned long long)st_wasted);
     }
   }
 }
@@ -497,33 +696,44 @@ __global__ __launch_bounds__(kScanBlock) void scan_add(uint64_t* out, uint32_t n
     if (base + k < n) out[base + k] += add;
 }
 
-template <int MODE, bool REC, int K>
+template <int MODE, bool REC, int K, int STAGE>
 hipError_t launch_one(const TraceParams& p, uint32_t grid, hipStream_t s) {
-  hipLaunchKernelGGL((trace_kernel<MODE, REC, K>), dim3(grid), dim3(kBlock), 0, s, p);
+  constexpr uint32_t BS = STAGE ? 64u : 256u;
+  hipLaunchKernelGGL((trace_kernel<MODE, REC, K, STAGE>), dim3(grid), dim3(BS), 0, s, p);
   return hipGetLastError();
 }
 
 template <int MODE, bool REC>
-hipError_t launch_k(const TraceParams& p, uint32_t lds_stack, uint32_t grid, hipStream_t s) {
-  switch (lds_stack) {
-    case 4:
-      return launch_one<MODE, REC, 4>(p, grid, s);
-    case 8:
-      return launch_one<MODE, REC, 8>(p, grid, s);
-    case 16:
-      return launch_one<MODE, REC, 16>(p, grid, s);
-    default:
-      return hipErrorInvalidValue;
+hipError_t launch_k(const TraceParams& p, uint32_t lds_stack, uint32_t stage, uint32_t grid, hipStream_t s) {
+  if (stage == 0) {
+    switch (lds_stack) {
+      case 4: return launch_one<MODE, REC, 4, 0>(p, grid, s);
+      case 8: return launch_one<MODE, REC, 8, 0>(p, grid, s);
+      case 16: return launch_one<MODE, REC, 16, 0>(p, grid, s);
+      default: return hipErrorInvalidValue;
+    }
+  }
+  switch (stage) {
+    case 4096: return launch_one<MODE, REC, 8, 4096>(p, grid, s);
+    case 8192: return launch_one<MODE, REC, 8, 8192>(p, grid, s);
+    case 16384: return launch_one<MODE, REC, 8, 16384>(p, grid, s);
+    case 32768: return launch_one<MODE, REC, 8, 32768>(p, grid, s);
+    case 65536: return launch_one<MODE, REC, 8, 65536>(p, grid, s);
+    default: return hipErrorInvalidValue;
   }
 }
 
 }  // namespace
 
-hipError_t launch_trace(const TraceParams& p, uint32_t mode, bool records, uint32_t lds_stack, uint32_t grid,
-                        hipStream_t s) {
+uint32_t trace_block_size(uint32_t stage_bytes) { return stage_bytes ? 64u : 256u; }
+
+hipError_t launch_trace(const TraceParams& p, uint32_t mode, bool records, uint32_t lds_stack, uint32_t stage_bytes,
+                        uint32_t grid, hipStream_t s) {
   if (mode == RTX_MODE_PLAIN)
-    return records ? launch_k<0, true>(p, lds_stack, grid, s) : launch_k<0, false>(p, lds_stack, grid, s);
-  return records ? launch_k<1, true>(p, lds_stack, grid, s) : launch_k<1, false>(p, lds_stack, grid, s);
+    return records ? launch_k<0, true>(p, lds_stack, stage_bytes, grid, s)
+                   : launch_k<0, false>(p, lds_stack, stage_bytes, grid, s);
+  return records ? launch_k<1, true>(p, lds_stack, stage_bytes, grid, s)
+                 : launch_k<1, false>(p, lds_stack, stage_bytes, grid, s);
 }
 
 hipError_t launch_scan_records(const rtx_hit* hits, uint32_t n, uint64_t* offsets, uint64_t* tile_sums,

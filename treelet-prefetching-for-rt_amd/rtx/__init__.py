@@ -59,7 +59,7 @@ class RtxError(RuntimeError):
 class _TraceOptions(ctypes.Structure):
     _fields_ = [("mode", ctypes.c_uint32), ("prefetch", ctypes.c_uint32), ("threshold", ctypes.c_float),
                 ("emit_records", ctypes.c_uint32), ("waves_per_cu", ctypes.c_uint32),
-                ("lds_stack", ctypes.c_uint32), ("reserved", ctypes.c_uint32 * 2)]
+                ("lds_stack", ctypes.c_uint32), ("stage_bytes", ctypes.c_uint32), ("refill", ctypes.c_uint32)]
 
 
 class _PrefetchStats(ctypes.Structure):
@@ -388,10 +388,11 @@ class Context:
         self.summary = {"n_treelets": len(r), "budget_bytes": budget, "n_list_entries": int(lo[-1])}
 
     def trace_device(self, rays_buf, n, hits_buf, mode=MODE_PLAIN, prefetch=PREFETCH_NONE, threshold=0.5,
-                     records_buf=None, record_capacity=0, offsets_buf=None, waves_per_cu=0, lds_stack=0):
+                     records_buf=None, record_capacity=0, offsets_buf=None, waves_per_cu=0, lds_stack=0,
+                     stage_bytes=0, refill=0):
         """rtx_trace on device buffers; returns the rtx_status (0 = ok)."""
         o = _TraceOptions(mode, prefetch, threshold, 1 if offsets_buf is not None else 0, waves_per_cu, lds_stack,
-                          (ctypes.c_uint32 * 2)(0, 0))
+                          stage_bytes, refill)
         return _lib.rtx_trace(self._h, rays_buf.ptr, n, ctypes.byref(o), hits_buf.ptr,
                               records_buf.ptr if records_buf is not None else None, record_capacity,
                               offsets_buf.ptr if offsets_buf is not None else None)
