@@ -215,6 +215,38 @@ rtx_status rtx_trace(rtx_ctx* ctx, const rtx_ray* rays_dev, uint32_t n, const rt
                      rtx_hit* hits_dev, rtx_record* records_dev, uint64_t record_capacity,
                      uint64_t* record_offsets_dev);
 rtx_status rtx_get_prefetch_stats(rtx_ctx* ctx, rtx_prefetch_stats* out);
+
+/* ---- multi-scene batches -------------------------------------------------
+ * No counterpart in the reference, which traces one ray per call against the
+ * one TLAS of the launch.  A context also holds a list of scenes; a batch
+ * traces ray ranges of any of them in one persistent launch whose waves pull
+ * rays from a single global counter, so batches of very different sizes share
+ * the GPU without a launch per scene.  Batches write hits only (no records). */
+typedef struct rtx_batch_job {
+  uint32_t scene;          /* id returned by rtx_add_scene */
+  uint32_t n_rays;
+  const rtx_ray* rays_dev;
+  rtx_hit* hits_dev;
+} rtx_batch_job;
+
+typedef struct rtx_batch_graph rtx_batch_graph;
+
+rtx_status rtx_add_scene(rtx_ctx* ctx, const void* arena_host, size_t bytes, uint64_t tlas_offset,
+                         uint32_t* scene_id);
+rtx_status rtx_build_scene_treelets(rtx_ctx* ctx, uint32_t scene_id, uint32_t budget_bytes,
+                                    rtx_treelet_summary* out);
+/* opts->prefetch must be RTX_PREFETCH_NONE; opts->emit_records must be 0. At most 1024 jobs. */
+rtx_status rtx_trace_batch(rtx_ctx* ctx, const rtx_batch_job* jobs, uint32_t n_jobs, const rtx_trace_options* opts);
+/* Capture the batch launch (counter clear + persistent kernel) in a HIP graph. */
+rtx_status rtx_batch_graph_create(rtx_ctx* ctx, const rtx_batch_job* jobs, uint32_t n_jobs,
+                                  const rtx_trace_options* opts, rtx_batch_graph** out);
+/* Replay the graph `replays` times on the context's stream, wait, report kernel errors. */
+rtx_status rtx_batch_graph_launch(rtx_batch_graph* g, uint32_t replays, float* ms);
+rtx_status rtx_batch_graph_destroy(rtx_batch_graph* g);
+/* Time candidate launch shapes (waves per CU, LDS stack depth, refill threshold)
+ * on the batch and return the fastest in *best (mode taken from best->mode). */
+rtx_status rtx_autotune_batch(rtx_ctx* ctx, const rtx_batch_job* jobs, uint32_t n_jobs, rtx_trace_options* best,
+                              float* best_ms);
 /* Duration of the traversal kernel of the last rtx_trace call (the hit pass),
  * measured with HIP events recorded on the context's stream right around it. */
 rtx_status rtx_last_kernel_ms(rtx_ctx* ctx, float* ms);

@@ -43,6 +43,24 @@ struct rtx_ctx {
   uint64_t* scan_tmp = nullptr;
   size_t scan_tmp_entries = 0;
   rtx_prefetch_stats last_stats{};
+  // multi-scene batches
+  struct SceneSlot {
+    uint8_t* arena = nullptr;
+    uint64_t bytes = 0;
+    uint64_t tlas = 0;
+    Partition part;
+  };
+  std::vector<SceneSlot> scenes;
+  BatchJob* jobs_dev = nullptr;
+};
+
+struct rtx_batch_graph {
+  rtx_ctx* ctx = nullptr;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  BatchJob* jobs = nullptr;
+  uint64_t* spill = nullptr;
+  uint32_t* counters = nullptr;
 };
 
 namespace {
@@ -183,6 +201,11 @@ rtx_status rtx_destroy(rtx_ctx* c) {
   if (c->counters) hipFree(c->counters);
   if (c->stats) hipFree(c->stats);
   if (c->scan_tmp) hipFree(c->scan_tmp);
+  for (auto& sc : c->scenes) {
+    if (sc.arena) hipFree(sc.arena);
+    partition_free(sc.part);
+  }
+  if (c->jobs_dev) hipFree(c->jobs_dev);
   if (c->ev0) hipEventDestroy(c->ev0);
   if (c->ev1) hipEventDestroy(c->ev1);
   if (c->evk0) hipEventDestroy(c->evk0);
@@ -471,6 +494,248 @@ rtx_status rtx_last_kernel_ms(rtx_ctx* c, float* ms) {
 rtx_status rtx_get_prefetch_stats(rtx_ctx* c, rtx_prefetch_stats* out) {
   if (!c || !out) return RTX_ERR_INVALID_ARGUMENT;
   *out = c->last_stats;
+  return RTX_OK;
+}
+
+}  // extern "C"
+
+// ---- multi-scene batches -----------------------------------------------------
+namespace {
+
+constexpr uint32_t kBatchSpillCap = 128;
+
+rtx_status make_jobs(rtx_ctx* c, const rtx_batch_job* jobs, uint32_t n_jobs, uint32_t mode,
+                     std::vector<BatchJob>& out, uint64_t* total) {
+  if (!jobs || n_jobs == 0 || n_jobs > kMaxBatchJobs) return RTX_ERR_INVALID_ARGUMENT;
+  out.assign(n_jobs, BatchJob{});
+  uint64_t base = 0;
+  for (uint32_t j = 0; j < n_jobs; ++j) {
+    const rtx_batch_job& jb = jobs[j];
+    if (jb.scene >= c->scenes.size()) return RTX_ERR_INVALID_ARGUMENT;
+    const auto& sc = c->scenes[jb.scene];
+    if (mode == RTX_MODE_TREELET && !sc.part.valid) return RTX_ERR_NO_PARTITION;
+    if (jb.n_rays > 0 && (!jb.rays_dev || !jb.hits_dev)) return RTX_ERR_INVALID_ARGUMENT;
+    if (((uintptr_t)jb.rays_dev & 15) || ((uintptr_t)jb.hits_dev & 15)) return RTX_ERR_INVALID_ARGUMENT;
+    BatchJob& b = out[j];
+    b.arena = sc.arena;
+    b.arena_bytes = sc.bytes;
+    b.tlas = sc.tlas;
+    b.rays = jb.rays_dev;
+    b.hits = jb.hits_dev;
+    b.owner = sc.part.valid ? sc.part.owner : nullptr;
+    b.first_treelet = sc.part.valid ? sc.part.first_treelet : 0;
+    b.n = jb.n_rays;
+    b.ray_base = (uint32_t)base;
+    base += jb.n_rays;
+    if (base >= 0xffffffffull) return RTX_ERR_UNSUPPORTED;
+  }
+  *total = base;
+  return RTX_OK;
+}
+
+struct BatchShape {
+  uint32_t grid, lds_stack, refill, ns;
+};
+
+rtx_status batch_shape(const rtx_ctx* c, const rtx_trace_options* o, uint64_t total, BatchShape* s) {
+  if (!o || (o->mode != RTX_MODE_PLAIN && o->mode != RTX_MODE_TREELET)) return RTX_ERR_INVALID_ARGUMENT;
+  if (o->prefetch != RTX_PREFETCH_NONE || o->emit_records) return RTX_ERR_UNSUPPORTED;
+  s->lds_stack = o->lds_stack ? o->lds_stack : 8;
+  if (s->lds_stack != 4 && s->lds_stack != 8 && s->lds_stack != 16) return RTX_ERR_INVALID_ARGUMENT;
+  if (o->refill > 64) return RTX_ERR_INVALID_ARGUMENT;
+  s->refill = o->refill ? o->refill : 32;
+  const uint32_t waves = o->waves_per_cu ? o->waves_per_cu : 16;
+  uint32_t grid = (uint32_t)c->n_cus * std::max(1u, waves / 4);
+  grid = (uint32_t)std::min<uint64_t>(grid, (total + kBlock - 1) / kBlock);
+  s->grid = std::max(grid, 1u);
+  s->ns = o->mode == RTX_MODE_TREELET ? 2 : 1;
+  return RTX_OK;
+}
+
+TraceParams batch_params(rtx_ctx* c, const BatchJob* jobs_dev, uint32_t n_jobs, uint64_t total, const BatchShape& s,
+                         uint64_t* spill, uint32_t* counters) {
+  TraceParams p{};
+  p.n = (uint32_t)total;
+  p.refill = s.refill;
+  p.spill = spill;
+  p.spill_cap = kBatchSpillCap;
+  p.ray_counter = counters;
+  p.error = counters + 1;
+  p.stats = c->stats;
+  p.jobs = jobs_dev;
+  p.n_jobs = n_jobs;
+  return p;
+}
+
+}  // namespace
+
+extern "C" {
+
+rtx_status rtx_add_scene(rtx_ctx* c, const void* host, size_t bytes, uint64_t tlas, uint32_t* id) {
+  if (!c || !host || !id) return RTX_ERR_INVALID_ARGUMENT;
+  rtx_status st = check_scene_args(bytes, tlas);
+  if (st != RTX_OK) return st;
+  HIP_TRY(hipSetDevice(c->device));
+  rtx_ctx::SceneSlot sc;
+  if (hipMalloc(&sc.arena, bytes) != hipSuccess) {
+    (void)hipGetLastError();
+    return RTX_ERR_OUT_OF_MEMORY;
+  }
+  if (hipMemcpy(sc.arena, host, bytes, hipMemcpyHostToDevice) != hipSuccess) {
+    hipFree(sc.arena);
+    return RTX_ERR_HIP;
+  }
+  sc.bytes = bytes;
+  sc.tlas = tlas;
+  c->scenes.push_back(sc);
+  *id = (uint32_t)(c->scenes.size() - 1);
+  return RTX_OK;
+}
+
+rtx_status rtx_build_scene_treelets(rtx_ctx* c, uint32_t id, uint32_t budget, rtx_treelet_summary* out) {
+  if (!c || id >= c->scenes.size()) return RTX_ERR_INVALID_ARGUMENT;
+  if (budget < 128 || budget % 64 != 0 || budget > (1u << 20)) return RTX_ERR_INVALID_ARGUMENT;
+  HIP_TRY(hipSetDevice(c->device));
+  auto& sc = c->scenes[id];
+  rtx_status st = partition_build(sc.arena, sc.bytes, sc.tlas, budget, c->stream, sc.part);
+  if (st != RTX_OK) return st;
+  if (out) *out = sc.part.summary;
+  return RTX_OK;
+}
+
+rtx_status rtx_trace_batch(rtx_ctx* c, const rtx_batch_job* jobs, uint32_t n_jobs, const rtx_trace_options* o) {
+  if (!c || !o) return RTX_ERR_INVALID_ARGUMENT;
+  std::vector<BatchJob> hj;
+  uint64_t total = 0;
+  rtx_status st = make_jobs(c, jobs, n_jobs, o->mode, hj, &total);
+  if (st != RTX_OK) return st;
+  BatchShape s;
+  st = batch_shape(c, o, total, &s);
+  if (st != RTX_OK) return st;
+  HIP_TRY(hipSetDevice(c->device));
+  if (total == 0) return RTX_OK;
+  if (!c->jobs_dev) HIP_TRY(hipMalloc(&c->jobs_dev, kMaxBatchJobs * sizeof(BatchJob)));
+  st = ensure_spill(c, (size_t)s.grid * kBlock * kBatchSpillCap * s.ns);
+  if (st != RTX_OK) return st;
+  HIP_TRY(hipMemcpyAsync(c->jobs_dev, hj.data(), n_jobs * sizeof(BatchJob), hipMemcpyHostToDevice, c->stream));
+  HIP_TRY(hipMemsetAsync(c->counters, 0, 2 * sizeof(uint32_t), c->stream));
+  TraceParams p = batch_params(c, c->jobs_dev, n_jobs, total, s, c->spill, c->counters);
+  c->kernel_timed = false;
+  HIP_TRY(hipEventRecord(c->evk0, c->stream));
+  HIP_TRY(launch_trace_batch(p, o->mode, s.lds_stack, s.grid, c->stream));
+  HIP_TRY(hipEventRecord(c->evk1, c->stream));
+  c->kernel_timed = true;
+  uint32_t err = 0;
+  HIP_TRY(hipMemcpyAsync(&err, c->counters + 1, sizeof(uint32_t), hipMemcpyDeviceToHost, c->stream));
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  return kernel_error_status(err);
+}
+
+rtx_status rtx_batch_graph_create(rtx_ctx* c, const rtx_batch_job* jobs, uint32_t n_jobs,
+                                  const rtx_trace_options* o, rtx_batch_graph** out) {
+  if (!c || !o || !out) return RTX_ERR_INVALID_ARGUMENT;
+  *out = nullptr;
+  std::vector<BatchJob> hj;
+  uint64_t total = 0;
+  rtx_status st = make_jobs(c, jobs, n_jobs, o->mode, hj, &total);
+  if (st != RTX_OK) return st;
+  if (total == 0) return RTX_ERR_INVALID_ARGUMENT;
+  BatchShape s;
+  st = batch_shape(c, o, total, &s);
+  if (st != RTX_OK) return st;
+  HIP_TRY(hipSetDevice(c->device));
+  rtx_batch_graph* g = new (std::nothrow) rtx_batch_graph();
+  if (!g) return RTX_ERR_OUT_OF_MEMORY;
+  g->ctx = c;
+  auto fail = [&](rtx_status e) {
+    rtx_batch_graph_destroy(g);
+    return e;
+  };
+  if (hipMalloc(&g->jobs, n_jobs * sizeof(BatchJob)) != hipSuccess ||
+      hipMalloc(&g->spill, (size_t)s.grid * kBlock * kBatchSpillCap * s.ns * sizeof(uint64_t)) != hipSuccess ||
+      hipMalloc(&g->counters, 64 * sizeof(uint32_t)) != hipSuccess) {
+    (void)hipGetLastError();
+    return fail(RTX_ERR_OUT_OF_MEMORY);
+  }
+  if (hipMemcpy(g->jobs, hj.data(), n_jobs * sizeof(BatchJob), hipMemcpyHostToDevice) != hipSuccess)
+    return fail(RTX_ERR_HIP);
+  if (hipStreamSynchronize(c->stream) != hipSuccess) return fail(RTX_ERR_HIP);
+  TraceParams p = batch_params(c, g->jobs, n_jobs, total, s, g->spill, g->counters);
+  if (hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal) != hipSuccess) return fail(RTX_ERR_HIP);
+  // counters are cleared by a kernel: no memset node inside the graph
+  hipError_t e1 = launch_fill_zero(g->counters, 2, c->stream);
+  hipError_t e2 = e1 == hipSuccess ? launch_trace_batch(p, o->mode, s.lds_stack, s.grid, c->stream) : e1;
+  hipError_t e3 = hipStreamEndCapture(c->stream, &g->graph);
+  if (e1 != hipSuccess || e2 != hipSuccess || e3 != hipSuccess) return fail(RTX_ERR_HIP);
+  if (hipGraphInstantiate(&g->exec, g->graph, nullptr, nullptr, 0) != hipSuccess) return fail(RTX_ERR_HIP);
+  *out = g;
+  return RTX_OK;
+}
+
+rtx_status rtx_batch_graph_launch(rtx_batch_graph* g, uint32_t replays, float* ms) {
+  if (!g || !g->exec || replays == 0) return RTX_ERR_INVALID_ARGUMENT;
+  rtx_ctx* c = g->ctx;
+  HIP_TRY(hipSetDevice(c->device));
+  HIP_TRY(hipEventRecord(c->ev0, c->stream));
+  for (uint32_t r = 0; r < replays; ++r) HIP_TRY(hipGraphLaunch(g->exec, c->stream));
+  HIP_TRY(hipEventRecord(c->ev1, c->stream));
+  uint32_t err = 0;
+  HIP_TRY(hipMemcpyAsync(&err, g->counters + 1, sizeof(uint32_t), hipMemcpyDeviceToHost, c->stream));
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  if (ms) HIP_TRY(hipEventElapsedTime(ms, c->ev0, c->ev1));
+  return kernel_error_status(err);
+}
+
+rtx_status rtx_batch_graph_destroy(rtx_batch_graph* g) {
+  if (!g) return RTX_ERR_INVALID_ARGUMENT;
+  if (g->ctx) {
+    hipSetDevice(g->ctx->device);
+    hipStreamSynchronize(g->ctx->stream);
+  }
+  if (g->exec) hipGraphExecDestroy(g->exec);
+  if (g->graph) hipGraphDestroy(g->graph);
+  if (g->jobs) hipFree(g->jobs);
+  if (g->spill) hipFree(g->spill);
+  if (g->counters) hipFree(g->counters);
+  delete g;
+  return RTX_OK;
+}
+
+rtx_status rtx_autotune_batch(rtx_ctx* c, const rtx_batch_job* jobs, uint32_t n_jobs, rtx_trace_options* best,
+                              float* best_ms) {
+  if (!c || !best) return RTX_ERR_INVALID_ARGUMENT;
+  const uint32_t waves[] = {8, 16, 24, 32};
+  const uint32_t stacks[] = {4, 8, 16};
+  const uint32_t refills[] = {32, 48, 64};
+  rtx_trace_options o = *best;
+  o.prefetch = RTX_PREFETCH_NONE;
+  o.emit_records = 0;
+  float best_t = 1e30f;
+  rtx_trace_options winner = o;
+  for (uint32_t w : waves)
+    for (uint32_t k : stacks)
+      for (uint32_t r : refills) {
+        o.waves_per_cu = w;
+        o.lds_stack = k;
+        o.refill = r;
+        rtx_status st = rtx_trace_batch(c, jobs, n_jobs, &o);  // warm-up
+        if (st != RTX_OK) return st;
+        float t = 0.f;
+        for (int rep = 0; rep < 2; ++rep) {
+          st = rtx_trace_batch(c, jobs, n_jobs, &o);
+          if (st != RTX_OK) return st;
+          float ms = 0.f;
+          st = rtx_last_kernel_ms(c, &ms);
+          if (st != RTX_OK) return st;
+          t += ms;
+        }
+        if (t < best_t) {
+          best_t = t;
+          winner = o;
+        }
+      }
+  *best = winner;
+  if (best_ms) *best_ms = best_t / 2;
   return RTX_OK;
 }
 

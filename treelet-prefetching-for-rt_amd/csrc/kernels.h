@@ -21,6 +21,21 @@ enum : uint32_t {
 
 constexpr uint32_t kBlock = 256;        // threads per block (4 waves of 64)
 constexpr uint32_t kMaxStepsPerRay = 1u << 24;
+constexpr uint32_t kMaxBatchJobs = 1024;
+
+// One entry of a multi-scene batch: a ray range of one scene (64 bytes).
+struct alignas(64) BatchJob {
+  const uint8_t* arena;
+  uint64_t arena_bytes;
+  uint64_t tlas;
+  const rtx_ray* rays;
+  rtx_hit* hits;
+  const uint32_t* owner;
+  uint32_t first_treelet;
+  uint32_t n;
+  uint32_t ray_base;  // rays of all earlier jobs
+  uint32_t pad[3];
+};
 
 struct TraceParams {
   const uint8_t* arena;
@@ -47,12 +62,19 @@ struct TraceParams {
   const uint64_t* image_offset;      // dense treelet -> byte offset of its image in `images`
   const uint32_t* image_bytes;       // dense treelet -> image size
   const uint8_t* images;             // treelet-contiguous copies of the arena
+  // multi-scene batch (batch kernels only): rays are numbered across jobs
+  const BatchJob* jobs;
+  uint32_t n_jobs;
 };
 
 // Launchers (defined in the .hip files).
 uint32_t trace_block_size(uint32_t stage_bytes);
 hipError_t launch_trace(const TraceParams& p, uint32_t mode, bool records, uint32_t lds_stack, uint32_t stage_bytes,
                         uint32_t grid, hipStream_t s);
+hipError_t launch_trace_batch(const TraceParams& p, uint32_t mode, uint32_t lds_stack, uint32_t grid,
+                              hipStream_t s);
+// Clears `words` 32-bit words with a kernel (usable inside graph capture).
+hipError_t launch_fill_zero(uint32_t* dst, uint32_t words, hipStream_t s);
 hipError_t launch_scan_records(const rtx_hit* hits, uint32_t n, uint64_t* offsets, uint64_t* block_tmp,
                                hipStream_t s);
 

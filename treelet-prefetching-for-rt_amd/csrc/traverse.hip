@@ -90,8 +90,9 @@ __device__ __forceinline__ uint32_t wave_sum(uint32_t v) {
   return v;
 }
 
-template <int MODE, bool REC, int K, int STAGE>
+template <int MODE, bool REC, int K, int STAGE, bool MULTI = false>
 __global__ __launch_bounds__(STAGE ? 64 : 256) void trace_kernel(TraceParams p) {
+  static_assert(!MULTI || (!REC && STAGE == 0), "batch kernels write hits only");
   constexpr uint32_t BS = STAGE ? 64u : 256u;
   constexpr int NS = MODE == 1 ? 2 : 1;
   constexpr uint32_t UNITS = STAGE / 64;
@@ -109,8 +110,20 @@ __global__ __launch_bounds__(STAGE ? 64 : 256) void trace_kernel(TraceParams p) 
   const uint64_t lane_slot = (uint64_t)blockIdx.x * BS + tid;
   uint64_t* gA = p.spill + lane_slot * (uint64_t)p.spill_cap * NS;
   uint64_t* gB = gA + (NS - 1) * (uint64_t)p.spill_cap;
+  // Scene of the lane's current ray (a batch kernel switches it per ray).
   const uint8_t* __restrict__ A = p.arena;
-  const uint64_t bytes = p.arena_bytes;
+  uint64_t bytes = p.arena_bytes;
+  uint64_t tlas = p.tlas;
+  const uint32_t* owner = p.owner;
+  uint32_t first_treelet = p.first_treelet;
+  rtx_hit* hits_out = p.hits;
+  const rtx_ray* rays_in = p.rays;
+  uint32_t local_id = 0;
+  __shared__ uint32_t job_base[MULTI ? kMaxBatchJobs + 1 : 1];
+  if constexpr (MULTI) {
+    for (uint32_t j = threadIdx.x; j <= p.n_jobs; j += BS) job_base[j] = j < p.n_jobs ? p.jobs[j].ray_base : p.n;
+    __syncthreads();
+  }
 
   Stack sa{0, 0, 0}, sb{0, 0, 0};
   bool active = false, exhausted = false;
@@ -148,7 +161,7 @@ __global__ __launch_bounds__(STAGE ? 64 : 256) void trace_kernel(TraceParams p) 
     if (REC) {
       uint64_t idx = rec_base + n_rec;
       if (idx < p.record_capacity) {
-        uint32_t tl = p.owner ? p.owner[off >> 6] : RTX_NO_TREELET;
+        uint32_t tl = owner ? owner[off >> 6] : RTX_NO_TREELET;
         uint4 v;
         v.x = (uint32_t)off;
         v.y = (uint32_t)(off >> 32);
@@ -175,7 +188,7 @@ __global__ __launch_bounds__(STAGE ? 64 : 256) void trace_kernel(TraceParams p) 
   auto goes_current = [&](uint32_t slot) -> bool {
     if (MODE == 0) return true;
     if (phase2) return false;
-    return p.owner[slot] == p.first_treelet;
+    return owner[slot] == first_treelet;
   };
 
   // Position of `slot` in the staged image, or kNone.
@@ -306,7 +319,28 @@ __global__ __launch_bounds__(STAGE ? 64 : 256) void trace_kernel(TraceParams p) 
         if (r < p.n) {
           ray_id = r;
           active = true;
-          const uint4* rq = reinterpret_cast<const uint4*>(p.rays + r);
+          local_id = r;
+          if constexpr (MULTI) {
+            // job holding ray r: job_base[j] <= r < job_base[j + 1]
+            uint32_t lo = 0, hi = p.n_jobs;
+            while (hi - lo > 1) {
+              const uint32_t mid = (lo + hi) >> 1;
+              if (job_base[mid] <= r)
+                lo = mid;
+              else
+                hi = mid;
+            }
+            const BatchJob& jb = p.jobs[lo];
+            A = jb.arena;
+            bytes = jb.arena_bytes;
+            tlas = jb.tlas;
+            owner = jb.owner;
+            first_treelet = jb.first_treelet;
+            hits_out = jb.hits;
+            rays_in = jb.rays;
+            local_id = r - job_base[lo];
+          }
+          const uint4* rq = reinterpret_cast<const uint4*>(rays_in + local_id);
           uint4 r0 = rq[0], r1 = rq[1], r2 = rq[2];
           ro = mk3(__uint_as_float(r0.x), __uint_as_float(r0.y), __uint_as_float(r0.z));
           rtmin = __uint_as_float(r0.w);
@@ -327,7 +361,7 @@ __global__ __launch_bounds__(STAGE ? 64 : 256) void trace_kernel(TraceParams p) 
           phase2 = false;
           err = 0;
           // TLAS header record and bounds test (vulkan_ray_tracing.cc:2444-2484)
-          const uint64_t t = p.tlas;
+          const uint64_t t = tlas;
           if (t + 64 > bytes) {
             err |= kErrRange;
           } else {
@@ -390,7 +424,7 @@ __global__ __launch_bounds__(STAGE ? 64 : 256) void trace_kernel(TraceParams p) 
         } else {
           o2.w = n_rec;
         }
-        uint4* hq = reinterpret_cast<uint4*>(p.hits + ray_id);
+        uint4* hq = reinterpret_cast<uint4*>(hits_out + local_id);
         hq[0] = o0;
         hq[1] = o1;
         hq[2] = o2;
@@ -419,7 +453,7 @@ __global__ __launch_bounds__(STAGE ? 64 : 256) void trace_kernel(TraceParams p) 
           cooldown--;
         } else if (nact > 0 && 2 * nmiss > nact) {
           uint32_t t = kNone;
-          if (want && pos == kNone && off + 64 <= bytes) t = p.owner[slot];
+          if (want && pos == kNone && off + 64 <= bytes) t = owner[slot];
           uint64_t rem = __ballot(t != kNone);
           uint32_t best = kNone, bestc = 0;
           for (int c = 0; c < 4 && rem; ++c) {
@@ -723,7 +757,41 @@ hipError_t launch_k(const TraceParams& p, uint32_t lds_stack, uint32_t stage, ui
   }
 }
 
+template <int MODE, int K>
+hipError_t launch_batch_one(const TraceParams& p, uint32_t grid, hipStream_t s) {
+  hipLaunchKernelGGL((trace_kernel<MODE, false, K, 0, true>), dim3(grid), dim3(256), 0, s, p);
+  return hipGetLastError();
+}
+
+__global__ void fill_zero_kernel(uint32_t* dst, uint32_t words) {
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < words; i += gridDim.x * blockDim.x) dst[i] = 0u;
+}
+
 }  // namespace
+
+hipError_t launch_trace_batch(const TraceParams& p, uint32_t mode, uint32_t lds_stack, uint32_t grid,
+                              hipStream_t s) {
+  if (p.n_jobs == 0 || p.n_jobs > kMaxBatchJobs) return hipErrorInvalidValue;
+  if (mode == RTX_MODE_PLAIN) {
+    switch (lds_stack) {
+      case 4: return launch_batch_one<0, 4>(p, grid, s);
+      case 8: return launch_batch_one<0, 8>(p, grid, s);
+      case 16: return launch_batch_one<0, 16>(p, grid, s);
+      default: return hipErrorInvalidValue;
+    }
+  }
+  switch (lds_stack) {
+    case 4: return launch_batch_one<1, 4>(p, grid, s);
+    case 8: return launch_batch_one<1, 8>(p, grid, s);
+    case 16: return launch_batch_one<1, 16>(p, grid, s);
+    default: return hipErrorInvalidValue;
+  }
+}
+
+hipError_t launch_fill_zero(uint32_t* dst, uint32_t words, hipStream_t s) {
+  hipLaunchKernelGGL(fill_zero_kernel, dim3(1), dim3(64), 0, s, dst, words);
+  return hipGetLastError();
+}
 
 uint32_t trace_block_size(uint32_t stage_bytes) { return stage_bytes ? 64u : 256u; }
 

@@ -73,6 +73,11 @@ class _TreeletSummary(ctypes.Structure):
                 ("max_treelet_bytes", ctypes.c_uint32), ("max_treelet_nodes", ctypes.c_uint32)]
 
 
+class _BatchJob(ctypes.Structure):
+    _fields_ = [("scene", ctypes.c_uint32), ("n_rays", ctypes.c_uint32), ("rays_dev", ctypes.c_void_p),
+                ("hits_dev", ctypes.c_void_p)]
+
+
 class _WorkloadDesc(ctypes.Structure):
     _fields_ = [("kind", ctypes.c_uint32), ("n_instances", ctypes.c_uint32), ("sphere_slices", ctypes.c_uint32),
                 ("sphere_stacks", ctypes.c_uint32), ("shared_blas", ctypes.c_uint32), ("width", ctypes.c_uint32),
@@ -116,6 +121,13 @@ _sig("rtx_set_treelets", _i32, _vp, _u32, _u32, _vp, _vp, _vp, _vp)
 _sig("rtx_trace", _i32, _vp, _vp, _u32, _P(_TraceOptions), _vp, _vp, _u64, _vp)
 _sig("rtx_get_prefetch_stats", _i32, _vp, _P(_PrefetchStats))
 _sig("rtx_last_kernel_ms", _i32, _vp, _P(_f32))
+_sig("rtx_add_scene", _i32, _vp, _vp, ctypes.c_size_t, _u64, _P(_u32))
+_sig("rtx_build_scene_treelets", _i32, _vp, _u32, _u32, _P(_TreeletSummary))
+_sig("rtx_trace_batch", _i32, _vp, _P(_BatchJob), _u32, _P(_TraceOptions))
+_sig("rtx_batch_graph_create", _i32, _vp, _P(_BatchJob), _u32, _P(_TraceOptions), _P(_vp))
+_sig("rtx_batch_graph_launch", _i32, _vp, _u32, _P(_f32))
+_sig("rtx_batch_graph_destroy", _i32, _vp)
+_sig("rtx_autotune_batch", _i32, _vp, _P(_BatchJob), _u32, _P(_TraceOptions), _P(_f32))
 _sig("rtx_builder_create", _i32, _P(_vp))
 _sig("rtx_builder_destroy", _i32, _vp)
 _sig("rtx_builder_add_blas", _i32, _vp, _vp, _u32, _u32, _P(_u32))
@@ -136,7 +148,8 @@ EXPORTED_SYMBOLS = [
     "rtx_malloc", "rtx_free", "rtx_memcpy_h2d", "rtx_memcpy_d2h", "rtx_memset", "rtx_timer_start",
     "rtx_timer_stop", "rtx_device_info", "rtx_set_scene", "rtx_set_scene_device", "rtx_validate_scene",
     "rtx_build_treelets", "rtx_get_treelets", "rtx_set_treelets", "rtx_trace", "rtx_get_prefetch_stats",
-    "rtx_last_kernel_ms",
+    "rtx_last_kernel_ms", "rtx_add_scene", "rtx_build_scene_treelets", "rtx_trace_batch", "rtx_batch_graph_create",
+    "rtx_batch_graph_launch", "rtx_batch_graph_destroy", "rtx_autotune_batch",
     "rtx_builder_create", "rtx_builder_destroy", "rtx_builder_add_blas", "rtx_builder_add_instance",
     "rtx_builder_finish", "rtx_builder_copy_arena", "rtx_builder_arena_info", "rtx_builder_counts", "rtx_builder_blas_offset",
     "rtx_mesh_displaced_sphere", "rtx_workload_scene", "rtx_workload_primary_rays", "rtx_workload_secondary_rays",
@@ -307,6 +320,29 @@ class DeviceBuffer:
             pass
 
 
+class BatchGraph:
+    """A captured multi-scene batch launch (rtx_batch_graph)."""
+
+    def __init__(self, handle):
+        self._h = handle
+
+    def launch(self, replays=1):
+        ms = ctypes.c_float(0)
+        st = _lib.rtx_batch_graph_launch(self._h, replays, ctypes.byref(ms))
+        return st, ms.value
+
+    def close(self):
+        if self._h:
+            _lib.rtx_batch_graph_destroy(self._h)
+            self._h = ctypes.c_void_p(None)
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 class Context:
     """One rtx_ctx on a device (VulkanRayTracing's static state, per context)."""
 
@@ -420,6 +456,52 @@ class Context:
         s = _PrefetchStats()
         _check(_lib.rtx_get_prefetch_stats(self._h, ctypes.byref(s)), "rtx_get_prefetch_stats")
         return {f: getattr(s, f) for f, _ in _PrefetchStats._fields_}
+
+    # ---- multi-scene batches ----------------------------------------------------
+    def add_scene(self, arena, tlas_offset=0):
+        a = np.ascontiguousarray(arena, dtype=np.uint8)
+        sid = ctypes.c_uint32(0)
+        _check(_lib.rtx_add_scene(self._h, _ptr(a), a.nbytes, tlas_offset, ctypes.byref(sid)), "rtx_add_scene")
+        return sid.value
+
+    def build_scene_treelets(self, scene_id, budget_bytes):
+        s = _TreeletSummary()
+        _check(_lib.rtx_build_scene_treelets(self._h, scene_id, budget_bytes, ctypes.byref(s)),
+               "rtx_build_scene_treelets")
+        return {f: getattr(s, f) for f, _ in _TreeletSummary._fields_}
+
+    @staticmethod
+    def _jobs(jobs):
+        """jobs: list of (scene_id, rays DeviceBuffer, n_rays, hits DeviceBuffer)."""
+        arr = (_BatchJob * len(jobs))()
+        for k, (sid, rb, n, hb) in enumerate(jobs):
+            arr[k] = _BatchJob(sid, n, rb.ptr.value if n else None, hb.ptr.value if n else None)
+        return arr
+
+    @staticmethod
+    def _opts(mode=MODE_PLAIN, waves_per_cu=0, lds_stack=0, refill=0):
+        return _TraceOptions(mode, PREFETCH_NONE, 0.0, 0, waves_per_cu, lds_stack, 0, refill)
+
+    def trace_batch(self, jobs, **opts):
+        arr = self._jobs(jobs)
+        o = self._opts(**opts)
+        return _lib.rtx_trace_batch(self._h, arr, len(jobs), ctypes.byref(o))
+
+    def batch_graph(self, jobs, **opts):
+        arr = self._jobs(jobs)
+        o = self._opts(**opts)
+        g = ctypes.c_void_p()
+        _check(_lib.rtx_batch_graph_create(self._h, arr, len(jobs), ctypes.byref(o), ctypes.byref(g)),
+               "rtx_batch_graph_create")
+        return BatchGraph(g)
+
+    def autotune_batch(self, jobs, mode=MODE_PLAIN):
+        arr = self._jobs(jobs)
+        o = self._opts(mode=mode)
+        ms = ctypes.c_float(0)
+        _check(_lib.rtx_autotune_batch(self._h, arr, len(jobs), ctypes.byref(o), ctypes.byref(ms)),
+               "rtx_autotune_batch")
+        return {"mode": o.mode, "waves_per_cu": o.waves_per_cu, "lds_stack": o.lds_stack, "refill": o.refill}, ms.value
 
     def last_kernel_ms(self):
         ms = ctypes.c_float(0)
