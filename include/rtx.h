@@ -125,7 +125,8 @@ typedef struct rtx_record {
   uint32_t treelet; /* dense treelet index of the node's owner, or RTX_NO_TREELET */
   uint16_t size;    /* bytes: 64, 128 or 8 */
   uint8_t type;     /* RTX_TX_* */
-  uint8_t reserved;
+  uint8_t level;    /* tree level of the node (tree_level_map, vulkan_ray_tracing.cc:2469,2569,2659):
+                       0 for the TLAS header, saturating at 255 */
 } rtx_record; /* 16 bytes */
 
 typedef struct rtx_trace_options {

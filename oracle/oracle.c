@@ -606,7 +606,7 @@ typedef struct {
   int overflow;
 } sink_t;
 
-static void emit(sink_t* s, uint64_t off, uint32_t size, uint32_t type) {
+static void emit(sink_t* s, uint64_t off, uint32_t size, uint32_t type, uint32_t level) {
   if (type != TX_DESC) s->fetch_bytes += size;
   if (s->out) {
     uint64_t idx = s->base + s->count;
@@ -617,7 +617,7 @@ static void emit(sink_t* s, uint64_t off, uint32_t size, uint32_t type) {
       if (s->part && off / 64 < s->part->arena_bytes / 64) r.treelet = s->part->owner[off / 64];
       r.size = (uint16_t)size;
       r.type = (uint8_t)type;
-      r.reserved = 0;
+      r.level = (uint8_t)(level > 255 ? 255 : level);
       s->out[idx] = r;
     } else {
       s->overflow = 1;
@@ -711,7 +711,7 @@ static int trace_plain(sink_t* s, const uint64_t tlas, const orc_ray* in, orc_hi
 
   CHECK_RANGE(tlas, 64);
   hdr_t top = unpack_hdr(A + tlas);
-  emit(s, tlas, 64, TX_STRUCT); /* :2444-2449 */
+  emit(s, tlas, 64, TX_STRUCT, 0); /* :2444-2449 */
   uint64_t topRoot = tlas + top.root_offset;
   level_set(ts, 1); /* tree_level_map[topRootAddr] = 1 (:2469) */
   {
@@ -747,7 +747,7 @@ static int trace_plain(sink_t* s, const uint64_t tlas, const orc_ray* in, orc_hi
       have_next = 0;
       CHECK_RANGE(node_addr, 64);
       inode_t node = unpack_inode(A + node_addr);
-      emit(s, node_addr, 64, TX_INTERNAL);
+      emit(s, node_addr, 64, TX_INTERNAL, node_level);
       n_nodes++;
       int child_hit[6];
       float thit[6];
@@ -789,13 +789,13 @@ static int trace_plain(sink_t* s, const uint64_t tlas, const orc_ray* in, orc_hi
       CHECK_RANGE(leaf_addr, 128);
       instanceLeaf = unpack_ileaf(A + leaf_addr);
       inst_addr = leaf_addr;
-      emit(s, leaf_addr, 128, TX_INSTANCE);
+      emit(s, leaf_addr, 128, TX_INSTANCE, le.level);
       n_nodes++;
       f4x4 w2o = leaf_matrix(instanceLeaf.w2o);
       uint64_t blas = leaf_addr + instanceLeaf.bvh_address;
       CHECK_RANGE(blas, 64);
       hdr_t bh = unpack_hdr(A + blas);
-      emit(s, blas, 64, TX_STRUCT);
+      emit(s, blas, 64, TX_STRUCT, le.level);
       objectRay = make_transformed_ray(&ray, &w2o, &tMult);
       uint64_t botRoot = blas + bh.root_offset;
       if (!ss_push(&st, (sent){botRoot, 0, 0, le.level})) goto oom;
@@ -812,7 +812,7 @@ static int trace_plain(sink_t* s, const uint64_t tlas, const orc_ray* in, orc_hi
           bhave = 0;
           CHECK_RANGE(node_addr, 64);
           inode_t node = unpack_inode(A + node_addr);
-          emit(s, node_addr, 64, TX_INTERNAL);
+          emit(s, node_addr, 64, TX_INTERNAL, node_level);
           n_nodes++;
           int child_hit[6];
           float thit[6];
@@ -845,9 +845,11 @@ static int trace_plain(sink_t* s, const uint64_t tlas, const orc_ray* in, orc_hi
         }
 
         while (st.n > 0 && !st.e[st.n - 1].top && st.e[st.n - 1].leaf) { /* bottom leaves, :2789-2985 */
-          uint64_t leaf = st.e[--st.n].addr;
+          const sent leaf_e = st.e[--st.n];
+          const uint64_t leaf = leaf_e.addr;
+          const uint32_t leaf_level = leaf_e.level;
           CHECK_RANGE(leaf, 64);
-          emit(s, leaf, 8, TX_DESC);
+          emit(s, leaf, 8, TX_DESC, leaf_level);
           qleaf_t q = unpack_qleaf(A + leaf);
           if (q.leaf_type == 0) {
             if (q.prim1_delta != 0) {
@@ -863,16 +865,16 @@ static int trace_plain(sink_t* s, const uint64_t tlas, const orc_ray* in, orc_hi
               closest_leaf = leaf;
               closest_inst = inst_addr;
               closest_obj = objectRay;
-              emit(s, leaf, 64, TX_QUAD_HIT);
+              emit(s, leaf, 64, TX_QUAD_HIT, leaf_level);
               n_nodes++;
               if (!skipAnyHitShader) n_any++;
               if (terminateOnFirstHit) st.n = 0;
             } else {
-              emit(s, leaf, 64, TX_QUAD);
+              emit(s, leaf, 64, TX_QUAD, leaf_level);
               n_nodes++;
             }
           } else {
-            emit(s, leaf, 64, TX_PROC); /* procedural leaf: intersection shader not modelled */
+            emit(s, leaf, 64, TX_PROC, leaf_level); /* procedural leaf: intersection shader not modelled */
             n_nodes++;
           }
         }
@@ -965,7 +967,7 @@ static int trace_treelet(sink_t* s, const uint64_t tlas, const orc_ray* in, orc_
 
   CHECK_RANGE(tlas, 64);
   hdr_t top = unpack_hdr(A + tlas);
-  emit(s, tlas, 64, TX_STRUCT);
+  emit(s, tlas, 64, TX_STRUCT, 0);
   uint64_t topRoot = tlas + top.root_offset;
   /* current_treelet_root starts as the TLAS header's device address (:1707).
    * Once a node is moved from the other list, the reference stores that
@@ -1002,7 +1004,7 @@ static int trace_treelet(sink_t* s, const uint64_t tlas, const orc_ray* in, orc_
     if (cn.top && !cn.leaf) { /* :1758-1870 */
       CHECK_RANGE(cn.addr, 64);
       inode_t node = unpack_inode(A + cn.addr);
-      emit(s, cn.addr, 64, TX_INTERNAL);
+      emit(s, cn.addr, 64, TX_INTERNAL, cn.level);
       n_nodes++;
       int child_hit[6];
       float thit[6];
@@ -1041,13 +1043,13 @@ static int trace_treelet(sink_t* s, const uint64_t tlas, const orc_ray* in, orc_
     } else if (cn.top && cn.leaf) { /* :1872-1955 */
       CHECK_RANGE(cn.addr, 128);
       ileaf_t il = unpack_ileaf(A + cn.addr);
-      emit(s, cn.addr, 128, TX_INSTANCE);
+      emit(s, cn.addr, 128, TX_INSTANCE, cn.level);
       n_nodes++;
       f4x4 w2o = leaf_matrix(il.w2o);
       uint64_t blas = cn.addr + il.bvh_address;
       CHECK_RANGE(blas, 64);
       hdr_t bh = unpack_hdr(A + blas);
-      emit(s, blas, 64, TX_STRUCT);
+      emit(s, blas, 64, TX_STRUCT, cn.level);
       tent e = z;
       e.obj = make_transformed_ray(&ray, &w2o, &e.tmult);
       e.addr = blas + bh.root_offset;
@@ -1065,7 +1067,7 @@ static int trace_treelet(sink_t* s, const uint64_t tlas, const orc_ray* in, orc_
     } else if (!cn.top && !cn.leaf) { /* :1957-2072 */
       CHECK_RANGE(cn.addr, 64);
       inode_t node = unpack_inode(A + cn.addr);
-      emit(s, cn.addr, 64, TX_INTERNAL);
+      emit(s, cn.addr, 64, TX_INTERNAL, cn.level);
       n_nodes++;
       int child_hit[6];
       float thit[6];
@@ -1098,7 +1100,7 @@ static int trace_treelet(sink_t* s, const uint64_t tlas, const orc_ray* in, orc_
       }
     } else { /* bottom leaf, :2073-2203 */
       CHECK_RANGE(cn.addr, 64);
-      emit(s, cn.addr, 8, TX_DESC);
+      emit(s, cn.addr, 8, TX_DESC, cn.level);
       qleaf_t q = unpack_qleaf(A + cn.addr);
       if (q.leaf_type == 0) {
         if (q.prim1_delta != 0) {
@@ -1114,18 +1116,18 @@ static int trace_treelet(sink_t* s, const uint64_t tlas, const orc_ray* in, orc_
           closest_leaf = cn.addr;
           closest_inst = cn.inst;
           closest_obj = cn.obj;
-          emit(s, cn.addr, 64, TX_QUAD_HIT);
+          emit(s, cn.addr, 64, TX_QUAD_HIT, cn.level);
           n_nodes++;
           if (terminateOnFirstHit) {
             cur->head = cur->tail;
             other->head = other->tail;
           }
         } else {
-          emit(s, cn.addr, 64, TX_QUAD);
+          emit(s, cn.addr, 64, TX_QUAD, cn.level);
           n_nodes++;
         }
       } else {
-        emit(s, cn.addr, 64, TX_PROC);
+        emit(s, cn.addr, 64, TX_PROC, cn.level);
         n_nodes++;
       }
     }

@@ -50,7 +50,7 @@ typedef struct orc_record {
   uint32_t treelet;
   uint16_t size;
   uint8_t type;
-  uint8_t reserved;
+  uint8_t level; /* tree level of the node (tree_level_map), 0 for the TLAS header, saturating at 255 */
 } orc_record;
 
 typedef struct orc_partition orc_partition;

@@ -31,7 +31,7 @@ HIT_DTYPE = np.dtype([("hit", "<u4"), ("primitive_index", "<u4"), ("geometry_ind
                       ("n_records", "<u4"), ("n_nodes", "<u4"), ("fetch_bytes", "<u4"), ("n_any_hits", "<u4"),
                       ("max_level", "<u4")])
 RECORD_DTYPE = np.dtype([("offset", "<u8"), ("treelet", "<u4"), ("size", "<u2"), ("type", "u1"),
-                         ("reserved", "u1")])
+                         ("level", "u1")])
 
 _vp = ctypes.c_void_p
 _u32, _u64, _i32, _f32 = ctypes.c_uint32, ctypes.c_uint64, ctypes.c_int, ctypes.c_float

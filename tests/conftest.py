@@ -80,6 +80,6 @@ def assert_hits_equal(a, b, what=""):
 
 def assert_records_equal(oa, ra, ob, rb, what=""):
     assert np.array_equal(oa, ob), f"{what}: record offsets differ"
-    for f in ("offset", "treelet", "size", "type"):
+    for f in ("offset", "treelet", "size", "type", "level"):
         bad = np.nonzero(ra[f] != rb[f])[0]
         assert len(bad) == 0, f"{what}: record field {f} differs at {len(bad)} records, first {bad[:5]}"

@@ -194,6 +194,8 @@ def _check_tiny(oracle, mode, case, partition=None):
     h, off, rec = oracle.trace(arena, 0, ray, mode=mode, partition=partition)
     got = [[int(r["offset"]), int(r["size"]), int(r["type"])] for r in rec]
     assert got == exp["records"]
+    levels = tiny_scene.expected()["levels"]
+    assert [int(r["level"]) for r in rec] == [levels[str(o)] for o, _, _ in exp["records"]]
     for k, v in exp["hit"].items():
         if isinstance(v, list):
             assert list(h[0][k]) == v, k

@@ -156,7 +156,7 @@ __global__ __launch_bounds__(STAGE ? 64 : 256) void trace_kernel(TraceParams p) 
     __builtin_amdgcn_wave_barrier();
   }
 
-  auto emit = [&](uint64_t off, uint32_t size, uint32_t type) {
+  auto emit = [&](uint64_t off, uint32_t size, uint32_t type, uint32_t lvl) {
     if (type != RTX_TX_BVH_PRIMITIVE_LEAF_DESCRIPTOR) fetch_bytes += size;
     if (REC) {
       uint64_t idx = rec_base + n_rec;
@@ -166,7 +166,7 @@ __global__ __launch_bounds__(STAGE ? 64 : 256) void trace_kernel(TraceParams p) 
         v.x = (uint32_t)off;
         v.y = (uint32_t)(off >> 32);
         v.z = tl;
-        v.w = size | (type << 16);
+        v.w = size | (type << 16) | ((lvl > 255u ? 255u : lvl) << 24);
         reinterpret_cast<uint4*>(p.records)[idx] = v;
       } else {
         err |= kErrRecords;
@@ -365,7 +365,7 @@ __global__ __launch_bounds__(STAGE ? 64 : 256) void trace_kernel(TraceParams p) 
           if (t + 64 > bytes) {
             err |= kErrRange;
           } else {
-            emit(t, 64, RTX_TX_BVH_STRUCTURE);
+            emit(t, 64, RTX_TX_BVH_STRUCTURE, 0);
             const uint4* hq = reinterpret_cast<const uint4*>(A + t);
             uint4 h0 = hq[0], h1 = hq[1];
             const uint64_t root = t + ((uint64_t)h0.x | ((uint64_t)h0.y << 32));
@@ -497,7 +497,7 @@ __global__ __launch_bounds__(STAGE ? 64 : 256) void trace_kernel(TraceParams p) 
           err |= kErrRange;
         } else {
           const Words16 nd = read_node(off, pos);
-          emit(off, 64, RTX_TX_BVH_INTERNAL_NODE);
+          emit(off, 64, RTX_TX_BVH_INTERNAL_NODE, level);
           n_nodes++;
           const bool top = (flags & kFlTop) != 0;
           const int ex = (int)(int8_t)byte_of(nd, 18);
@@ -581,14 +581,14 @@ __global__ __launch_bounds__(STAGE ? 64 : 256) void trace_kernel(TraceParams p) 
                          __uint_as_float(b.x), __uint_as_float(b.y), __uint_as_float(b.z), __uint_as_float(b.w),
                          __uint_as_float(c.x), __uint_as_float(c.y), __uint_as_float(c.z), __uint_as_float(c.w)};
           const uint64_t bvh = (uint64_t)d.x | ((uint64_t)d.y << 32);
-          emit(off, 128, RTX_TX_BVH_INSTANCE_LEAF);
+          emit(off, 128, RTX_TX_BVH_INSTANCE_LEAF, level);
           n_nodes++;
           const uint64_t blas = off + bvh;
           if ((blas & 63) || blas + 64 > bytes) {
             err |= kErrRange;
           } else {
             const uint2 ro2 = *reinterpret_cast<const uint2*>(A + blas);
-            emit(blas, 64, RTX_TX_BVH_STRUCTURE);
+            emit(blas, 64, RTX_TX_BVH_STRUCTURE, level);
             cur = transform_ray(m, ro, rd, rtmin, rtmax);
             cur_inst = slot;
             if (slot >= (1u << 22)) err |= kErrInstSlot;
@@ -607,7 +607,7 @@ __global__ __launch_bounds__(STAGE ? 64 : 256) void trace_kernel(TraceParams p) 
         if (off + 64 > bytes) {
           err |= kErrRange;
         } else {
-          emit(off, 8, RTX_TX_BVH_PRIMITIVE_LEAF_DESCRIPTOR);
+          emit(off, 8, RTX_TX_BVH_PRIMITIVE_LEAF_DESCRIPTOR, level);
           const Words16 q = read_node(off, pos);
           if (((q.w[1] >> 29) & 1u) == 0u) {
             if ((q.w[3] & 0x1ffffu) != 0u) err |= kErrType;
@@ -631,7 +631,7 @@ __global__ __launch_bounds__(STAGE ? 64 : 256) void trace_kernel(TraceParams p) 
               closest_inst = cur_inst;
               cobj_o = cur.o;
               cobj_d = cur.d;
-              emit(off, 64, RTX_TX_BVH_QUAD_LEAF_HIT);
+              emit(off, 64, RTX_TX_BVH_QUAD_LEAF_HIT, level);
               n_nodes++;
               if (MODE == 0 && !(rflags & RTX_RAY_FLAG_OPAQUE)) n_any++;
               if (rflags & RTX_RAY_FLAG_TERMINATE_ON_FIRST_HIT) {
@@ -639,11 +639,11 @@ __global__ __launch_bounds__(STAGE ? 64 : 256) void trace_kernel(TraceParams p) 
                 sb = Stack{0, 0, 0};
               }
             } else {
-              emit(off, 64, RTX_TX_BVH_QUAD_LEAF);
+              emit(off, 64, RTX_TX_BVH_QUAD_LEAF, level);
               n_nodes++;
             }
           } else {
-            emit(off, 64, RTX_TX_BVH_PROCEDURAL_LEAF);
+            emit(off, 64, RTX_TX_BVH_PROCEDURAL_LEAF, level);
             n_nodes++;
           }
         }

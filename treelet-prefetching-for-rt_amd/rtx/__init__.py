@@ -46,7 +46,7 @@ HIT_DTYPE = np.dtype([("hit", "<u4"), ("primitive_index", "<u4"), ("geometry_ind
                       ("n_records", "<u4"), ("n_nodes", "<u4"), ("fetch_bytes", "<u4"), ("n_any_hits", "<u4"),
                       ("max_level", "<u4")])
 RECORD_DTYPE = np.dtype([("offset", "<u8"), ("treelet", "<u4"), ("size", "<u2"), ("type", "u1"),
-                         ("reserved", "u1")])
+                         ("level", "u1")])
 assert RAY_DTYPE.itemsize == 64 and HIT_DTYPE.itemsize == 64 and RECORD_DTYPE.itemsize == 16
 
 
