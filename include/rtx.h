@@ -139,6 +139,10 @@ typedef struct rtx_trace_options {
   uint32_t stage_bytes;  /* LDS bytes per wave for a staged treelet (4-64 KiB, power of two);
                             0 = the treelet budget, capped at 16 KiB */
   uint32_t refill;       /* a wave fetches new rays once this many lanes are idle (1-64); 0 = default */
+  uint32_t slot_bytes;   /* staging: LDS slot size; stage_bytes / slot_bytes treelets stay resident
+                            at once, least recently used evicted first.  0 = one slot */
+  float min_score;       /* LEARNED: summed lane probability a predicted treelet needs before it is
+                            staged (0 = default 2.0) */
 } rtx_trace_options;
 
 /* Prefetch accounting of the last rtx_trace call, modelled on the
@@ -149,9 +153,9 @@ typedef struct rtx_prefetch_stats {
   uint64_t staged_treelets;   /* treelet loads into LDS */
   uint64_t staged_bytes;      /* bytes copied from HBM into LDS */
   uint64_t wasted_bytes;      /* staged bytes never read before eviction (never used) */
-  uint64_t predictions;       /* learned mode: predictions made */
-  uint64_t prediction_hits;   /* learned mode: predicted top-k contained the next treelet */
-  uint64_t fallbacks;         /* learned mode: heuristic used instead */
+  uint64_t predictions;       /* LEARNED: treelets staged because the table predicted them */
+  uint64_t prediction_hits;   /* LEARNED: predicted stagings that served a node read before eviction */
+  uint64_t fallbacks;         /* LEARNED: staging decisions where no lane had a table entry */
 } rtx_prefetch_stats;
 
 typedef struct rtx_treelet_summary {
@@ -216,6 +220,56 @@ rtx_status rtx_trace(rtx_ctx* ctx, const rtx_ray* rays_dev, uint32_t n, const rt
                      rtx_hit* hits_dev, rtx_record* records_dev, uint64_t record_capacity,
                      uint64_t* record_offsets_dev);
 rtx_status rtx_get_prefetch_stats(rtx_ctx* ctx, rtx_prefetch_stats* out);
+
+/* ---- learned next-treelet prefetch -----------------------------------------
+ * No counterpart in the reference (its RT unit has popularity heuristics
+ * only, src/gpgpu-sim/shader.cc:3419-3559).  RTX_PREFETCH_LEARNED reads a
+ * prediction table of the current partition: row 8*t + o holds, for a ray in
+ * treelet t whose direction lies in octant o (bit0 x<0, bit1 y<0, bit2 z<0),
+ * up to 4 entries (treelet + 1) | (probability * 255) << 24, 0 = empty, most
+ * likely first.  At a staging decision the wave stages the heuristic's choice
+ * and then each predicted treelet whose summed lane probability reaches
+ * min_score.  The table only moves node reads between LDS and global memory:
+ * hits and records are the same whatever it holds. */
+#define RTX_OCTANTS 8u
+rtx_status rtx_set_prefetch_table(rtx_ctx* ctx, const uint32_t* table_host, uint64_t n_rows);
+
+/* The network behind the table: a BF16 MLP (FP32 accumulation, MFMA
+ *   16x16x32) that maps
+ *   [E[cur] | octant one-hot | depth | E[h1..h4]] -> 128 -> 128 -> 32 -> C
+ * classes, where E is a 32-wide embedding of n_treelets (+1 "none") treelets,
+ * h1..h4 are the ray's last four treelets on its path from the root (nearest
+ * first), and class c scores q . O[c] + o[c].  Samples live in device memory. */
+typedef struct rtx_mlp_sample {
+  uint32_t cur;     /* treelet the ray is in */
+  uint32_t octant;  /* 0-7 */
+  float depth;      /* tree level of the node / 32 */
+  uint32_t hist[4]; /* treelets, RTX_NO_TREELET for none */
+  uint32_t target;  /* class of the next treelet (training only) */
+} rtx_mlp_sample;   /* 32 bytes */
+
+typedef struct rtx_mlp rtx_mlp;
+rtx_status rtx_mlp_create(rtx_ctx* ctx, uint32_t n_treelets, uint32_t n_classes, uint64_t seed, rtx_mlp** out);
+rtx_status rtx_mlp_destroy(rtx_mlp* m);
+/* Parameters are one FP32 vector: E, W1 (192x128, rows 169.. zero), b1, W2, b2,
+ * W3 (128x32), b3, O (C x 32), o. */
+rtx_status rtx_mlp_param_count(const rtx_mlp* m, uint64_t* count);
+rtx_status rtx_mlp_get_params(rtx_mlp* m, float* params_host, float* grads_host);
+rtx_status rtx_mlp_set_params(rtx_mlp* m, const float* params_host);
+/* Mean softmax cross-entropy over n samples and its gradient (left in the
+ * gradient vector, read with rtx_mlp_get_params). */
+rtx_status rtx_mlp_loss_grad(rtx_mlp* m, const rtx_mlp_sample* samples_dev, uint32_t n, float* loss);
+/* Adam (0.9, 0.999, 1e-8) over minibatches in order, `epochs` passes. */
+rtx_status rtx_mlp_train(rtx_mlp* m, const rtx_mlp_sample* samples_dev, uint32_t n, uint32_t epochs,
+                         uint32_t batch, float lr, float* last_epoch_loss);
+/* Top-4 classes of each sample (top4_dev[4n]) and their softmax probabilities
+ * (probs_dev[4n], may be NULL). */
+rtx_status rtx_mlp_predict(rtx_mlp* m, const rtx_mlp_sample* samples_dev, uint32_t n, uint32_t* top4_dev,
+                           float* probs_dev);
+/* C = op(A) . op(B) on the MLP's MFMA GEMM (M x K by K x N, row-major FP32,
+ * operands rounded to BF16): exposes the kernel for testing. */
+rtx_status rtx_mlp_gemm(rtx_ctx* ctx, int M, int N, int K, int trans_a, int trans_b, const float* A_dev,
+                        const float* B_dev, float* C_dev);
 
 /* ---- multi-scene batches -------------------------------------------------
  * No counterpart in the reference, which traces one ray per call against the

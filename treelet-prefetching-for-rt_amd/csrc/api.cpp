@@ -43,6 +43,9 @@ struct rtx_ctx {
   uint64_t* scan_tmp = nullptr;
   size_t scan_tmp_entries = 0;
   rtx_prefetch_stats last_stats{};
+  // learned prefetch table of the current partition
+  uint4* pred_table = nullptr;
+  uint32_t pred_treelets = 0;
   // multi-scene batches
   struct SceneSlot {
     uint8_t* arena = nullptr;
@@ -124,6 +127,25 @@ rtx_status ensure_scan_tmp(rtx_ctx* c, size_t entries) {
 
 }  // namespace
 
+// The prediction table belongs to one partition: dropped whenever it changes.
+static void drop_table(rtx_ctx* c) {
+  if (c->pred_table) {
+    hipStreamSynchronize(c->stream);
+    hipFree(c->pred_table);
+  }
+  c->pred_table = nullptr;
+  c->pred_treelets = 0;
+}
+
+namespace rtx_internal {
+rtx_status ctx_stream(rtx_ctx* c, hipStream_t* s, int* device) {
+  if (!c || !s || !device) return RTX_ERR_INVALID_ARGUMENT;
+  *s = c->stream;
+  *device = c->device;
+  return RTX_OK;
+}
+}  // namespace rtx_internal
+
 extern "C" {
 
 const char* rtx_status_string(rtx_status s) {
@@ -197,6 +219,7 @@ rtx_status rtx_destroy(rtx_ctx* c) {
   if (c->stream) hipStreamSynchronize(c->stream);
   free_scene(c);
   partition_free(c->part);
+  drop_table(c);
   if (c->spill) hipFree(c->spill);
   if (c->counters) hipFree(c->counters);
   if (c->stats) hipFree(c->stats);
@@ -307,6 +330,7 @@ rtx_status rtx_set_scene(rtx_ctx* c, const void* host, size_t bytes, uint64_t tl
   HIP_TRY(hipStreamSynchronize(c->stream));
   free_scene(c);
   partition_free(c->part);
+  drop_table(c);
   uint8_t* d = nullptr;
   if (hipMalloc(&d, bytes) != hipSuccess) {
     (void)hipGetLastError();
@@ -329,6 +353,7 @@ rtx_status rtx_set_scene_device(rtx_ctx* c, const void* dev, size_t bytes, uint6
   HIP_TRY(hipStreamSynchronize(c->stream));
   free_scene(c);
   partition_free(c->part);
+  drop_table(c);
   c->arena = (const uint8_t*)dev;
   c->arena_bytes = bytes;
   c->tlas = tlas;
@@ -347,6 +372,7 @@ rtx_status rtx_build_treelets(rtx_ctx* c, uint32_t budget, rtx_treelet_summary* 
   if (!c->arena) return RTX_ERR_NO_SCENE;
   if (budget < 128 || budget % 64 != 0 || budget > (1u << 20)) return RTX_ERR_INVALID_ARGUMENT;
   HIP_TRY(hipSetDevice(c->device));
+  drop_table(c);
   rtx_status st = partition_build(c->arena, c->arena_bytes, c->tlas, budget, c->stream, c->part);
   if (st != RTX_OK) return st;
   if (out) *out = c->part.summary;
@@ -366,6 +392,7 @@ rtx_status rtx_set_treelets(rtx_ctx* c, uint32_t budget, uint32_t n, const uint6
   if (!c || !roots || !list_offsets || !list_nodes || !owner || n == 0) return RTX_ERR_INVALID_ARGUMENT;
   if (!c->arena) return RTX_ERR_NO_SCENE;
   HIP_TRY(hipSetDevice(c->device));
+  drop_table(c);
   return partition_install(c->arena, c->arena_bytes, c->tlas, budget, n, roots, list_offsets, list_nodes, owner,
                            c->stream, c->part);
 }
@@ -398,7 +425,10 @@ rtx_status rtx_trace(rtx_ctx* c, const rtx_ray* rays, uint32_t n, const rtx_trac
       while (stage < c->part.budget && stage < 16384) stage *= 2;
     }
     if (stage < 4096 || stage > 65536 || (stage & (stage - 1))) return RTX_ERR_INVALID_ARGUMENT;
-    if (opts->prefetch == RTX_PREFETCH_LEARNED) return RTX_ERR_UNSUPPORTED;
+    if (opts->slot_bytes && (opts->slot_bytes < 64 || opts->slot_bytes > stage ||
+                             (opts->slot_bytes & (opts->slot_bytes - 1))))
+      return RTX_ERR_INVALID_ARGUMENT;
+    if (!(opts->min_score >= 0.f)) return RTX_ERR_INVALID_ARGUMENT;
     lds_stack = 8;
   }
   const uint32_t bs = trace_block_size(stage);
@@ -438,6 +468,12 @@ rtx_status rtx_trace(rtx_ctx* c, const rtx_ray* rays, uint32_t n, const rtx_trac
     p.image_offset = c->part.image_offset;
     p.image_bytes = c->part.image_bytes;
     p.images = c->part.images;
+  }
+  p.slot_units = opts->slot_bytes / 64;
+  p.min_score = opts->min_score > 0.f ? opts->min_score : 2.0f;
+  if (opts->prefetch == RTX_PREFETCH_LEARNED && c->pred_table && c->pred_treelets == c->part.n) {
+    p.pred_table = c->pred_table;
+    p.pred_treelets = c->pred_treelets;
   }
 
   HIP_TRY(hipMemsetAsync(c->counters, 0, 2 * sizeof(uint32_t), c->stream));
@@ -480,6 +516,31 @@ rtx_status rtx_trace(rtx_ctx* c, const rtx_ray* rays, uint32_t n, const rtx_trac
   c->last_stats.prediction_hits = stats[6];
   c->last_stats.fallbacks = stats[7];
   return kernel_error_status(err);
+}
+
+rtx_status rtx_set_prefetch_table(rtx_ctx* c, const uint32_t* table, uint64_t n_rows) {
+  if (!c || !table) return RTX_ERR_INVALID_ARGUMENT;
+  if (!c->part.valid) return RTX_ERR_NO_PARTITION;
+  if (n_rows != (uint64_t)c->part.n * RTX_OCTANTS) return RTX_ERR_INVALID_ARGUMENT;
+  for (uint64_t i = 0; i < n_rows * 4; ++i) {
+    const uint32_t t = table[i] & 0xffffffu;
+    if (t != 0 && t - 1 >= c->part.n) return RTX_ERR_INVALID_ARGUMENT;
+  }
+  HIP_TRY(hipSetDevice(c->device));
+  drop_table(c);
+  uint4* d = nullptr;
+  if (hipMalloc(&d, n_rows * sizeof(uint4)) != hipSuccess) {
+    (void)hipGetLastError();
+    return RTX_ERR_OUT_OF_MEMORY;
+  }
+  if (hipMemcpy(d, table, n_rows * sizeof(uint4), hipMemcpyHostToDevice) != hipSuccess) {
+    (void)hipGetLastError();
+    hipFree(d);
+    return RTX_ERR_HIP;
+  }
+  c->pred_table = d;
+  c->pred_treelets = c->part.n;
+  return RTX_OK;
 }
 
 rtx_status rtx_last_kernel_ms(rtx_ctx* c, float* ms) {

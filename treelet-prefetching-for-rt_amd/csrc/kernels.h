@@ -62,10 +62,17 @@ struct TraceParams {
   const uint64_t* image_offset;      // dense treelet -> byte offset of its image in `images`
   const uint32_t* image_bytes;       // dense treelet -> image size
   const uint8_t* images;             // treelet-contiguous copies of the arena
+  uint32_t slot_units;               // staging slot size in 64-byte units (0: one slot)
+  float min_score;                   // learned policy: summed lane probability to stage a prediction
+  const uint4* pred_table;           // learned policy: [pred_treelets * 8] rows of 4 packed predictions
+  uint32_t pred_treelets;
   // multi-scene batch (batch kernels only): rays are numbered across jobs
   const BatchJob* jobs;
   uint32_t n_jobs;
 };
+
+// Most treelets resident at once in a wave's staging area.
+constexpr uint32_t kMaxStageSlots = 16;
 
 // Launchers (defined in the .hip files).
 uint32_t trace_block_size(uint32_t stage_bytes);

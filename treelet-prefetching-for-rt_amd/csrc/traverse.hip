@@ -23,15 +23,18 @@
 // Stacks: a ring of K entries per lane in LDS, spilling the oldest entries to
 // a per-lane region in global memory (exact LIFO order).
 //
-// Treelet staging (STAGE > 0, one wave per workgroup): the wave keeps one
-// treelet image in LDS, copied from the treelet-contiguous image buffer with
-// coalesced 16-byte loads, and an LDS hash table from node slot to position in
-// the image.  A node read first probes the table; hits read LDS, misses read
-// global memory.  When most lanes miss, the wave gathers the owning treelet of
-// each lane's next node and stages the most popular one, following the
-// RT-unit policy of src/gpgpu-sim/shader.cc:3419-3559 (heuristic 0 always,
-// 1 only above a popularity share, 2 a popularity-sized prefix).  Staging
-// changes where node bytes are read from, never which nodes are visited.
+// Treelet staging (STAGE > 0, one wave per workgroup): the wave keeps up to
+// stage/slot treelet images in LDS slots, copied from the treelet-contiguous
+// image buffer with coalesced 16-byte loads, and an LDS hash table from node
+// slot to position in the staging area.  A node read first probes the table;
+// hits read LDS, misses read global memory.  When most lanes miss, the wave
+// gathers the owning treelet of each lane's next node and stages the most
+// popular one into the least recently read slot, following the RT-unit policy
+// of src/gpgpu-sim/shader.cc:3419-3559 (heuristic 0 always, 1 only above a
+// popularity share, 2 a popularity-sized prefix).  The learned policy then
+// also stages the treelets a prediction table names as the lanes' likely next
+// ones.  Staging changes where node bytes are read from, never which nodes
+// are visited.
 #include "device_common.h"
 #include "kernels.h"
 
@@ -102,6 +105,10 @@ __global__ __launch_bounds__(STAGE ? 64 : 256) void trace_kernel(TraceParams p) 
   __shared__ uint32_t hkey[HSIZE];
   __shared__ uint16_t hpos[HSIZE];
   __shared__ uint8_t used[STAGE ? UNITS : 1];
+  // slot table of the staging area: treelet, units held, iteration of the last
+  // read served, staged on a prediction
+  constexpr uint32_t MAXSLOTS = STAGE ? kMaxStageSlots : 1;
+  __shared__ uint32_t sl_t[MAXSLOTS], sl_units[MAXSLOTS], sl_last[MAXSLOTS], sl_pred[MAXSLOTS];
 
   const uint32_t tid = threadIdx.x;
   const uint32_t lane = tid & 63u;
@@ -147,13 +154,27 @@ __global__ __launch_bounds__(STAGE ? 64 : 256) void trace_kernel(TraceParams p) 
   uint64_t e = 0;
 
   // staging state (wave-uniform)
-  uint32_t staged = kNone, staged_units = 0, cooldown = 0;
-  uint64_t st_treelets = 0, st_bytes = 0, st_wasted = 0;
+  uint32_t cooldown = 0, iter = 0, n_staged = 0;
+  uint64_t st_treelets = 0, st_bytes = 0, st_wasted = 0, st_pred = 0, st_pred_hits = 0, st_fallback = 0;
   uint32_t reads = 0, lds_hits = 0;
-
+  uint32_t octant = 0;  // direction octant of the lane's ray (learned policy)
+  // slots of `cap` units each (powers of two): nslots treelets stay resident
+  uint32_t cap_shift = 0, nslots = 1;
   if constexpr (STAGE > 0) {
+    uint32_t cap = UNITS;
+    if (p.slot_units > 0 && p.slot_units < UNITS) cap = p.slot_units;
+    while ((1u << cap_shift) < cap) cap_shift++;
+    nslots = min(MAXSLOTS, UNITS >> cap_shift);
     for (uint32_t i = lane; i < HSIZE; i += 64) hkey[i] = 0;
+    for (uint32_t i = lane; i < MAXSLOTS; i += 64) {
+      sl_t[i] = kNone;
+      sl_units[i] = 0;
+      sl_last[i] = 0;
+      sl_pred[i] = 0;
+    }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
   }
 
   auto emit = [&](uint64_t off, uint32_t size, uint32_t type, uint32_t lvl) {
@@ -194,7 +215,7 @@ __global__ __launch_bounds__(STAGE ? 64 : 256) void trace_kernel(TraceParams p) 
   // Position of `slot` in the staged image, or kNone.
   auto staged_pos = [&](uint32_t slot) -> uint32_t {
     if constexpr (STAGE > 0) {
-      if (staged == kNone) return kNone;
+      if (n_staged == 0) return kNone;
       uint32_t h = slot_hash(slot) & (HSIZE - 1);
       for (uint32_t probe = 0; probe < HSIZE; ++probe) {
         const uint32_t k = hkey[h];
@@ -215,6 +236,7 @@ __global__ __launch_bounds__(STAGE ? 64 : 256) void trace_kernel(TraceParams p) 
       if (pos != kNone) {
         lds_hits++;
         used[pos] = 1;
+        sl_last[pos >> cap_shift] = iter;
         const uint4* q = stage_img + pos * 4 + first / 16;
         for (int k = 0; k < count; ++k) w[k] = q[k];
         return;
@@ -251,57 +273,110 @@ __global__ __launch_bounds__(STAGE ? 64 : 256) void trace_kernel(TraceParams p) 
     cur_inst = inst;
   };
 
-  // Account the staged image's unused units, then forget it (wave-uniform).
-  auto evict = [&]() {
-    if constexpr (STAGE > 0) {
-      if (staged == kNone) return;
-      uint32_t u = 0;
-      for (uint32_t i = lane; i < staged_units; i += 64) u += used[i] ? 1u : 0u;
-      u = wave_sum(u);
-      st_wasted += (uint64_t)(staged_units - u) * 64;
-      staged = kNone;
-      staged_units = 0;
-    }
+  auto wave_sync = [&]() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
   };
 
-  // Copy treelet t (its first `max_units` 64-byte units) into LDS (wave-uniform).
-  auto stage = [&](uint32_t t, uint32_t max_units) {
+  // Account slot j's unused units, then empty it (wave-uniform).
+  auto evict_slot = [&](uint32_t j) {
+    if constexpr (STAGE > 0) {
+      const uint32_t units = sl_units[j];
+      if (sl_t[j] == kNone) return;
+      const uint32_t base = j << cap_shift;
+      uint32_t u = 0;
+      for (uint32_t i = lane; i < units; i += 64) u += used[base + i] ? 1u : 0u;
+      u = wave_sum(u);
+      st_wasted += (uint64_t)(units - u) * 64;
+      if (sl_pred[j] && u > 0) st_pred_hits++;
+      wave_sync();
+      if (lane == 0) {
+        sl_t[j] = kNone;
+        sl_units[j] = 0;
+        sl_pred[j] = 0;
+      }
+      wave_sync();
+      n_staged--;
+    }
+  };
+  auto evict_all = [&]() {
+    if constexpr (STAGE > 0)
+      for (uint32_t j = 0; j < nslots; ++j) evict_slot(j);
+  };
+
+  // Copy treelet t (its first `max_units` 64-byte units) into a slot
+  // (wave-uniform): the slot already holding it, else an empty one, else the
+  // least recently read one that served no read in this iteration.  The hash
+  // table is rebuilt over every resident slot.  Returns whether it copied.
+  auto stage = [&](uint32_t t, uint32_t max_units, bool predicted) -> bool {
     if constexpr (STAGE > 0) {
       const uint32_t tt = __builtin_amdgcn_readfirstlane(t);
-      const uint64_t ioff = p.image_offset[tt];
       uint32_t units = p.image_bytes[tt] / 64;
-      units = min(units, min(UNITS, max_units));
+      units = min(units, min(1u << cap_shift, max_units));
       units = __builtin_amdgcn_readfirstlane(units);
-      if (units == 0) return;
-      evict();
-      const uint4* src = reinterpret_cast<const uint4*>(p.images + ioff);
-      for (uint32_t i = lane; i < units * 4; i += 64) stage_img[i] = src[i];
+      if (units == 0) return false;
+      uint32_t victim = kNone, empty = kNone, lru = kNone, lru_it = 0;
+      for (uint32_t j = 0; j < nslots; ++j) {
+        const uint32_t sj = sl_t[j];
+        if (sj == tt) victim = j;
+        if (sj == kNone) {
+          if (empty == kNone) empty = j;
+        } else if (sl_last[j] < iter && (lru == kNone || sl_last[j] < lru_it)) {
+          lru = j;
+          lru_it = sl_last[j];
+        }
+      }
+      if (victim != kNone) {
+        if (sl_units[victim] >= units) {
+          if (lane == 0) sl_last[victim] = iter;
+          wave_sync();
+          return false;
+        }
+      } else {
+        victim = empty != kNone ? empty : lru;
+        if (victim == kNone) return false;
+      }
+      victim = __builtin_amdgcn_readfirstlane(victim);
+      evict_slot(victim);
+      const uint32_t base = victim << cap_shift;
+      const uint4* src = reinterpret_cast<const uint4*>(p.images + p.image_offset[tt]);
+      for (uint32_t i = lane; i < units * 4; i += 64) stage_img[base * 4 + i] = src[i];
+      for (uint32_t i = lane; i < units; i += 64) used[base + i] = 0;
       for (uint32_t i = lane; i < HSIZE; i += 64) hkey[i] = 0;
-      for (uint32_t i = lane; i < units; i += 64) used[i] = 0;
-      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-      __builtin_amdgcn_wave_barrier();
-      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-      const uint32_t* slots = p.image_slots + ioff / 64;
-      for (uint32_t u = lane; u < units; u += 64) {
-        const uint32_t s = slots[u];
+      wave_sync();
+      if (lane == 0) {
+        sl_t[victim] = tt;
+        sl_units[victim] = units;
+        sl_last[victim] = iter;
+        sl_pred[victim] = predicted ? 1u : 0u;
+      }
+      wave_sync();
+      // insert every resident unit: unit i of the staging area is unit
+      // i & (cap - 1) of slot i >> cap_shift
+      for (uint32_t i = lane; i < (nslots << cap_shift); i += 64) {
+        const uint32_t j = i >> cap_shift, u = i & ((1u << cap_shift) - 1);
+        const uint32_t sj = sl_t[j];
+        if (sj == kNone || u >= sl_units[j]) continue;
+        const uint32_t s = p.image_slots[p.image_offset[sj] / 64 + u];
         if (s == kNone) continue;
         uint32_t h = slot_hash(s) & (HSIZE - 1);
         for (uint32_t probe = 0; probe < HSIZE; ++probe) {
           if (atomicCAS(&hkey[h], 0u, s + 1) == 0u) {
-            hpos[h] = (uint16_t)u;
+            hpos[h] = (uint16_t)i;
             break;
           }
           h = (h + 1) & (HSIZE - 1);
         }
       }
-      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-      __builtin_amdgcn_wave_barrier();
-      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-      staged = tt;
-      staged_units = units;
+      wave_sync();
+      n_staged++;
       st_treelets++;
       st_bytes += (uint64_t)units * 64;
+      if (predicted) st_pred++;
+      return true;
     }
+    return false;
   };
 
   while (true) {
@@ -348,6 +423,7 @@ __global__ __launch_bounds__(STAGE ? 64 : 256) void trace_kernel(TraceParams p) 
           rtmax = __uint_as_float(r1.w);
           rflags = r2.x;
           ridir = calc_idir(rd);
+          octant = (rd.x < 0.f ? 1u : 0u) | (rd.y < 0.f ? 2u : 0u) | (rd.z < 0.f ? 4u : 0u);
           min_thit = rtmax;
           min_thit_obj = 0.f;
           closest_leaf = kNoSlot;
@@ -444,6 +520,7 @@ __global__ __launch_bounds__(STAGE ? 64 : 256) void trace_kernel(TraceParams p) 
     // ---- staging decision (wave converged) -----------------------------------
     uint32_t pos = kNone;
     if constexpr (STAGE > 0) {
+      iter++;
       const bool want = has_e && err == 0;
       if (want) pos = staged_pos(slot);
       if (p.prefetch != RTX_PREFETCH_NONE) {
@@ -452,8 +529,8 @@ __global__ __launch_bounds__(STAGE ? 64 : 256) void trace_kernel(TraceParams p) 
         if (cooldown > 0) {
           cooldown--;
         } else if (nact > 0 && 2 * nmiss > nact) {
-          uint32_t t = kNone;
-          if (want && pos == kNone && off + 64 <= bytes) t = owner[slot];
+          const uint32_t own = (want && off + 64 <= bytes) ? owner[slot] : kNone;
+          const uint32_t t = pos == kNone ? own : kNone;
           uint64_t rem = __ballot(t != kNone);
           uint32_t best = kNone, bestc = 0;
           for (int c = 0; c < 4 && rem; ++c) {
@@ -476,11 +553,40 @@ __global__ __launch_bounds__(STAGE ? 64 : 256) void trace_kernel(TraceParams p) 
               const uint32_t units = p.image_bytes[best] / 64;
               maxu = (uint32_t)((float)units * share + 0.5f);
             }
-            stage(best, maxu);
-            if (want) pos = staged_pos(slot);
+            stage(best, maxu, false);
           } else {
             cooldown = 8;
           }
+          if (p.prefetch == RTX_PREFETCH_LEARNED) {
+            // Each lane's predicted next treelets for (its treelet, its
+            // octant); a treelet is staged when the lanes' summed probability
+            // reaches min_score, at most two per decision.
+            uint4 row = make_uint4(0, 0, 0, 0);
+            if (own != kNone && p.pred_table && own < p.pred_treelets)
+              row = p.pred_table[(size_t)own * RTX_OCTANTS + octant];
+            uint32_t ent[4] = {row.x, row.y, row.z, row.w};
+            if (__ballot(row.x != 0) == 0) st_fallback++;
+            uint32_t n_new = 0;
+            for (int round = 0; round < 8 && n_new < 2; ++round) {
+              uint32_t mine = 0;
+              for (int k = 3; k >= 0; --k)
+                if (ent[k] != 0) mine = ent[k];
+              const uint64_t has = __ballot(mine != 0);
+              if (has == 0) break;
+              const uint32_t lead = (uint32_t)__ffsll((long long)has) - 1u;
+              const uint32_t cand = __shfl(mine, (int)lead, 64) & 0xffffffu;
+              float sc = 0.f;
+              for (int k = 0; k < 4; ++k)
+                if (ent[k] != 0 && (ent[k] & 0xffffffu) == cand) {
+                  sc += (float)(ent[k] >> 24) * (1.f / 255.f);
+                  ent[k] = 0;
+                }
+              for (int o = 32; o > 0; o >>= 1) sc += __shfl_xor(sc, o, 64);
+              const uint32_t ct = cand - 1u;
+              if (sc >= p.min_score && ct < p.pred_treelets) n_new += stage(ct, UNITS, true) ? 1u : 0u;
+            }
+          }
+          if (want) pos = staged_pos(slot);
         }
       }
     }
@@ -573,7 +679,9 @@ __global__ __launch_bounds__(STAGE ? 64 : 256) void trace_kernel(TraceParams p) 
           err |= kErrRange;
         } else {
           // a 128-byte leaf is served from LDS only when both of its units are staged
-          const uint32_t ipos = (pos != kNone && pos + 1 < staged_units) ? pos : kNone;
+          uint32_t ipos = kNone;
+          if constexpr (STAGE > 0)
+            if (pos != kNone && (pos & ((1u << cap_shift) - 1)) + 1 < sl_units[pos >> cap_shift]) ipos = pos;
           uint4 w4[4];
           read_words(off, ipos, 16, w4, 4);
           const uint4 a = w4[0], b = w4[1], c = w4[2], d = w4[3];
@@ -652,7 +760,7 @@ __global__ __launch_bounds__(STAGE ? 64 : 256) void trace_kernel(TraceParams p) 
   }
 
   if constexpr (STAGE > 0) {
-    evict();
+    evict_all();
     const uint32_t r = wave_sum(reads), hl = wave_sum(lds_hits);
     if (lane == 0) {
       atomicAdd(&p.stats[0], (unsigned long long)r);
@@ -660,6 +768,9 @@ __global__ __launch_bounds__(STAGE ? 64 : 256) void trace_kernel(TraceParams p) 
       atomicAdd(&p.stats[2], (unsigned long long)st_treelets);
       atomicAdd(&p.stats[3], (unsigned long long)st_bytes);
       atomicAdd(&p.stats[4], (unsigned long long)st_wasted);
+      atomicAdd(&p.stats[5], (unsigned long long)st_pred);
+      atomicAdd(&p.stats[6], (unsigned long long)st_pred_hits);
+      atomicAdd(&p.stats[7], (unsigned long long)st_fallback);
     }
   }
 }

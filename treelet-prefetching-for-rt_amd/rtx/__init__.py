@@ -59,7 +59,8 @@ class RtxError(RuntimeError):
 class _TraceOptions(ctypes.Structure):
     _fields_ = [("mode", ctypes.c_uint32), ("prefetch", ctypes.c_uint32), ("threshold", ctypes.c_float),
                 ("emit_records", ctypes.c_uint32), ("waves_per_cu", ctypes.c_uint32),
-                ("lds_stack", ctypes.c_uint32), ("stage_bytes", ctypes.c_uint32), ("refill", ctypes.c_uint32)]
+                ("lds_stack", ctypes.c_uint32), ("stage_bytes", ctypes.c_uint32), ("refill", ctypes.c_uint32),
+                ("slot_bytes", ctypes.c_uint32), ("min_score", ctypes.c_float)]
 
 
 class _PrefetchStats(ctypes.Structure):
@@ -128,6 +129,16 @@ _sig("rtx_batch_graph_create", _i32, _vp, _P(_BatchJob), _u32, _P(_TraceOptions)
 _sig("rtx_batch_graph_launch", _i32, _vp, _u32, _P(_f32))
 _sig("rtx_batch_graph_destroy", _i32, _vp)
 _sig("rtx_autotune_batch", _i32, _vp, _P(_BatchJob), _u32, _P(_TraceOptions), _P(_f32))
+_sig("rtx_set_prefetch_table", _i32, _vp, _vp, _u64)
+_sig("rtx_mlp_create", _i32, _vp, _u32, _u32, _u64, _P(_vp))
+_sig("rtx_mlp_destroy", _i32, _vp)
+_sig("rtx_mlp_param_count", _i32, _vp, _P(_u64))
+_sig("rtx_mlp_get_params", _i32, _vp, _vp, _vp)
+_sig("rtx_mlp_set_params", _i32, _vp, _vp)
+_sig("rtx_mlp_loss_grad", _i32, _vp, _vp, _u32, _P(_f32))
+_sig("rtx_mlp_train", _i32, _vp, _vp, _u32, _u32, _u32, _f32, _P(_f32))
+_sig("rtx_mlp_predict", _i32, _vp, _vp, _u32, _vp, _vp)
+_sig("rtx_mlp_gemm", _i32, _vp, _i32, _i32, _i32, _i32, _i32, _vp, _vp, _vp)
 _sig("rtx_builder_create", _i32, _P(_vp))
 _sig("rtx_builder_destroy", _i32, _vp)
 _sig("rtx_builder_add_blas", _i32, _vp, _vp, _u32, _u32, _P(_u32))
@@ -150,6 +161,8 @@ EXPORTED_SYMBOLS = [
     "rtx_build_treelets", "rtx_get_treelets", "rtx_set_treelets", "rtx_trace", "rtx_get_prefetch_stats",
     "rtx_last_kernel_ms", "rtx_add_scene", "rtx_build_scene_treelets", "rtx_trace_batch", "rtx_batch_graph_create",
     "rtx_batch_graph_launch", "rtx_batch_graph_destroy", "rtx_autotune_batch",
+    "rtx_set_prefetch_table", "rtx_mlp_create", "rtx_mlp_destroy", "rtx_mlp_param_count", "rtx_mlp_get_params",
+    "rtx_mlp_set_params", "rtx_mlp_loss_grad", "rtx_mlp_train", "rtx_mlp_predict", "rtx_mlp_gemm",
     "rtx_builder_create", "rtx_builder_destroy", "rtx_builder_add_blas", "rtx_builder_add_instance",
     "rtx_builder_finish", "rtx_builder_copy_arena", "rtx_builder_arena_info", "rtx_builder_counts", "rtx_builder_blas_offset",
     "rtx_mesh_displaced_sphere", "rtx_workload_scene", "rtx_workload_primary_rays", "rtx_workload_secondary_rays",
@@ -320,6 +333,83 @@ class DeviceBuffer:
             pass
 
 
+MLP_SAMPLE_DTYPE = np.dtype([("cur", "<u4"), ("octant", "<u4"), ("depth", "<f4"), ("hist", "<u4", 4),
+                             ("target", "<u4")])
+
+
+class MLP:
+    """The next-treelet network (rtx_mlp): training and inference on the device."""
+
+    def __init__(self, ctx, n_treelets, n_classes, seed=1):
+        self.ctx = ctx
+        self.n_treelets, self.n_classes = int(n_treelets), int(n_classes)
+        h = ctypes.c_void_p()
+        _check(_lib.rtx_mlp_create(ctx._h, self.n_treelets, self.n_classes, seed, ctypes.byref(h)), "rtx_mlp_create")
+        self._h = h
+        c = ctypes.c_uint64(0)
+        _check(_lib.rtx_mlp_param_count(h, ctypes.byref(c)), "rtx_mlp_param_count")
+        self.n_params = c.value
+        self._samples = None
+
+    def upload(self, samples):
+        """Copy samples (MLP_SAMPLE_DTYPE) to the device; returns a DeviceBuffer."""
+        s = np.ascontiguousarray(samples, MLP_SAMPLE_DTYPE)
+        return self.ctx.to_device(s)
+
+    def _buf(self, samples):
+        return samples if isinstance(samples, DeviceBuffer) else self.upload(samples)
+
+    def params(self):
+        p = np.empty(self.n_params, np.float32)
+        _check(_lib.rtx_mlp_get_params(self._h, _ptr(p), None), "rtx_mlp_get_params")
+        return p
+
+    def grads(self):
+        g = np.empty(self.n_params, np.float32)
+        _check(_lib.rtx_mlp_get_params(self._h, None, _ptr(g)), "rtx_mlp_get_params")
+        return g
+
+    def set_params(self, p):
+        p = np.ascontiguousarray(p, np.float32)
+        assert len(p) == self.n_params
+        _check(_lib.rtx_mlp_set_params(self._h, _ptr(p)), "rtx_mlp_set_params")
+
+    def loss_grad(self, samples, n=None):
+        b = self._buf(samples)
+        n = len(samples) if n is None else n
+        loss = ctypes.c_float(0)
+        _check(_lib.rtx_mlp_loss_grad(self._h, b.ptr, n, ctypes.byref(loss)), "rtx_mlp_loss_grad")
+        return loss.value
+
+    def train(self, samples, n=None, epochs=1, batch=1024, lr=1e-3):
+        b = self._buf(samples)
+        n = len(samples) if n is None else n
+        loss = ctypes.c_float(0)
+        _check(_lib.rtx_mlp_train(self._h, b.ptr, n, epochs, batch, lr, ctypes.byref(loss)), "rtx_mlp_train")
+        return loss.value
+
+    def predict(self, samples, n=None):
+        """Top-4 classes and their probabilities, arrays [n, 4]."""
+        b = self._buf(samples)
+        n = len(samples) if n is None else n
+        tb = self.ctx.buffer(n * 16)
+        pb = self.ctx.buffer(n * 16)
+        _check(_lib.rtx_mlp_predict(self._h, b.ptr, n, tb.ptr, pb.ptr), "rtx_mlp_predict")
+        return tb.download(np.uint32, 4 * n).reshape(n, 4), pb.download(np.float32, 4 * n).reshape(n, 4)
+
+    def close(self):
+        if self._h:
+            _lib.rtx_mlp_destroy(self._h)
+            self._h = ctypes.c_void_p(None)
+
+    def __del__(self):
+        try:
+            if self.ctx._h:
+                self.close()
+        except Exception:
+            pass
+
+
 class BatchGraph:
     """A captured multi-scene batch launch (rtx_batch_graph)."""
 
@@ -425,10 +515,10 @@ class Context:
 
     def trace_device(self, rays_buf, n, hits_buf, mode=MODE_PLAIN, prefetch=PREFETCH_NONE, threshold=0.5,
                      records_buf=None, record_capacity=0, offsets_buf=None, waves_per_cu=0, lds_stack=0,
-                     stage_bytes=0, refill=0):
+                     stage_bytes=0, refill=0, slot_bytes=0, min_score=0.0):
         """rtx_trace on device buffers; returns the rtx_status (0 = ok)."""
         o = _TraceOptions(mode, prefetch, threshold, 1 if offsets_buf is not None else 0, waves_per_cu, lds_stack,
-                          stage_bytes, refill)
+                          stage_bytes, refill, slot_bytes, min_score)
         return _lib.rtx_trace(self._h, rays_buf.ptr, n, ctypes.byref(o), hits_buf.ptr,
                               records_buf.ptr if records_buf is not None else None, record_capacity,
                               offsets_buf.ptr if offsets_buf is not None else None)
@@ -451,6 +541,25 @@ class Context:
         recb = self.buffer(max(total, 1) * RECORD_DTYPE.itemsize)
         _check(self.trace_device(rb, n, hb, mode, prefetch, threshold, recb, total, ob, **kw), "rtx_trace")
         return hb.download(HIT_DTYPE, n), ob.download(np.uint64, n + 1), recb.download(RECORD_DTYPE, total)
+
+    def set_prefetch_table(self, table):
+        """Install the learned-prefetch table of the current partition: uint32 [n_treelets * 8, 4]."""
+        t = np.ascontiguousarray(table, np.uint32).reshape(-1, 4)
+        _check(_lib.rtx_set_prefetch_table(self._h, _ptr(t), len(t)), "rtx_set_prefetch_table")
+
+    def mlp(self, n_treelets, n_classes, seed=1):
+        return MLP(self, n_treelets, n_classes, seed)
+
+    def gemm(self, a, b, trans_a=False, trans_b=False):
+        """The MLP's BF16 MFMA GEMM on host arrays: op(a) @ op(b) in FP32."""
+        a = np.ascontiguousarray(a, np.float32)
+        b = np.ascontiguousarray(b, np.float32)
+        m, k = (a.shape[1], a.shape[0]) if trans_a else a.shape
+        k2, n = (b.shape[1], b.shape[0]) if trans_b else b.shape
+        assert k == k2
+        ab, bb, cb = self.to_device(a), self.to_device(b), self.buffer(m * n * 4)
+        _check(_lib.rtx_mlp_gemm(self._h, m, n, k, int(trans_a), int(trans_b), ab.ptr, bb.ptr, cb.ptr), "rtx_mlp_gemm")
+        return cb.download(np.float32, m * n).reshape(m, n)
 
     def prefetch_stats(self):
         s = _PrefetchStats()
@@ -480,7 +589,7 @@ class Context:
 
     @staticmethod
     def _opts(mode=MODE_PLAIN, waves_per_cu=0, lds_stack=0, refill=0):
-        return _TraceOptions(mode, PREFETCH_NONE, 0.0, 0, waves_per_cu, lds_stack, 0, refill)
+        return _TraceOptions(mode, PREFETCH_NONE, 0.0, 0, waves_per_cu, lds_stack, 0, refill, 0, 0.0)
 
     def trace_batch(self, jobs, **opts):
         arr = self._jobs(jobs)
