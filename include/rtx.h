@@ -142,7 +142,9 @@ typedef struct rtx_trace_options {
   uint32_t slot_bytes;   /* staging: LDS slot size; stage_bytes / slot_bytes treelets stay resident
                             at once, least recently used evicted first.  0 = one slot */
   float min_score;       /* LEARNED: summed lane probability a predicted treelet needs before it is
-                            staged (0 = default 2.0) */
+                            staged (0 = default 16.0) */
+  float pred_fraction;   /* LEARNED: share of a predicted treelet's image staged, root first
+                            (0 = default 0.25; 1 = the whole image) */
 } rtx_trace_options;
 
 /* Prefetch accounting of the last rtx_trace call, modelled on the

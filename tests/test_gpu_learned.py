@@ -78,3 +78,49 @@ def test_table_is_dropped_with_its_partition(gpu, instanced, rtx):
     bad[0, 0] = s["n_treelets"] + 1  # names treelet n_treelets: out of range
     with pytest.raises(rtx.RtxError):
         gpu.set_prefetch_table(bad)
+
+
+def test_trained_network_gains_over_heuristic(gpu, rtx):
+    """The network trained on this library's own traces (80/20 split: held-out
+    pixels and a held-out bounce seed) against the heuristic it falls back to,
+    same slots, on the held-out rays: more node reads served from LDS and no
+    more wasted staging bytes, with identical hits."""
+    from rtx import learned
+
+    n_inst = 256
+    d = rtx.workload_desc(rtx.WORKLOAD_INSTANCED_SPHERES, n_instances=n_inst, slices=32, stacks=16, shared_blas=0,
+                          width=512, height=512, seed=2023, box_size=100.0 * (n_inst / 1024) ** (1 / 3),
+                          scale_min=0.5, scale_max=2.0, displacement=0.05)
+    scene = rtx.workload_scene(d)
+    gpu.set_scene(scene.arena, scene.tlas_offset)
+    gpu.build_treelets(1024)
+    prim = rtx.primary_rays(d)
+    hits = gpu.trace(prim)
+    idx = np.arange(len(prim))
+    tr, te = idx % 5 != 0, idx % 5 == 0
+    train = [prim[tr][::2]]
+    for seed in (1, 2, 3, 4):
+        train.append(scene.secondary_rays(prim[tr][::4], hits[tr][::4], kind=0, seed=seed)[0])
+    test = np.concatenate([prim[te], scene.secondary_rays(prim[te], hits[te], kind=0, seed=5)[0]])
+    lp = learned.LearnedPrefetcher(gpu, scene.arena, scene.tlas_offset, n_classes=16384)
+    try:
+        losses = lp.fit(np.concatenate(train), epochs=2, max_samples=1_200_000)
+        top1, top4, n_tr = lp.accuracy(test)
+        lp.install()
+    finally:
+        lp.close()
+    kw = dict(stage_bytes=8192, slot_bytes=1024)
+    h_heur = gpu.trace(test, prefetch=rtx.PREFETCH_ALWAYS, **kw)
+    heur = gpu.prefetch_stats()
+    h_learn = gpu.trace(test, prefetch=rtx.PREFETCH_LEARNED, **kw)
+    lrn = gpu.prefetch_stats()
+    hr = lambda s: s["lds_hits"] / s["node_reads"]  # noqa: E731
+    wb = lambda s: s["wasted_bytes"] / s["node_reads"]  # noqa: E731
+    print(f"losses {losses}; held-out next-treelet accuracy top1 {top1:.3f} top4 {top4:.3f} over {n_tr}; "
+          f"LDS hit rate heuristic {hr(heur):.4f} learned {hr(lrn):.4f}; wasted B/read heuristic {wb(heur):.3f} "
+          f"learned {wb(lrn):.3f}; predicted stagings {lrn['predictions']} used {lrn['prediction_hits']}")
+    assert_hits_equal(h_learn, h_heur, "learned vs heuristic")
+    assert losses[-1] < losses[0] and top4 > 0.4
+    assert lrn["predictions"] > 0 and lrn["prediction_hits"] > 0.3 * lrn["predictions"]
+    assert hr(lrn) > hr(heur)
+    assert wb(lrn) <= 1.03 * wb(heur)

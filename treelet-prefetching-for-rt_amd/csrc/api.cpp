@@ -429,6 +429,7 @@ rtx_status rtx_trace(rtx_ctx* c, const rtx_ray* rays, uint32_t n, const rtx_trac
                              (opts->slot_bytes & (opts->slot_bytes - 1))))
       return RTX_ERR_INVALID_ARGUMENT;
     if (!(opts->min_score >= 0.f)) return RTX_ERR_INVALID_ARGUMENT;
+    if (!(opts->pred_fraction >= 0.f && opts->pred_fraction <= 1.f)) return RTX_ERR_INVALID_ARGUMENT;
     lds_stack = 8;
   }
   const uint32_t bs = trace_block_size(stage);
@@ -470,7 +471,11 @@ rtx_status rtx_trace(rtx_ctx* c, const rtx_ray* rays, uint32_t n, const rtx_trac
     p.images = c->part.images;
   }
   p.slot_units = opts->slot_bytes / 64;
-  p.min_score = opts->min_score > 0.f ? opts->min_score : 2.0f;
+  // defaults: the best hit rate / wasted bytes pair of a sweep on held-out
+  // diffuse rays of the 1M-triangle instanced-sphere scene (1 KiB treelets,
+  // 8 KiB staging area in 1 KiB slots)
+  p.min_score = opts->min_score > 0.f ? opts->min_score : 16.0f;
+  p.pred_fraction = opts->pred_fraction > 0.f ? opts->pred_fraction : 0.25f;
   if (opts->prefetch == RTX_PREFETCH_LEARNED && c->pred_table && c->pred_treelets == c->part.n) {
     p.pred_table = c->pred_table;
     p.pred_treelets = c->pred_treelets;

@@ -64,6 +64,7 @@ struct TraceParams {
   const uint8_t* images;             // treelet-contiguous copies of the arena
   uint32_t slot_units;               // staging slot size in 64-byte units (0: one slot)
   float min_score;                   // learned policy: summed lane probability to stage a prediction
+  float pred_fraction;               // learned policy: share of a predicted image staged (root first)
   const uint4* pred_table;           // learned policy: [pred_treelets * 8] rows of 4 packed predictions
   uint32_t pred_treelets;
   // multi-scene batch (batch kernels only): rays are numbered across jobs

@@ -316,6 +316,8 @@ __global__ __launch_bounds__(STAGE ? 64 : 256) void trace_kernel(TraceParams p) 
       units = min(units, min(1u << cap_shift, max_units));
       units = __builtin_amdgcn_readfirstlane(units);
       if (units == 0) return false;
+      // victim: the slot holding t, else an empty slot, else the least
+      // recently read slot that served no read in this iteration
       uint32_t victim = kNone, empty = kNone, lru = kNone, lru_it = 0;
       for (uint32_t j = 0; j < nslots; ++j) {
         const uint32_t sj = sl_t[j];
@@ -559,8 +561,9 @@ __global__ __launch_bounds__(STAGE ? 64 : 256) void trace_kernel(TraceParams p) 
           }
           if (p.prefetch == RTX_PREFETCH_LEARNED) {
             // Each lane's predicted next treelets for (its treelet, its
-            // octant); a treelet is staged when the lanes' summed probability
-            // reaches min_score, at most two per decision.
+            // octant).  A predicted treelet (its first pred_fraction of units)
+            // is staged when the lanes' summed probability reaches min_score,
+            // at most two per decision.
             uint4 row = make_uint4(0, 0, 0, 0);
             if (own != kNone && p.pred_table && own < p.pred_treelets)
               row = p.pred_table[(size_t)own * RTX_OCTANTS + octant];
@@ -583,7 +586,11 @@ __global__ __launch_bounds__(STAGE ? 64 : 256) void trace_kernel(TraceParams p) 
                 }
               for (int o = 32; o > 0; o >>= 1) sc += __shfl_xor(sc, o, 64);
               const uint32_t ct = cand - 1u;
-              if (sc >= p.min_score && ct < p.pred_treelets) n_new += stage(ct, UNITS, true) ? 1u : 0u;
+              if (sc >= p.min_score && ct < p.pred_treelets) {
+                const uint32_t units = p.image_bytes[ct] / 64;
+                const uint32_t part = max(1u, (uint32_t)((float)units * p.pred_fraction + 0.5f));
+                n_new += stage(ct, part, true) ? 1u : 0u;
+              }
             }
           }
           if (want) pos = staged_pos(slot);
