@@ -11,7 +11,12 @@ A step traces the whole batch (primary + bounce rays) once.  `value` is the
 batch's rays per second of the headline mode over the timed steps (wall clock
 around synchronous rtx_trace calls); every mode is measured and reported.
 
-  python bench.py [--steps K] [--warmup W] [--budget BYTES] [--mode NAME]
+--workload large: 20,480 instances of the same sphere (20,971,520 triangles,
+1.7 GiB arena) in a 271^3 box, 3840x2160 pixels x 4 samples (a stratified 2x2
+grid per pixel) and 3 cosine-weighted diffuse bounces per path without Russian
+roulette; a step traces the four segments (camera rays, then each bounce) once.
+
+  python bench.py [--steps K] [--warmup W] [--budget BYTES] [--mode NAME] [--workload diffuse|large]
 """
 import argparse
 import json
@@ -43,6 +48,13 @@ DEFAULT_MODES = "plain,treelet,plain+stage,treelet+stage,plain+slots,plain+learn
 HEADLINE = "plain"
 
 
+def c5_desc(rtx, width=7680, height=4320):
+    n_inst = 20480
+    return rtx.workload_desc(rtx.WORKLOAD_INSTANCED_SPHERES, n_instances=n_inst, slices=32, stacks=16, shared_blas=0,
+                             width=width, height=height, seed=2025, box_size=100.0 * (n_inst / 1024) ** (1 / 3),
+                             scale_min=0.5, scale_max=2.0, displacement=0.05)
+
+
 def c2_desc(rtx, width=1024, height=1024):
     return rtx.workload_desc(rtx.WORKLOAD_INSTANCED_SPHERES, n_instances=1024, slices=32, stacks=16, shared_blas=0,
                              width=width, height=height, seed=2023, box_size=100.0, scale_min=0.5, scale_max=2.0,
@@ -53,31 +65,29 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def cpu_baseline(scene, batch, target_s=10.0):
-    """The oracle (C restatement of traceRay) on host cores, plain order, over a strided sample."""
+def cpu_baseline(scene, sample, what, target_s=10.0):
+    """The oracle (C restatement of traceRay) on host cores, plain order: passes over
+    `sample` until target_s seconds of work have run."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle
 
     threads = max(1, min(16, os.cpu_count() or 1))
-    probe = batch[:: max(1, len(batch) // 8192)]
+    rays = done = 0
     t0 = time.perf_counter()
-    oracle.trace_parallel(scene.arena, scene.tlas_offset, probe, mode=0, threads=threads)
-    rate = len(probe) / max(time.perf_counter() - t0, 1e-6)
-    n = int(min(len(batch), max(len(probe), rate * target_s)))
-    sample = batch[:: max(1, len(batch) // n)]
-    t0 = time.perf_counter()
-    oracle.trace_parallel(scene.arena, scene.tlas_offset, sample, mode=0, threads=threads)
-    dt = time.perf_counter() - t0
-    return {"value": round(len(sample) / dt / 1e6, 4), "unit": "Mrays/s", "cores": threads, "kind": "port",
-            "sample": f"{len(sample)} rays (every {max(1, len(batch) // n)}th ray of the primary+bounce batch), "
-                      f"plain order, {dt:.1f} s on {threads} host threads"}
+    while True:
+        oracle.trace_parallel(scene.arena, scene.tlas_offset, sample, mode=0, threads=threads)
+        rays += len(sample)
+        done += 1
+        dt = time.perf_counter() - t0
+        if dt >= target_s:
+            break
+    return {"value": round(rays / dt / 1e6, 4), "unit": "Mrays/s", "cores": threads, "kind": "port",
+            "sample": f"{what}: {len(sample)} rays x {done} passes, plain order, {dt:.1f} s on {threads} host threads"}
 
 
-def train_learned(rtx, ctx, scene, primary, hits, args):
-    """Train the next-treelet network on traces of rays other than the timed batch
-    (a fifth of the pixels held out, diffuse bounces of other seeds) and install its table."""
-    from rtx import learned
-
+def training_rays_diffuse(scene, primary, hits):
+    """Rays other than the timed batch: a fifth of the pixels held out, diffuse
+    bounces of other seeds."""
     idx = np.arange(len(primary))
     keep = idx % 5 != 0
     tp, th = primary[keep], hits[keep]
@@ -85,7 +95,13 @@ def train_learned(rtx, ctx, scene, primary, hits, args):
     for seed in (1, 2, 3, 4):
         b, _ = scene.secondary_rays(tp[::4], th[::4], kind=0, seed=seed)
         train.append(b)
-    train = np.concatenate(train)
+    return np.concatenate(train)
+
+
+def train_learned(ctx, scene, train, args):
+    """Train the next-treelet network on traces of `train` and install its table."""
+    from rtx import learned
+
     t0 = time.perf_counter()
     lp = learned.LearnedPrefetcher(ctx, scene.arena, scene.tlas_offset, n_classes=16384)
     losses = lp.fit(train, epochs=args.learn_epochs, max_samples=args.learn_samples)
@@ -100,10 +116,10 @@ def train_learned(rtx, ctx, scene, primary, hits, args):
     return info
 
 
-def load_traffic(mode):
+def load_traffic(workload, mode):
     """HBM bytes per launch of the traversal kernel from the committed counter
     profile (FETCH_SIZE + WRITE_SIZE, calibrated with a copy kernel), if present."""
-    path = os.path.join(ROOT, "profiles", "traffic_c2_diffuse.json")
+    path = os.path.join(ROOT, "profiles", f"traffic_{workload}.json")
     try:
         with open(path) as f:
             d = json.load(f)
@@ -126,7 +142,11 @@ def main():
     ap.add_argument("--modes", default=DEFAULT_MODES, help="modes to measure")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
-    ap.add_argument("--width", type=int, default=1024)
+    ap.add_argument("--width", type=int, default=0, help="override the image width (height follows the aspect)")
+    ap.add_argument("--workload", default="diffuse", choices=["diffuse", "large"])
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
+                    help="copy MIB MiB device to device once before tracing: a dispatch of known size "
+                         "that calibrates the FETCH_SIZE / WRITE_SIZE counters")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (rays shard trivially across "
@@ -135,7 +155,12 @@ def main():
 
     import rtx
 
-    desc = c2_desc(rtx, args.width, args.width)
+    if args.workload == "diffuse":
+        w = args.width or 1024
+        desc = c2_desc(rtx, w, w)
+    else:
+        w = args.width or 7680
+        desc = c5_desc(rtx, w, w * 9 // 16)
     t0 = time.perf_counter()
     scene = rtx.workload_scene(desc)
     log(f"scene: {scene.n_triangles} tris, {scene.n_blas} BLAS, {scene.arena.nbytes / 2**20:.1f} MiB arena, "
@@ -143,15 +168,51 @@ def main():
     ctx = rtx.Context(0)
     info = ctx.device_info()
     ctx.set_scene(scene.arena, scene.tlas_offset)
+    if args.copy_probe:
+        a, b = ctx.buffer(args.copy_probe << 20), ctx.buffer(args.copy_probe << 20)
+        b.copy_from(a, args.copy_probe << 20)
+        ctx.synchronize()
+        log(f"copy probe: {args.copy_probe} MiB device to device")
+        a.free()
+        b.free()
 
+    t0 = time.perf_counter()
     primary = rtx.primary_rays(desc)
     hits = ctx.trace(primary, mode=0)
-    bounce, _ = scene.secondary_rays(primary, hits, kind=0, seed=99)
-    batch = np.concatenate([primary, bounce])
-    n = len(batch)
-    log(f"batch: {len(primary)} primary ({int(hits['hit'].sum())} hits) + {len(bounce)} diffuse = {n} rays")
-    rays_d = ctx.to_device(batch)
-    hits_d = ctx.buffer(n * rtx.HIT_DTYPE.itemsize)
+    if args.workload == "diffuse":
+        bounce, _ = scene.secondary_rays(primary, hits, kind=0, seed=99)
+        segments = [np.concatenate([primary, bounce])]
+        log(f"batch: {len(primary)} primary ({int(hits['hit'].sum())} hits) + {len(bounce)} diffuse = "
+            f"{len(segments[0])} rays")
+        train_rays = training_rays_diffuse(scene, primary, hits)
+        workload = (f"1,024 instances x 1,024-tri displaced spheres ({scene.n_triangles} tris), {w}x{w} primary + "
+                    f"1 cosine diffuse bounce per hit ({len(segments[0])} rays/step)")
+    else:
+        # camera rays, then three diffuse bounces from each segment's hits; the
+        # training traces come from an earlier frame (other bounce seeds)
+        segments = [primary]
+        train = [primary[::16]]
+        seg, sh = primary, hits
+        tseg, tsh = primary[::16], hits[::16]
+        for b in range(3):
+            nxt, _ = scene.secondary_rays(seg, sh, kind=0, seed=1000 + b)
+            segments.append(nxt)
+            tn, _ = scene.secondary_rays(tseg, tsh, kind=0, seed=2000 + b)
+            train.append(tn)
+            if b < 2:
+                sh = ctx.trace(nxt, mode=0)
+                tsh = ctx.trace(tn, mode=0)
+            seg, tseg = nxt, tn
+        train_rays = np.concatenate(train)
+        log("segments: " + ", ".join(str(len(x)) for x in segments) + f" rays; set up in "
+            f"{time.perf_counter() - t0:.1f} s")
+        workload = (f"20,480 instances x 1,024-tri displaced spheres ({scene.n_triangles} tris), {w // 2}x"
+                    f"{w * 9 // 32} pixels x 4 spp (2x2 stratified) x (camera + 3 diffuse bounces, no Russian "
+                    f"roulette) ({sum(len(x) for x in segments)} rays/step)")
+    del hits
+    n = sum(len(x) for x in segments)
+    dev = [(ctx.to_device(x), len(x), ctx.buffer(len(x) * rtx.HIT_DTYPE.itemsize)) for x in segments]
+    batch = np.concatenate(segments) if args.workload == "diffuse" else None
 
     modes = [m for m in args.modes.split(",") if m in MODES]
     if args.mode not in modes:
@@ -173,35 +234,46 @@ def main():
             log(f"treelets: {partitions[cur_budget]} at {cur_budget} B budget, built in "
                 f"{time.perf_counter() - t0:.2f} s")
         if cfg["prefetch"] == 4:
-            learned_info = train_learned(rtx, ctx, scene, primary, hits, args)
+            learned_info = train_learned(ctx, scene, train_rays, args)
         kw = dict(mode=cfg["order"], prefetch=cfg["prefetch"], threshold=args.threshold, stage_bytes=cfg["stage"],
                   slot_bytes=cfg["slot"])
+
+        def step():
+            ms = 0.0
+            for rays_d, k, hits_d in dev:
+                st = ctx.trace_device(rays_d, k, hits_d, **kw)
+                if st != 0:
+                    raise rtx.RtxError(st, f"rtx_trace {name}")
+                ms += ctx.last_kernel_ms()
+            return ms
+
         for _ in range(args.warmup):
-            st = ctx.trace_device(rays_d, n, hits_d, **kw)
-            if st != 0:
-                raise rtx.RtxError(st, f"rtx_trace {name}")
+            step()
         kms = []
         ctx.synchronize()
         t0 = time.perf_counter()
         for _ in range(args.steps):
-            st = ctx.trace_device(rays_d, n, hits_d, **kw)
-            if st != 0:
-                raise rtx.RtxError(st, f"rtx_trace {name}")
-            kms.append(ctx.last_kernel_ms())
+            kms.append(step())
         ctx.synchronize()
         wall = (time.perf_counter() - t0) / args.steps
-        h = hits_d.download(rtx.HIT_DTYPE, n)
-        fetch = float(h["fetch_bytes"].astype(np.float64).sum())
+        fetch = 0.0
+        nodes = 0.0
+        pst = {}
+        for rays_d, k, hits_d in dev:
+            ctx.trace_device(rays_d, k, hits_d, **kw)
+            for key, v in ctx.prefetch_stats().items():
+                pst[key] = pst.get(key, 0) + v
+            h = hits_d.download(rtx.HIT_DTYPE, k)
+            fetch += float(h["fetch_bytes"].astype(np.float64).sum())
+            nodes += float(h["n_nodes"].astype(np.float64).sum())
         kernel_ms = float(np.mean(kms))
-        pst = ctx.prefetch_stats()
         results[name] = {
             "mrays_s": n / wall / 1e6,
             "ms_per_step": wall * 1e3,
             "kernel_ms": kernel_ms,
             "node_fetch_gbs": fetch / (kernel_ms * 1e-3) / 1e9,
             "fetch_bytes": fetch,
-            "nodes_per_ray": float(h["n_nodes"].mean()),
-            "hits": int(h["hit"].sum()),
+            "nodes_per_ray": nodes / n,
             "config": cfg,
             "prefetch": pst,
         }
@@ -229,8 +301,7 @@ def main():
         "dtype": "fp32",
         "data": "synthetic",
         "config": {
-            "workload": f"1,024 instances x 1,024-tri displaced spheres ({scene.n_triangles - 0} tris), "
-                        f"{args.width}x{args.width} primary + 1 cosine diffuse bounce per hit ({n} rays/step)",
+            "workload": workload,
             "mode": args.mode,
             "partitions": {str(k): v for k, v in partitions.items()},
             "arena_mib": round(scene.arena.nbytes / 2**20, 2),
@@ -243,7 +314,7 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": round(head["node_fetch_gbs"] / HBM_PEAK_GBS, 4),
-            "traffic": load_traffic(args.mode),
+            "traffic": load_traffic(args.workload, args.mode),
             "kernel": "trace_kernel",
             "kernel_ms": round(head["kernel_ms"], 4),
             "bytes_per_launch": head["fetch_bytes"],
@@ -258,7 +329,12 @@ def main():
     }
     if not args.no_cpu_baseline:
         log("cpu baseline ...")
-        out["cpu_baseline"] = cpu_baseline(scene, batch, args.cpu_seconds)
+        if batch is not None:
+            smp, what = batch, "the whole primary+bounce batch"
+        else:
+            smp = np.concatenate([x[:: max(1, len(x) // 250000)] for x in segments])
+            what = "about 250,000 evenly strided rays of each segment"
+        out["cpu_baseline"] = cpu_baseline(scene, smp, what, args.cpu_seconds)
     print(json.dumps(out), flush=True)
     ctx.close()
     return 0

@@ -182,6 +182,7 @@ rtx_status rtx_malloc(rtx_ctx* ctx, size_t bytes, void** out_dev);
 rtx_status rtx_free(rtx_ctx* ctx, void* dev);
 rtx_status rtx_memcpy_h2d(rtx_ctx* ctx, void* dst_dev, const void* src_host, size_t bytes);
 rtx_status rtx_memcpy_d2h(rtx_ctx* ctx, void* dst_host, const void* src_dev, size_t bytes);
+rtx_status rtx_memcpy_d2d(rtx_ctx* ctx, void* dst_dev, const void* src_dev, size_t bytes);
 rtx_status rtx_memset(rtx_ctx* ctx, void* dev, int value, size_t bytes);
 /* Stream-ordered timer around work issued on the context's stream. */
 rtx_status rtx_timer_start(rtx_ctx* ctx);

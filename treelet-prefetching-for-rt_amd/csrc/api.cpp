@@ -275,6 +275,15 @@ rtx_status rtx_memcpy_h2d(rtx_ctx* c, void* dst, const void* src, size_t bytes) 
   return RTX_OK;
 }
 
+rtx_status rtx_memcpy_d2d(rtx_ctx* c, void* dst, const void* src, size_t bytes) {
+  if (!c || (!dst && bytes) || (!src && bytes)) return RTX_ERR_INVALID_ARGUMENT;
+  if (!bytes) return RTX_OK;
+  HIP_TRY(hipSetDevice(c->device));
+  HIP_TRY(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, c->stream));
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  return RTX_OK;
+}
+
 rtx_status rtx_memcpy_d2h(rtx_ctx* c, void* dst, const void* src, size_t bytes) {
   if (!c || (!dst && bytes) || (!src && bytes)) return RTX_ERR_INVALID_ARGUMENT;
   if (!bytes) return RTX_OK;

@@ -109,6 +109,7 @@ _sig("rtx_malloc", _i32, _vp, ctypes.c_size_t, _P(_vp))
 _sig("rtx_free", _i32, _vp, _vp)
 _sig("rtx_memcpy_h2d", _i32, _vp, _vp, _vp, ctypes.c_size_t)
 _sig("rtx_memcpy_d2h", _i32, _vp, _vp, _vp, ctypes.c_size_t)
+_sig("rtx_memcpy_d2d", _i32, _vp, _vp, _vp, ctypes.c_size_t)
 _sig("rtx_memset", _i32, _vp, _vp, _i32, ctypes.c_size_t)
 _sig("rtx_timer_start", _i32, _vp)
 _sig("rtx_timer_stop", _i32, _vp, _P(_f32))
@@ -156,7 +157,7 @@ _sig("rtx_workload_secondary_rays", _i32, _vp, _vp, _vp, _u32, _u32, _u64, _vp, 
 #: every symbol declared in include/rtx.h and include/rtx_scene.h
 EXPORTED_SYMBOLS = [
     "rtx_status_string", "rtx_version", "rtx_device_count", "rtx_create", "rtx_destroy", "rtx_synchronize",
-    "rtx_malloc", "rtx_free", "rtx_memcpy_h2d", "rtx_memcpy_d2h", "rtx_memset", "rtx_timer_start",
+    "rtx_malloc", "rtx_free", "rtx_memcpy_h2d", "rtx_memcpy_d2h", "rtx_memcpy_d2d", "rtx_memset", "rtx_timer_start",
     "rtx_timer_stop", "rtx_device_info", "rtx_set_scene", "rtx_set_scene_device", "rtx_validate_scene",
     "rtx_build_treelets", "rtx_get_treelets", "rtx_set_treelets", "rtx_trace", "rtx_get_prefetch_stats",
     "rtx_last_kernel_ms", "rtx_add_scene", "rtx_build_scene_treelets", "rtx_trace_batch", "rtx_batch_graph_create",
@@ -313,6 +314,11 @@ class DeviceBuffer:
         a = np.ascontiguousarray(arr)
         assert a.nbytes <= self.nbytes
         _check(_lib.rtx_memcpy_h2d(self.ctx._h, self.ptr, _ptr(a), a.nbytes), "rtx_memcpy_h2d")
+
+    def copy_from(self, other, nbytes):
+        """Device-to-device copy of nbytes from another buffer of the same context."""
+        assert nbytes <= min(self.nbytes, other.nbytes)
+        _check(_lib.rtx_memcpy_d2d(self.ctx._h, self.ptr, other.ptr, nbytes), "rtx_memcpy_d2d")
 
     def download(self, dtype, count):
         out = np.empty(count, dtype=dtype)
