@@ -53,6 +53,27 @@ __device__ __forceinline__ bool ray_box(F3 lo3, F3 hi3, F3 idir, F3 o, float tmi
   return mn <= mx;
 }
 
+// ray_box with the compare-selects done by v_min_f32 / v_max_f32.  The two
+// agree whenever no operand is NaN: they can differ only in the sign of a
+// zero result, and thit and the returned flag only ever enter comparisons, in
+// which -0 == +0.  Callers take this form only when the ray (origin, inverse
+// direction, t range) and the node origin hold no NaN or infinity, which keeps
+// every slab product NaN-free.
+__device__ __forceinline__ bool ray_box_fast(F3 lo3, F3 hi3, F3 idir, F3 o, float tmin, float tmax, float& thit) {
+  float lox = (lo3.x - o.x) * idir.x, loy = (lo3.y - o.y) * idir.y, loz = (lo3.z - o.z) * idir.z;
+  float hix = (hi3.x - o.x) * idir.x, hiy = (hi3.y - o.y) * idir.y, hiz = (hi3.z - o.z) * idir.z;
+  float mn = __builtin_fmaxf(__builtin_fminf(loz, hiz),
+                             __builtin_fmaxf(__builtin_fminf(loy, hiy), __builtin_fmaxf(__builtin_fminf(lox, hix), tmin)));
+  float mx = __builtin_fminf(__builtin_fmaxf(loz, hiz),
+                             __builtin_fminf(__builtin_fmaxf(loy, hiy), __builtin_fminf(__builtin_fmaxf(lox, hix), tmax)));
+  thit = mn;
+  return mn <= mx;
+}
+
+__device__ __forceinline__ bool finite3(F3 v) {
+  return __builtin_isfinite(v.x) && __builtin_isfinite(v.y) && __builtin_isfinite(v.z);
+}
+
 // mt_ray_triangle_test, vulkan_ray_tracing.cc:3089-3111
 __device__ __forceinline__ bool mt_test(F3 p0, F3 p1, F3 p2, F3 o, F3 d, float& thit) {
   F3 v0v1 = sub3(p1, p0);

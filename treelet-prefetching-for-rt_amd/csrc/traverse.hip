@@ -35,6 +35,8 @@
 // also stages the treelets a prediction table names as the lanes' likely next
 // ones.  Staging changes where node bytes are read from, never which nodes
 // are visited.
+#include <type_traits>
+
 #include "device_common.h"
 #include "kernels.h"
 
@@ -635,22 +637,33 @@ __global__ __launch_bounds__(STAGE ? 64 : 256) void trace_kernel(TraceParams p) 
           uint64_t coff[6];
           uint32_t ctype[6];
           uint32_t hitmask = 0;
+          // v_min/v_max slab tests when no lane can produce a NaN (finite
+          // ray, non-NaN node origin); compare-selects otherwise
+          const bool ray_ok = finite3(o) && finite3(idir) && !__builtin_isnan(tmin) && !__builtin_isnan(tmax);
+          const bool fast = ray_ok && !__builtin_isnan(f_of(nd, 0) + f_of(nd, 1) + f_of(nd, 2));
+          auto test_children = [&](auto fast_tag) {
+            constexpr bool FAST = decltype(fast_tag)::value;
 #pragma unroll
-          for (int i = 0; i < 6; ++i) {
-            const uint32_t info = byte_of(nd, 22 + i) & 0x3fu;
-            const uint32_t size = info & 3u;
-            ctype[i] = info >> 2;
-            coff[i] = child;
-            child += (uint64_t)size * 64;
-            if (size > 0) {
-              F3 lo, hi;
-              child_box(nd, i, ex, ey, ez, lo, hi);
-              float th;
-              bool h = ray_box(lo, hi, idir, o, tmin, tmax, th);
-              if (h && th >= cull) h = false;
-              if (h) hitmask |= 1u << i;
+            for (int i = 0; i < 6; ++i) {
+              const uint32_t info = byte_of(nd, 22 + i) & 0x3fu;
+              const uint32_t size = info & 3u;
+              ctype[i] = info >> 2;
+              coff[i] = child;
+              child += (uint64_t)size * 64;
+              if (size > 0) {
+                F3 lo, hi;
+                child_box(nd, i, ex, ey, ez, lo, hi);
+                float th;
+                bool h = FAST ? ray_box_fast(lo, hi, idir, o, tmin, tmax, th) : ray_box(lo, hi, idir, o, tmin, tmax, th);
+                if (h && th >= cull) h = false;
+                if (h) hitmask |= 1u << i;
+              }
             }
-          }
+          };
+          if (__ballot(!fast) == 0)
+            test_children(std::true_type{});
+          else
+            test_children(std::false_type{});
           if (hitmask) {
             const uint32_t lvl = level + 1;
             if (lvl > max_level) max_level = lvl;
