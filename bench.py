@@ -287,6 +287,15 @@ def main():
                f" B/read" if cfg["prefetch"] else ""))
 
     head = results[args.mode]
+    # host-buffer rate of the headline mode: upload, trace, download (PCIe-inclusive; not `value`)
+    hcfg = MODES[args.mode]
+    t0 = time.perf_counter()
+    for seg in segments:
+        ctx.trace(seg, mode=hcfg["order"], prefetch=hcfg["prefetch"], stage_bytes=hcfg["stage"],
+                  slot_bytes=hcfg["slot"])
+    pcie_mrays = n / (time.perf_counter() - t0) / 1e6
+    log(f"host buffers in and out ({args.mode}): {pcie_mrays:.1f} Mrays/s")
+
     out = {
         "metric": METRIC,
         "value": round(head["mrays_s"], 3),
@@ -326,6 +335,7 @@ def main():
                           "wasted_bytes_per_read": round(v["wasted_bytes_per_read"], 3)} if "lds_hit_rate" in v else {})}
                   for k, v in results.items()},
         "learned": learned_info,
+        "pcie_inclusive_mrays_s": round(pcie_mrays, 2),
     }
     if not args.no_cpu_baseline:
         log("cpu baseline ...")
