@@ -19,6 +19,9 @@
 
 using namespace rtxk;
 
+// bytes of the control block cleared before and read after a traversal
+static constexpr size_t kCtlBytes = 16 * sizeof(uint32_t) + 16 * sizeof(unsigned long long);
+
 struct rtx_ctx {
   int device = 0;
   hipStream_t stream = nullptr;
@@ -38,7 +41,11 @@ struct rtx_ctx {
   // scratch
   uint64_t* spill = nullptr;
   size_t spill_entries = 0;
-  uint32_t* counters = nullptr;  // [0] ray counter, [1] error word
+  // One 256-byte control block: [0] ray counter, [1] error word, stats from
+  // word 16 (16 x u64).  Cleared with one memset, read back with one copy into
+  // pinned host memory.
+  uint32_t* counters = nullptr;
+  uint32_t* ctl_host = nullptr;
   unsigned long long* stats = nullptr;
   uint64_t* scan_tmp = nullptr;
   size_t scan_tmp_entries = 0;
@@ -208,7 +215,10 @@ rtx_status rtx_create(int device, rtx_ctx** out) {
   if (hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess) return fail(RTX_ERR_HIP);
   if (hipEventCreate(&c->evk0) != hipSuccess || hipEventCreate(&c->evk1) != hipSuccess) return fail(RTX_ERR_HIP);
   if (hipMalloc(&c->counters, 64 * sizeof(uint32_t)) != hipSuccess) return fail(RTX_ERR_OUT_OF_MEMORY);
-  if (hipMalloc(&c->stats, 16 * sizeof(unsigned long long)) != hipSuccess) return fail(RTX_ERR_OUT_OF_MEMORY);
+  c->stats = reinterpret_cast<unsigned long long*>(c->counters + 16);
+  if (hipHostMalloc(&c->ctl_host, 64 * sizeof(uint32_t), hipHostMallocDefault) != hipSuccess)
+    return fail(RTX_ERR_OUT_OF_MEMORY);
+  if (hipMemset(c->counters, 0, 64 * sizeof(uint32_t)) != hipSuccess) return fail(RTX_ERR_HIP);
   *out = c;
   return RTX_OK;
 }
@@ -222,7 +232,7 @@ rtx_status rtx_destroy(rtx_ctx* c) {
   drop_table(c);
   if (c->spill) hipFree(c->spill);
   if (c->counters) hipFree(c->counters);
-  if (c->stats) hipFree(c->stats);
+  if (c->ctl_host) hipHostFree(c->ctl_host);
   if (c->scan_tmp) hipFree(c->scan_tmp);
   for (auto& sc : c->scenes) {
     if (sc.arena) hipFree(sc.arena);
@@ -423,7 +433,9 @@ rtx_status rtx_trace(rtx_ctx* c, const rtx_ray* rays, uint32_t n, const rtx_trac
     HIP_TRY(hipStreamSynchronize(c->stream));
     return RTX_OK;
   }
-  uint32_t lds_stack = opts->lds_stack ? opts->lds_stack : 8;
+  // 16 LDS entries per lane for plain order (one stack, 32 KiB per block);
+  // treelet order keeps two stacks, so 8 (the same LDS per block)
+  uint32_t lds_stack = opts->lds_stack ? opts->lds_stack : (opts->mode == RTX_MODE_PLAIN ? 16 : 8);
   if (lds_stack != 4 && lds_stack != 8 && lds_stack != 16) return RTX_ERR_INVALID_ARGUMENT;
   // treelet staging: one wave per workgroup with stage_bytes of LDS for the image
   uint32_t stage = 0;
@@ -490,8 +502,7 @@ rtx_status rtx_trace(rtx_ctx* c, const rtx_ray* rays, uint32_t n, const rtx_trac
     p.pred_treelets = c->pred_treelets;
   }
 
-  HIP_TRY(hipMemsetAsync(c->counters, 0, 2 * sizeof(uint32_t), c->stream));
-  HIP_TRY(hipMemsetAsync(c->stats, 0, 16 * sizeof(unsigned long long), c->stream));
+  HIP_TRY(hipMemsetAsync(c->counters, 0, kCtlBytes, c->stream));
   c->kernel_timed = false;
   HIP_TRY(hipEventRecord(c->evk0, c->stream));
   HIP_TRY(launch_trace(p, opts->mode, false, lds_stack, stage, grid, c->stream));
@@ -512,15 +523,13 @@ rtx_status rtx_trace(rtx_ctx* c, const rtx_ray* rays, uint32_t n, const rtx_trac
     p.records = records;
     p.record_capacity = capacity;
     p.record_offsets = offsets;
-    HIP_TRY(hipMemsetAsync(c->counters, 0, 2 * sizeof(uint32_t), c->stream));
-    HIP_TRY(hipMemsetAsync(c->stats, 0, 16 * sizeof(unsigned long long), c->stream));
+    HIP_TRY(hipMemsetAsync(c->counters, 0, kCtlBytes, c->stream));
     HIP_TRY(launch_trace(p, opts->mode, true, lds_stack, stage, grid, c->stream));
   }
-  uint32_t err = 0;
-  unsigned long long stats[16];
-  HIP_TRY(hipMemcpyAsync(&err, c->counters + 1, sizeof(uint32_t), hipMemcpyDeviceToHost, c->stream));
-  HIP_TRY(hipMemcpyAsync(stats, c->stats, sizeof(stats), hipMemcpyDeviceToHost, c->stream));
+  HIP_TRY(hipMemcpyAsync(c->ctl_host, c->counters, kCtlBytes, hipMemcpyDeviceToHost, c->stream));
   HIP_TRY(hipStreamSynchronize(c->stream));
+  const uint32_t err = c->ctl_host[1];
+  const unsigned long long* stats = reinterpret_cast<const unsigned long long*>(c->ctl_host + 16);
   c->last_stats.node_reads = stats[0];
   c->last_stats.lds_hits = stats[1];
   c->last_stats.staged_treelets = stats[2];
