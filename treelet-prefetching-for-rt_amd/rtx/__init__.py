@@ -140,6 +140,11 @@ _sig("rtx_mlp_loss_grad", _i32, _vp, _vp, _u32, _P(_f32))
 _sig("rtx_mlp_train", _i32, _vp, _vp, _u32, _u32, _u32, _f32, _P(_f32))
 _sig("rtx_mlp_predict", _i32, _vp, _vp, _u32, _vp, _vp)
 _sig("rtx_mlp_gemm", _i32, _vp, _i32, _i32, _i32, _i32, _i32, _vp, _vp, _vp)
+_sig("rtx_format_treelet_trace", _i32, _vp, _vp, _u32, _vp, _u32, _u64, _u32, _vp, ctypes.c_size_t,
+     _P(ctypes.c_size_t))
+_sig("rtx_as_dump_write", _i32, _vp, _u64, _u64, ctypes.c_char_p)
+_sig("rtx_as_dump_info", _i32, ctypes.c_char_p, _P(_u64), _P(_u64))
+_sig("rtx_as_dump_read", _i32, ctypes.c_char_p, _vp, _u64)
 _sig("rtx_builder_create", _i32, _P(_vp))
 _sig("rtx_builder_destroy", _i32, _vp)
 _sig("rtx_builder_add_blas", _i32, _vp, _vp, _u32, _u32, _P(_u32))
@@ -167,6 +172,7 @@ EXPORTED_SYMBOLS = [
     "rtx_builder_create", "rtx_builder_destroy", "rtx_builder_add_blas", "rtx_builder_add_instance",
     "rtx_builder_finish", "rtx_builder_copy_arena", "rtx_builder_arena_info", "rtx_builder_counts", "rtx_builder_blas_offset",
     "rtx_mesh_displaced_sphere", "rtx_workload_scene", "rtx_workload_primary_rays", "rtx_workload_secondary_rays",
+    "rtx_format_treelet_trace", "rtx_as_dump_write", "rtx_as_dump_info", "rtx_as_dump_read",
 ]
 
 
@@ -299,6 +305,38 @@ def primary_rays(desc):
     rays = np.zeros(desc.width * desc.height, dtype=RAY_DTYPE)
     _check(_lib.rtx_workload_primary_rays(ctypes.byref(desc), _ptr(rays)), "rtx_workload_primary_rays")
     return rays
+
+
+# ---- reference formats ------------------------------------------------------------
+def format_treelet_trace(offsets, records, roots, base_address=0, first_ray_id=1):
+    """Per-ray treelet trace lines of traceRayWithTreelets ("RayID,<id>,0x<root>,...")."""
+    offsets = np.ascontiguousarray(offsets, np.uint64)
+    records = np.ascontiguousarray(records, RECORD_DTYPE)
+    roots = np.ascontiguousarray(roots, np.uint64)
+    n = len(offsets) - 1
+    length = ctypes.c_size_t(0)
+    args = (_ptr(records) if len(records) else None, _ptr(offsets), n, _ptr(roots) if len(roots) else None,
+            len(roots), base_address, first_ray_id)
+    _check(_lib.rtx_format_treelet_trace(*args, None, 0, ctypes.byref(length)), "rtx_format_treelet_trace")
+    buf = ctypes.create_string_buffer(length.value + 1)
+    _check(_lib.rtx_format_treelet_trace(*args, buf, length.value + 1, ctypes.byref(length)),
+           "rtx_format_treelet_trace")
+    return buf.raw[: length.value].decode()
+
+
+def as_dump_write(arena, tlas_offset, prefix):
+    """Write the split-file AS dump (.asmain/.asback/.asfront/.asmetadata) of an arena."""
+    a = np.ascontiguousarray(arena, np.uint8)
+    _check(_lib.rtx_as_dump_write(_ptr(a), a.nbytes, tlas_offset, str(prefix).encode()), "rtx_as_dump_write")
+
+
+def as_dump_read(prefix):
+    """Rebuild an arena from a split-file AS dump; returns (arena, tlas_offset)."""
+    nb, tl = ctypes.c_uint64(0), ctypes.c_uint64(0)
+    _check(_lib.rtx_as_dump_info(str(prefix).encode(), ctypes.byref(nb), ctypes.byref(tl)), "rtx_as_dump_info")
+    a = np.zeros(nb.value, np.uint8)
+    _check(_lib.rtx_as_dump_read(str(prefix).encode(), _ptr(a), a.nbytes), "rtx_as_dump_read")
+    return a, tl.value
 
 
 # ---- device context -------------------------------------------------------------
