@@ -8,7 +8,8 @@ pytestmark = pytest.mark.gpu
 
 
 @pytest.mark.parametrize("ta,tb", [(False, False), (False, True), (True, False), (True, True)])
-@pytest.mark.parametrize("shape", [(16, 16, 32), (70, 50, 45), (129, 65, 200), (1, 300, 7)])
+@pytest.mark.parametrize("shape", [(16, 16, 32), (70, 50, 45), (129, 65, 200), (1, 300, 7),
+                                   (100, 32, 2048), (300, 20, 700), (64, 96, 1536), (257, 33, 1000)])
 def test_gemm_matches_bf16_oracle(gpu, ta, tb, shape):
     m, n, k = shape
     print(f"gemm m={m} n={n} k={k} ta={ta} tb={tb}", flush=True)

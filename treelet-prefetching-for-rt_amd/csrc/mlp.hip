@@ -33,30 +33,36 @@ typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 
 // ---- GEMM: C[M x N] (+)= op(A)[M x K] . op(B)[K x N], FP32 in memory --------
-// 256 threads = 4 waves; a block computes a 64x64 tile, a wave 32x32 as 2x2
-// MFMA 16x16 tiles.  MFMA 16x16x32 operand layout: lane l supplies row/column
-// l % 16 and the 8 consecutive k values starting at 8 * (l / 16); the result
-// holds rows 4 * (l / 16) + i (i < 4) of column l % 16.
-template <bool TA, bool TB>
-__global__ __launch_bounds__(256) void gemm_bf16(int M, int N, int K, const float* __restrict__ A, int lda,
-                                                 const float* __restrict__ B, int ldb, float* __restrict__ C, int ldc,
-                                                 int accumulate) {
+// 256 threads = 4 waves, each wave a 32x32 tile as 2x2 MFMA 16x16 tiles; the
+// block tile is 64x64 (2x2 waves) or, for N <= 32, 128x32 (4x1 waves).
+// MFMA 16x16x32 operand layout: lane l supplies row/column l % 16 and the 8
+// consecutive k values starting at 8 * (l / 16); the result holds rows
+// 4 * (l / 16) + i (i < 4) of column l % 16.  With gridDim.z > 1 the K range
+// is split across blocks and the partial products are added into C atomically
+// (C already holds the addend: zero or the value to accumulate into).
+template <bool TA, bool TB, int BN>
+__global__ __launch_bounds__(256) void gemm_bf16(int M, int N, int K, int k_chunk, const float* __restrict__ A,
+                                                 int lda, const float* __restrict__ B, int ldb, float* __restrict__ C,
+                                                 int ldc, int accumulate) {
+  constexpr int WAVES_N = BN / 32;
   const int lane = threadIdx.x & 63;
   const int wave = threadIdx.x >> 6;
-  const int row0 = blockIdx.y * 64 + (wave >> 1) * 32;
-  const int col0 = blockIdx.x * 64 + (wave & 1) * 32;
+  const int row0 = blockIdx.y * (128 / WAVES_N) + (wave / WAVES_N) * 32;
+  const int col0 = blockIdx.x * BN + (wave % WAVES_N) * 32;
   const int r16 = lane & 15, kq = (lane >> 4) * 8;
+  const int kb = blockIdx.z * k_chunk;
+  const int ke = min(K, kb + k_chunk);
   f32x4 acc[2][2];
   for (int i = 0; i < 2; ++i)
     for (int j = 0; j < 2; ++j) acc[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
-  for (int k0 = 0; k0 < K; k0 += 32) {
+  for (int k0 = kb; k0 < ke; k0 += 32) {
     bf16x8 a[2], b[2];
     for (int i = 0; i < 2; ++i) {
       const int m = row0 + i * 16 + r16;
       for (int t = 0; t < 8; ++t) {
         const int k = k0 + kq + t;
         float v = 0.f;
-        if (m < M && k < K) v = TA ? A[(size_t)k * lda + m] : A[(size_t)m * lda + k];
+        if (m < M && k < ke) v = TA ? A[(size_t)k * lda + m] : A[(size_t)m * lda + k];
         a[i][t] = (__bf16)v;
       }
     }
@@ -65,13 +71,14 @@ __global__ __launch_bounds__(256) void gemm_bf16(int M, int N, int K, const floa
       for (int t = 0; t < 8; ++t) {
         const int k = k0 + kq + t;
         float v = 0.f;
-        if (n < N && k < K) v = TB ? B[(size_t)n * ldb + k] : B[(size_t)k * ldb + n];
+        if (n < N && k < ke) v = TB ? B[(size_t)n * ldb + k] : B[(size_t)k * ldb + n];
         b[j][t] = (__bf16)v;
       }
     }
     for (int i = 0; i < 2; ++i)
       for (int j = 0; j < 2; ++j) acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[i], b[j], acc[i][j], 0, 0, 0);
   }
+  const bool split = gridDim.z > 1;
   for (int i = 0; i < 2; ++i)
     for (int j = 0; j < 2; ++j) {
       const int n = col0 + j * 16 + r16;
@@ -79,10 +86,19 @@ __global__ __launch_bounds__(256) void gemm_bf16(int M, int N, int K, const floa
         const int m = row0 + i * 16 + (lane >> 4) * 4 + t;
         if (m < M && n < N) {
           float* c = C + (size_t)m * ldc + n;
-          *c = accumulate ? *c + acc[i][j][t] : acc[i][j][t];
+          if (split)
+            atomicAdd(c, acc[i][j][t]);
+          else
+            *c = accumulate ? *c + acc[i][j][t] : acc[i][j][t];
         }
       }
     }
+}
+
+__global__ void zero_rows(float* C, int M, int N, int ldc) {
+  const size_t total = (size_t)M * N;
+  for (size_t k = blockIdx.x * (size_t)blockDim.x + threadIdx.x; k < total; k += (size_t)gridDim.x * blockDim.x)
+    C[(k / N) * ldc + k % N] = 0.f;
 }
 
 // ---- elementwise and reductions ---------------------------------------------------
@@ -125,13 +141,15 @@ __global__ void relu_mask(float* dH, const float* H, size_t total) {
     if (!(H[k] > 0.f)) dH[k] = 0.f;
 }
 
-// column sums of M[n x cols] into out (accumulated)
+// column sums of M[n x cols] added into out; blockIdx.y takes rows
+// [64 * blockIdx.y, 64 * blockIdx.y + 64)
 __global__ void colsum(const float* M, int n, int cols, float* out) {
   const int c = blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= cols) return;
+  const int r0 = blockIdx.y * 64, r1 = min(n, r0 + 64);
   float s = 0.f;
-  for (int r = 0; r < n; ++r) s += M[(size_t)r * cols + c];
-  out[c] += s;
+  for (int r = r0; r < r1; ++r) s += M[(size_t)r * cols + c];
+  atomicAdd(&out[c], s);
 }
 
 __device__ float block_reduce(float v, float* sm, bool is_max) {
@@ -271,12 +289,31 @@ __global__ __launch_bounds__(64) void top4(const float* L, int n, int C, const f
   }
 }
 
+template <bool TA, bool TB, int BN>
+hipError_t gemm_launch(hipStream_t s, int M, int N, int K, const float* A, int lda, const float* B, int ldb, float* C,
+                       int ldc, int accumulate) {
+  constexpr int BM = BN == 64 ? 64 : 128;
+  const int tiles = ((N + BN - 1) / BN) * ((M + BM - 1) / BM);
+  // split K when the output has too few tiles to fill the chip
+  int split = 1;
+  if (tiles < 256 && K >= 512) split = std::max(1, std::min(K / 256, (512 + tiles - 1) / tiles));
+  int k_chunk = (K + split - 1) / split;
+  k_chunk = (k_chunk + 31) / 32 * 32;
+  split = (K + k_chunk - 1) / k_chunk;
+  if (split > 1 && !accumulate) {
+    hipLaunchKernelGGL(zero_rows, dim3(256), dim3(256), 0, s, C, M, N, ldc);
+  }
+  dim3 grid((N + BN - 1) / BN, (M + BM - 1) / BM, split);
+  hipLaunchKernelGGL((gemm_bf16<TA, TB, BN>), grid, dim3(256), 0, s, M, N, K, k_chunk, A, lda, B, ldb, C, ldc,
+                     accumulate);
+  return hipGetLastError();
+}
+
 template <bool TA, bool TB>
 hipError_t gemm(hipStream_t s, int M, int N, int K, const float* A, int lda, const float* B, int ldb, float* C,
                 int ldc, int accumulate) {
-  dim3 grid((N + 63) / 64, (M + 63) / 64);
-  hipLaunchKernelGGL((gemm_bf16<TA, TB>), grid, dim3(256), 0, s, M, N, K, A, lda, B, ldb, C, ldc, accumulate);
-  return hipGetLastError();
+  if (N <= 32) return gemm_launch<TA, TB, 32>(s, M, N, K, A, lda, B, ldb, C, ldc, accumulate);
+  return gemm_launch<TA, TB, 64>(s, M, N, K, A, lda, B, ldb, C, ldc, accumulate);
 }
 
 }  // namespace
@@ -361,21 +398,22 @@ rtx_status mlp_fwd_bwd(rtx_mlp* m, const rtx_mlp_sample* s, uint32_t n, float in
   MK(hipGetLastError());
   // output layer: dO += dL^T . Q ; dc += colsum(dL) ; dQ = dL . O
   MK(gemm<true, false>(st, m->C, kEmb, n, m->L, m->C, m->Q, kEmb, G + m->oO, kEmb, 1));
-  hipLaunchKernelGGL(colsum, dim3((m->C + 255) / 256), dim3(256), 0, st, m->L, (int)n, (int)m->C, G + m->oc);
+  hipLaunchKernelGGL(colsum, dim3((m->C + 255) / 256, (n + 63) / 64), dim3(256), 0, st, m->L, (int)n, (int)m->C,
+                     G + m->oc);
   MK(gemm<false, false>(st, n, kEmb, m->C, m->L, m->C, P + m->oO, kEmb, m->dQ, kEmb, 0));
   // q = h2 W3 + b3
   MK(gemm<true, false>(st, kHid, kEmb, n, m->H2, kHid, m->dQ, kEmb, G + m->oW3, kEmb, 1));
-  hipLaunchKernelGGL(colsum, dim3(1), dim3(64), 0, st, m->dQ, (int)n, kEmb, G + m->ob3);
+  hipLaunchKernelGGL(colsum, dim3(1, (n + 63) / 64), dim3(64), 0, st, m->dQ, (int)n, kEmb, G + m->ob3);
   MK(gemm<false, true>(st, n, kHid, kEmb, m->dQ, kEmb, P + m->oW3, kEmb, m->dH2, kHid, 0));
   hipLaunchKernelGGL(relu_mask, dim3(256), dim3(256), 0, st, m->dH2, m->H2, (size_t)n * kHid);
   // h2 = relu(h1 W2 + b2)
   MK(gemm<true, false>(st, kHid, kHid, n, m->H1, kHid, m->dH2, kHid, G + m->oW2, kHid, 1));
-  hipLaunchKernelGGL(colsum, dim3(1), dim3(128), 0, st, m->dH2, (int)n, kHid, G + m->ob2);
+  hipLaunchKernelGGL(colsum, dim3(1, (n + 63) / 64), dim3(128), 0, st, m->dH2, (int)n, kHid, G + m->ob2);
   MK(gemm<false, true>(st, n, kHid, kHid, m->dH2, kHid, P + m->oW2, kHid, m->dH1, kHid, 0));
   hipLaunchKernelGGL(relu_mask, dim3(256), dim3(256), 0, st, m->dH1, m->H1, (size_t)n * kHid);
   // h1 = relu(x W1 + b1)
   MK(gemm<true, false>(st, kInPad, kHid, n, m->X, kInPad, m->dH1, kHid, G + m->oW1, kHid, 1));
-  hipLaunchKernelGGL(colsum, dim3(1), dim3(128), 0, st, m->dH1, (int)n, kHid, G + m->ob1);
+  hipLaunchKernelGGL(colsum, dim3(1, (n + 63) / 64), dim3(128), 0, st, m->dH1, (int)n, kHid, G + m->ob1);
   MK(gemm<false, true>(st, n, kInPad, kHid, m->dH1, kHid, P + m->oW1, kHid, m->dX, kInPad, 0));
   hipLaunchKernelGGL(scatter_embed, dim3(n), dim3(64), 0, st, m->dX, s, (int)n, (int)m->T, G + m->oE);
   MK(hipGetLastError());
