@@ -1,0 +1,96 @@
+"""GPU traversal against the hand-derived vectors of the tiny arena, and the
+edge cases: malformed arenas, empty batches, record-buffer overflow, budgets
+smaller than a node."""
+import os
+import sys
+
+import numpy as np
+import pytest
+
+sys.path.insert(0, os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden"))
+import tiny_scene  # noqa: E402
+
+pytestmark = pytest.mark.gpu
+
+
+def _check(h, off, rec, exp, levels):
+    got = [[int(r["offset"]), int(r["size"]), int(r["type"])] for r in rec]
+    assert got == exp["records"]
+    assert [int(r["level"]) for r in rec] == [levels[str(o)] for o, _, _ in exp["records"]]
+    for k, v in exp["hit"].items():
+        if isinstance(v, list):
+            assert list(h[0][k]) == v, k
+        else:
+            assert h[0][k] == v, k
+    assert int(off[1]) == len(exp["records"])
+
+
+@pytest.mark.parametrize("case", ["plain_opaque", "plain_non_opaque", "plain_terminate"])
+def test_tiny_plain_hand_derived(gpu, rtx, case):
+    exp, levels = tiny_scene.expected()[case], tiny_scene.expected()["levels"]
+    gpu.set_scene(tiny_scene.tiny_arena(), 0)
+    h, off, rec = gpu.trace(tiny_scene.tiny_ray(rtx.RAY_DTYPE, flags=exp["flags"]), mode=0, records=True)
+    _check(h, off, rec, exp, levels)
+
+
+@pytest.mark.parametrize("case", ["treelet_opaque", "treelet_terminate"])
+@pytest.mark.parametrize("budget", [8192, 256, 192])
+def test_tiny_treelet_hand_derived(gpu, rtx, case, budget):
+    exp, levels = tiny_scene.expected()[case], tiny_scene.expected()["levels"]
+    part = tiny_scene.expected()["partitions"][str(budget)]
+    gpu.set_scene(tiny_scene.tiny_arena(), 0)
+    gpu.build_treelets(budget)
+    t = gpu.get_treelets()
+    assert list(t["roots"]) == part["roots"]
+    h, off, rec = gpu.trace(tiny_scene.tiny_ray(rtx.RAY_DTYPE, flags=exp["flags"]), mode=1, records=True)
+    _check(h, off, rec, exp, levels)
+    for r in rec:
+        owner_root = max(root for root, lst in zip(part["roots"], part["lists"]) if int(r["offset"]) in lst)
+        assert part["roots"][int(r["treelet"])] == owner_root
+
+
+@pytest.mark.parametrize("budget", [128, 64])
+def test_tiny_budget_smaller_than_a_node(gpu, rtx, budget):
+    gpu.set_scene(tiny_scene.tiny_arena(), 0)
+    with pytest.raises(rtx.RtxError) as e:
+        gpu.build_treelets(budget)
+    assert e.value.status in (1, 4)  # 64 is below the accepted minimum, 128 cannot hold instance leaf + header
+    if budget == 128:
+        assert e.value.status == 4
+
+
+def test_malformed_arenas_report_like_the_oracle(gpu, rtx, oracle):
+    a = tiny_scene.tiny_arena()
+    ray = tiny_scene.tiny_ray(rtx.RAY_DTYPE)
+    bad = a.copy()
+    bad[64 + 12:64 + 16] = np.frombuffer(np.int32(1000).tobytes(), np.uint8)  # child offset out of the arena
+    gpu.set_scene(bad, 0)
+    with pytest.raises(rtx.RtxError) as e:
+        gpu.trace(ray, mode=0)
+    assert e.value.status == 3
+    bad = a.copy()
+    bad[64 + 22] = 2 | (4 << 2)  # TLAS child typed as a quad leaf
+    gpu.set_scene(bad, 0)
+    with pytest.raises(rtx.RtxError) as e:
+        gpu.trace(ray, mode=0)
+    assert e.value.status == 2
+    # the context still works afterwards
+    gpu.set_scene(a, 0)
+    h = gpu.trace(ray, mode=0)
+    assert h[0]["hit"] == 1 and h[0]["primitive_index"] == 10
+
+
+def test_empty_batch_and_record_overflow(gpu, rtx, c1):
+    scene, rays = c1
+    gpu.set_scene(scene.arena, scene.tlas_offset)
+    h, off, rec = gpu.trace(rays[:0], records=True)
+    assert len(h) == 0 and len(rec) == 0 and int(off[0]) == 0
+    n = 256
+    rb, hb, ob = gpu.to_device(rays[:n]), gpu.buffer(n * 64), gpu.buffer((n + 1) * 8)
+    recb = gpu.buffer(16 * 8)
+    st = gpu.trace_device(rb, n, hb, records_buf=recb, record_capacity=8, offsets_buf=ob)
+    assert st == 5  # RTX_ERR_TRACE_OVERFLOW: the count pass found more than 8 records
+    total = int(ob.download(np.uint64, n + 1)[-1])
+    assert total > 8
+    recb = gpu.buffer(total * 16)
+    assert gpu.trace_device(rb, n, hb, records_buf=recb, record_capacity=total, offsets_buf=ob) == 0
