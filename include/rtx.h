@@ -305,6 +305,13 @@ rtx_status rtx_batch_graph_destroy(rtx_batch_graph* g);
  * on the batch and return the fastest in *best (mode taken from best->mode). */
 rtx_status rtx_autotune_batch(rtx_ctx* ctx, const rtx_batch_job* jobs, uint32_t n_jobs, rtx_trace_options* best,
                               float* best_ms);
+/* Treelet-order batches: for each candidate budget, rebuild the partition of
+ * every scene the jobs name, then run rtx_autotune_batch; the fastest budget
+ * and launch shape win.  The scenes keep the partitions of the winning budget.
+ * best->mode must be RTX_MODE_TREELET; budgets are 64-byte multiples >= 256. */
+rtx_status rtx_autotune_batch_treelets(rtx_ctx* ctx, const rtx_batch_job* jobs, uint32_t n_jobs,
+                                       const uint32_t* budgets, uint32_t n_budgets, rtx_trace_options* best,
+                                       uint32_t* best_budget, float* best_ms);
 /* Duration of the traversal kernel of the last rtx_trace call (the hit pass),
  * measured with HIP events recorded on the context's stream right around it. */
 rtx_status rtx_last_kernel_ms(rtx_ctx* ctx, float* ms);

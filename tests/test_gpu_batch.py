@@ -77,3 +77,12 @@ def test_autotune(rtx, oracle, gpu, batch):
     assert best["waves_per_cu"] in (8, 16, 24, 32) and best["lds_stack"] in (4, 8, 16) and ms > 0
     assert gpu.trace_batch(jobs, **best) == 0
     _check(rtx, oracle, jobs, refs, 0)
+
+
+def test_autotune_treelet_budget(rtx, oracle, gpu, batch):
+    jobs, refs = batch
+    best, budget, ms = gpu.autotune_batch_treelets(jobs, budgets=(2048, 8192))
+    assert budget in (2048, 8192) and ms > 0 and best["mode"] == 1
+    assert gpu.trace_batch(jobs, **best) == 0
+    parts = [oracle.Partition(scene.arena, scene.tlas_offset, budget) for scene, _ in refs]
+    _check(rtx, oracle, jobs, refs, 1, parts)

@@ -823,4 +823,48 @@ rtx_status rtx_autotune_batch(rtx_ctx* c, const rtx_batch_job* jobs, uint32_t n_
   return RTX_OK;
 }
 
+rtx_status rtx_autotune_batch_treelets(rtx_ctx* c, const rtx_batch_job* jobs, uint32_t n_jobs, const uint32_t* budgets,
+                                       uint32_t n_budgets, rtx_trace_options* best, uint32_t* best_budget,
+                                       float* best_ms) {
+  if (!c || !jobs || !budgets || n_budgets == 0 || !best || !best_budget || best->mode != RTX_MODE_TREELET)
+    return RTX_ERR_INVALID_ARGUMENT;
+  std::vector<uint32_t> ids;
+  for (uint32_t j = 0; j < n_jobs; ++j) {
+    if (jobs[j].scene >= c->scenes.size()) return RTX_ERR_INVALID_ARGUMENT;
+    ids.push_back(jobs[j].scene);
+  }
+  std::sort(ids.begin(), ids.end());
+  ids.erase(std::unique(ids.begin(), ids.end()), ids.end());
+  for (uint32_t k = 0; k < n_budgets; ++k)
+    if (budgets[k] < 256 || budgets[k] % 64 != 0 || budgets[k] > (1u << 20)) return RTX_ERR_INVALID_ARGUMENT;
+  float win_ms = 1e30f;
+  uint32_t win_budget = budgets[0];
+  rtx_trace_options win = *best;
+  for (uint32_t k = 0; k < n_budgets; ++k) {
+    for (uint32_t id : ids) {
+      rtx_status st = rtx_build_scene_treelets(c, id, budgets[k], nullptr);
+      if (st != RTX_OK) return st;
+    }
+    rtx_trace_options o = *best;
+    float ms = 0.f;
+    rtx_status st = rtx_autotune_batch(c, jobs, n_jobs, &o, &ms);
+    if (st != RTX_OK) return st;
+    if (ms < win_ms) {
+      win_ms = ms;
+      win_budget = budgets[k];
+      win = o;
+    }
+  }
+  if (win_budget != budgets[n_budgets - 1]) {
+    for (uint32_t id : ids) {
+      rtx_status st = rtx_build_scene_treelets(c, id, win_budget, nullptr);
+      if (st != RTX_OK) return st;
+    }
+  }
+  *best = win;
+  *best_budget = win_budget;
+  if (best_ms) *best_ms = win_ms;
+  return RTX_OK;
+}
+
 }  // extern "C"

@@ -74,6 +74,11 @@ __device__ __forceinline__ bool finite3(F3 v) {
   return __builtin_isfinite(v.x) && __builtin_isfinite(v.y) && __builtin_isfinite(v.z);
 }
 
+// A ray whose slab products can never be NaN (given a non-NaN node origin).
+__device__ __forceinline__ bool slab_inputs_ok(F3 o, F3 idir, float tmin, float tmax) {
+  return finite3(o) && finite3(idir) && !__builtin_isnan(tmin) && !__builtin_isnan(tmax);
+}
+
 // mt_ray_triangle_test, vulkan_ray_tracing.cc:3089-3111
 __device__ __forceinline__ bool mt_test(F3 p0, F3 p1, F3 p2, F3 o, F3 d, float& thit) {
   F3 v0v1 = sub3(p1, p0);

@@ -147,6 +147,8 @@ __global__ __launch_bounds__(STAGE ? 64 : 256) void trace_kernel(TraceParams p) 
   F3 cobj_o{}, cobj_d{};
   ObjRay cur{};
   uint32_t cur_inst = kNoSlot;
+  // the world ray / current object ray can take the v_min/v_max slab tests
+  bool ray_ok = false, cur_ok = false;
   uint32_t n_rec = 0, n_nodes = 0, fetch_bytes = 0, n_any = 0, max_level = 0, steps = 0;
   uint64_t rec_base = 0;
   bool phase2 = false;
@@ -272,6 +274,7 @@ __global__ __launch_bounds__(STAGE ? 64 : 256) void trace_kernel(TraceParams p) 
                    __uint_as_float(b.x), __uint_as_float(b.y), __uint_as_float(b.z), __uint_as_float(b.w),
                    __uint_as_float(c.x), __uint_as_float(c.y), __uint_as_float(c.z), __uint_as_float(c.w)};
     cur = transform_ray(m, ro, rd, rtmin, rtmax);
+    cur_ok = slab_inputs_ok(cur.o, cur.idir, cur.tmin, cur.tmax);
     cur_inst = inst;
   };
 
@@ -427,6 +430,7 @@ __global__ __launch_bounds__(STAGE ? 64 : 256) void trace_kernel(TraceParams p) 
           rtmax = __uint_as_float(r1.w);
           rflags = r2.x;
           ridir = calc_idir(rd);
+          ray_ok = slab_inputs_ok(ro, ridir, rtmin, rtmax);
           octant = (rd.x < 0.f ? 1u : 0u) | (rd.y < 0.f ? 2u : 0u) | (rd.z < 0.f ? 4u : 0u);
           min_thit = rtmax;
           min_thit_obj = 0.f;
@@ -639,8 +643,7 @@ __global__ __launch_bounds__(STAGE ? 64 : 256) void trace_kernel(TraceParams p) 
           uint32_t hitmask = 0;
           // v_min/v_max slab tests when no lane can produce a NaN (finite
           // ray, non-NaN node origin); compare-selects otherwise
-          const bool ray_ok = finite3(o) && finite3(idir) && !__builtin_isnan(tmin) && !__builtin_isnan(tmax);
-          const bool fast = ray_ok && !__builtin_isnan(f_of(nd, 0) + f_of(nd, 1) + f_of(nd, 2));
+          const bool fast = (top ? ray_ok : cur_ok) && !__builtin_isnan(f_of(nd, 0) + f_of(nd, 1) + f_of(nd, 2));
           auto test_children = [&](auto fast_tag) {
             constexpr bool FAST = decltype(fast_tag)::value;
 #pragma unroll
@@ -718,6 +721,7 @@ __global__ __launch_bounds__(STAGE ? 64 : 256) void trace_kernel(TraceParams p) 
             const uint2 ro2 = *reinterpret_cast<const uint2*>(A + blas);
             emit(blas, 64, RTX_TX_BVH_STRUCTURE, level);
             cur = transform_ray(m, ro, rd, rtmin, rtmax);
+            cur_ok = slab_inputs_ok(cur.o, cur.idir, cur.tmin, cur.tmax);
             cur_inst = slot;
             if (slot >= (1u << 22)) err |= kErrInstSlot;
             const uint64_t root = blas + ((uint64_t)ro2.x | ((uint64_t)ro2.y << 32));

@@ -130,6 +130,7 @@ _sig("rtx_batch_graph_create", _i32, _vp, _P(_BatchJob), _u32, _P(_TraceOptions)
 _sig("rtx_batch_graph_launch", _i32, _vp, _u32, _P(_f32))
 _sig("rtx_batch_graph_destroy", _i32, _vp)
 _sig("rtx_autotune_batch", _i32, _vp, _P(_BatchJob), _u32, _P(_TraceOptions), _P(_f32))
+_sig("rtx_autotune_batch_treelets", _i32, _vp, _P(_BatchJob), _u32, _vp, _u32, _P(_TraceOptions), _P(_u32), _P(_f32))
 _sig("rtx_set_prefetch_table", _i32, _vp, _vp, _u64)
 _sig("rtx_mlp_create", _i32, _vp, _u32, _u32, _u64, _P(_vp))
 _sig("rtx_mlp_destroy", _i32, _vp)
@@ -166,7 +167,7 @@ EXPORTED_SYMBOLS = [
     "rtx_timer_stop", "rtx_device_info", "rtx_set_scene", "rtx_set_scene_device", "rtx_validate_scene",
     "rtx_build_treelets", "rtx_get_treelets", "rtx_set_treelets", "rtx_trace", "rtx_get_prefetch_stats",
     "rtx_last_kernel_ms", "rtx_add_scene", "rtx_build_scene_treelets", "rtx_trace_batch", "rtx_batch_graph_create",
-    "rtx_batch_graph_launch", "rtx_batch_graph_destroy", "rtx_autotune_batch",
+    "rtx_batch_graph_launch", "rtx_batch_graph_destroy", "rtx_autotune_batch", "rtx_autotune_batch_treelets",
     "rtx_set_prefetch_table", "rtx_mlp_create", "rtx_mlp_destroy", "rtx_mlp_param_count", "rtx_mlp_get_params",
     "rtx_mlp_set_params", "rtx_mlp_loss_grad", "rtx_mlp_train", "rtx_mlp_predict", "rtx_mlp_gemm",
     "rtx_builder_create", "rtx_builder_destroy", "rtx_builder_add_blas", "rtx_builder_add_instance",
@@ -655,6 +656,17 @@ class Context:
         _check(_lib.rtx_autotune_batch(self._h, arr, len(jobs), ctypes.byref(o), ctypes.byref(ms)),
                "rtx_autotune_batch")
         return {"mode": o.mode, "waves_per_cu": o.waves_per_cu, "lds_stack": o.lds_stack, "refill": o.refill}, ms.value
+
+    def autotune_batch_treelets(self, jobs, budgets=(4096, 8192, 16384, 32768, 65536)):
+        """Treelet-order batch: pick the treelet budget and launch shape; scenes keep the winning partitions."""
+        arr = self._jobs(jobs)
+        o = self._opts(mode=MODE_TREELET)
+        b = np.ascontiguousarray(budgets, np.uint32)
+        win, ms = ctypes.c_uint32(0), ctypes.c_float(0)
+        _check(_lib.rtx_autotune_batch_treelets(self._h, arr, len(jobs), _ptr(b), len(b), ctypes.byref(o),
+                                                ctypes.byref(win), ctypes.byref(ms)), "rtx_autotune_batch_treelets")
+        return ({"mode": o.mode, "waves_per_cu": o.waves_per_cu, "lds_stack": o.lds_stack, "refill": o.refill},
+                win.value, ms.value)
 
     def last_kernel_ms(self):
         ms = ctypes.c_float(0)
