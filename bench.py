@@ -149,8 +149,9 @@ def main():
                          "that calibrates the FETCH_SIZE / WRITE_SIZE counters")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (rays shard trivially across "
-              "GPUs, see DESIGN.md)", file=sys.stderr)
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported: this library traces on one "
+              "device (rays are independent, so a multi-GPU run would split the batch with the arena "
+              "replicated)", file=sys.stderr)
         return 2
 
     import rtx
