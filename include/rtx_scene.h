@@ -39,6 +39,11 @@ rtx_status rtx_builder_add_instance(rtx_builder* b, uint32_t blas_id, const floa
                                     uint32_t instance_id, uint32_t hit_group_offset);
 /* Build every BLAS (threads = 0: all hardware threads) and the TLAS, and lay out the arena. */
 rtx_status rtx_builder_finish(rtx_builder* b, uint32_t threads, uint64_t* arena_bytes, uint64_t* tlas_offset);
+/* The same, with every BLAS built on the device of `ctx` in one set of launches
+ * (Morton-ordered radix tree collapsed to 6-wide nodes, the same node layout);
+ * the TLAS is built on the host.  Faster to build, more nodes per ray than the
+ * SAH build. */
+rtx_status rtx_builder_finish_device(rtx_builder* b, rtx_ctx* ctx, uint64_t* arena_bytes, uint64_t* tlas_offset);
 rtx_status rtx_builder_copy_arena(const rtx_builder* b, void* dst, size_t capacity);
 /* Size of the finished arena and offset of the TLAS header inside it. */
 rtx_status rtx_builder_arena_info(const rtx_builder* b, uint64_t* arena_bytes, uint64_t* tlas_offset);
@@ -76,6 +81,8 @@ enum {
 
 /* Build the scene of a workload into a fresh builder (finished). */
 rtx_status rtx_workload_scene(const rtx_workload_desc* d, uint32_t threads, rtx_builder** out);
+/* The same scene with its BLASes built on the device of ctx (rtx_builder_finish_device). */
+rtx_status rtx_workload_scene_device(const rtx_workload_desc* d, rtx_ctx* ctx, rtx_builder** out);
 /* Pinhole primary rays through pixel centres, one per pixel, ordered by 8x8
  * pixel tiles (tiles row-major, pixels row-major inside a tile). */
 rtx_status rtx_workload_primary_rays(const rtx_workload_desc* d, rtx_ray* rays_out);

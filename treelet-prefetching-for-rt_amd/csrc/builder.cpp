@@ -19,6 +19,8 @@
 #include <thread>
 #include <vector>
 
+#include <hip/hip_runtime_api.h>
+
 #include "layout.h"
 #include "rtx_scene.h"
 
@@ -454,21 +456,21 @@ rtx_status rtx_builder_add_instance(rtx_builder* b, uint32_t blas_id, const floa
   return RTX_OK;
 }
 
-rtx_status rtx_builder_finish(rtx_builder* b, uint32_t threads, uint64_t* arena_bytes, uint64_t* tlas_offset) {
-  if (!b || b->finished || b->blas.empty() || b->inst.empty()) return RTX_ERR_INVALID_ARGUMENT;
-  unsigned nt = threads ? threads : std::max(1u, std::thread::hardware_concurrency());
-  nt = std::min<unsigned>(nt, (unsigned)b->blas.size());
-  std::atomic<size_t> next{0};
-  auto worker = [&]() {
-    for (size_t i = next++; i < b->blas.size(); i = next++) build_blas(b->blas[i]);
-  };
-  std::vector<std::thread> pool;
-  for (unsigned t = 1; t < nt; ++t) pool.emplace_back(worker);
-  worker();
-  for (auto& t : pool) t.join();
-  for (auto& bl : b->blas)
-    if (!bl.ok) return RTX_ERR_INVALID_ARGUMENT;
+}  // extern "C"
 
+namespace rtx_internal {
+rtx_status ctx_stream(rtx_ctx* c, hipStream_t* s, int* device);
+rtx_status gpu_build_blas_pool(hipStream_t s, const float* tris_host, const uint32_t* pofs_host,
+                               const uint32_t* geom_host, uint32_t n_blas, std::vector<uint8_t>& pool_out,
+                               std::vector<float>& boxes_out);
+}  // namespace rtx_internal
+
+namespace {
+
+// TLAS over the instances and the arena: TLAS at 0, then the bottom levels.
+// With `pool` set, the bottom levels are one pool whose BLAS b header sits at
+// byte 64 b; otherwise each BLAS's blob follows the previous one.
+rtx_status assemble(rtx_builder* b, const std::vector<uint8_t>* pool, uint64_t* arena_bytes, uint64_t* tlas_offset) {
   // Instance world boxes: the 8 transformed corners of the BLAS box, rounded outward.
   std::vector<Box> boxes(b->inst.size());
   for (size_t k = 0; k < b->inst.size(); ++k) {
@@ -523,11 +525,15 @@ rtx_status rtx_builder_finish(rtx_builder* b, uint32_t threads, uint64_t* arena_
       }
     }
   }
-  // Place TLAS at 0, then the BLAS blobs.
   uint64_t cursor = tlas.size();
-  for (auto& bl : b->blas) {
-    bl.offset = cursor;
-    cursor += bl.blob.size();
+  if (pool) {
+    for (size_t k = 0; k < b->blas.size(); ++k) b->blas[k].offset = cursor + 64 * k;
+    cursor += pool->size();
+  } else {
+    for (auto& bl : b->blas) {
+      bl.offset = cursor;
+      cursor += bl.blob.size();
+    }
   }
   try {
     b->arena.assign(cursor, 0);
@@ -535,9 +541,13 @@ rtx_status rtx_builder_finish(rtx_builder* b, uint32_t threads, uint64_t* arena_
     return RTX_ERR_OUT_OF_MEMORY;
   }
   memcpy(b->arena.data(), tlas.data(), tlas.size());
-  for (auto& bl : b->blas) {
-    memcpy(b->arena.data() + bl.offset, bl.blob.data(), bl.blob.size());
-    std::vector<uint8_t>().swap(bl.blob);
+  if (pool) {
+    memcpy(b->arena.data() + tlas.size(), pool->data(), pool->size());
+  } else {
+    for (auto& bl : b->blas) {
+      memcpy(b->arena.data() + bl.offset, bl.blob.data(), bl.blob.size());
+      std::vector<uint8_t>().swap(bl.blob);
+    }
   }
   for (size_t k = 0; k < b->inst.size(); ++k) {
     uint64_t lp = leaf_pos[k];
@@ -549,6 +559,71 @@ rtx_status rtx_builder_finish(rtx_builder* b, uint32_t threads, uint64_t* arena_
   if (arena_bytes) *arena_bytes = b->arena.size();
   if (tlas_offset) *tlas_offset = 0;
   return RTX_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+rtx_status rtx_builder_finish(rtx_builder* b, uint32_t threads, uint64_t* arena_bytes, uint64_t* tlas_offset) {
+  if (!b || b->finished || b->blas.empty() || b->inst.empty()) return RTX_ERR_INVALID_ARGUMENT;
+  unsigned nt = threads ? threads : std::max(1u, std::thread::hardware_concurrency());
+  nt = std::min<unsigned>(nt, (unsigned)b->blas.size());
+  std::atomic<size_t> next{0};
+  auto worker = [&]() {
+    for (size_t i = next++; i < b->blas.size(); i = next++) build_blas(b->blas[i]);
+  };
+  std::vector<std::thread> pool;
+  for (unsigned t = 1; t < nt; ++t) pool.emplace_back(worker);
+  worker();
+  for (auto& t : pool) t.join();
+  for (auto& bl : b->blas)
+    if (!bl.ok) return RTX_ERR_INVALID_ARGUMENT;
+  return assemble(b, nullptr, arena_bytes, tlas_offset);
+}
+
+rtx_status rtx_builder_finish_device(rtx_builder* b, rtx_ctx* ctx, uint64_t* arena_bytes, uint64_t* tlas_offset) {
+  if (!b || !ctx || b->finished || b->blas.empty() || b->inst.empty()) return RTX_ERR_INVALID_ARGUMENT;
+  hipStream_t s;
+  int dev = 0;
+  if (rtx_internal::ctx_stream(ctx, &s, &dev) != RTX_OK) return RTX_ERR_INVALID_ARGUMENT;
+  if (hipSetDevice(dev) != hipSuccess) return RTX_ERR_HIP;
+  const uint32_t B = (uint32_t)b->blas.size();
+  std::vector<uint32_t> pofs(B + 1, 0), geom(B);
+  uint64_t n = 0;
+  for (uint32_t k = 0; k < B; ++k) {
+    pofs[k] = (uint32_t)n;
+    const uint64_t nt = b->blas[k].tris.size() / 9;
+    if (nt == 0) return RTX_ERR_INVALID_ARGUMENT;
+    n += nt;
+    geom[k] = b->blas[k].geometry_index;
+  }
+  if (n >= (1ull << 31)) return RTX_ERR_INVALID_ARGUMENT;
+  pofs[B] = (uint32_t)n;
+  std::vector<float> tris;
+  try {
+    tris.resize(n * 9);
+  } catch (...) {
+    return RTX_ERR_OUT_OF_MEMORY;
+  }
+  for (uint32_t k = 0; k < B; ++k) memcpy(tris.data() + 9ull * pofs[k], b->blas[k].tris.data(), b->blas[k].tris.size() * 4);
+  std::vector<uint8_t> pool;
+  std::vector<float> boxes;
+  rtx_status st = rtx_internal::gpu_build_blas_pool(s, tris.data(), pofs.data(), geom.data(), B, pool, boxes);
+  if (st != RTX_OK) return st;
+  const uint64_t slots = pool.size() / 64;
+  for (uint32_t k = 0; k < B; ++k) {
+    Blas& bl = b->blas[k];
+    for (int a = 0; a < 3; ++a) {
+      bl.box.lo[a] = boxes[6 * k + a];
+      bl.box.hi[a] = boxes[6 * k + 3 + a];
+    }
+    bl.n_internal = 0;
+    bl.ok = true;
+  }
+  // every slot of the pool is a header, an internal node or a quad leaf
+  b->blas[0].n_internal = slots - B - n;
+  return assemble(b, &pool, arena_bytes, tlas_offset);
 }
 
 rtx_status rtx_builder_copy_arena(const rtx_builder* b, void* dst, size_t capacity) {

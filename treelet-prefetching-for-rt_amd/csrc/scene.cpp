@@ -112,11 +112,13 @@ rtx_status rtx_mesh_displaced_sphere(uint32_t slices, uint32_t stacks, float rad
   return RTX_OK;
 }
 
-rtx_status rtx_workload_scene(const rtx_workload_desc* d, uint32_t threads, rtx_builder** out) {
-  if (!d || !out) return RTX_ERR_INVALID_ARGUMENT;
-  rtx_builder* b = nullptr;
-  rtx_status st = rtx_builder_create(&b);
-  if (st != RTX_OK) return st;
+}  // extern "C"
+
+namespace {
+
+// Adds the workload's meshes and instances to a fresh builder.
+rtx_status populate_workload(const rtx_workload_desc* d, rtx_builder* b) {
+  rtx_status st = RTX_OK;
   const float identity[12] = {1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1, 0};
   if (d->kind == RTX_WORKLOAD_SPHERE_AND_QUAD) {
     uint32_t n = 2 * d->sphere_slices * d->sphere_stacks;
@@ -130,10 +132,8 @@ rtx_status rtx_workload_scene(const rtx_workload_desc* d, uint32_t threads, rtx_
     if (st == RTX_OK) st = rtx_builder_add_instance(b, sphere, identity, 0, 0);
     if (st == RTX_OK) st = rtx_builder_add_instance(b, quad, identity, 1, 0);
   } else if (d->kind == RTX_WORKLOAD_INSTANCED_SPHERES) {
-    if (d->n_instances == 0 || d->box_size <= 0.0f || d->scale_min <= 0.0f || d->scale_max < d->scale_min) {
-      rtx_builder_destroy(b);
+    if (d->n_instances == 0 || d->box_size <= 0.0f || d->scale_min <= 0.0f || d->scale_max < d->scale_min)
       return RTX_ERR_INVALID_ARGUMENT;
-    }
     uint32_t n = 2 * d->sphere_slices * d->sphere_stacks;
     std::vector<float> tris(9 * (size_t)n);
     uint32_t shared = 0;
@@ -158,7 +158,35 @@ rtx_status rtx_workload_scene(const rtx_workload_desc* d, uint32_t threads, rtx_
   } else {
     st = RTX_ERR_INVALID_ARGUMENT;
   }
+  return st;
+}
+
+}  // namespace
+
+extern "C" {
+
+rtx_status rtx_workload_scene(const rtx_workload_desc* d, uint32_t threads, rtx_builder** out) {
+  if (!d || !out) return RTX_ERR_INVALID_ARGUMENT;
+  rtx_builder* b = nullptr;
+  rtx_status st = rtx_builder_create(&b);
+  if (st != RTX_OK) return st;
+  st = populate_workload(d, b);
   if (st == RTX_OK) st = rtx_builder_finish(b, threads, nullptr, nullptr);
+  if (st != RTX_OK) {
+    rtx_builder_destroy(b);
+    return st;
+  }
+  *out = b;
+  return RTX_OK;
+}
+
+rtx_status rtx_workload_scene_device(const rtx_workload_desc* d, rtx_ctx* ctx, rtx_builder** out) {
+  if (!d || !ctx || !out) return RTX_ERR_INVALID_ARGUMENT;
+  rtx_builder* b = nullptr;
+  rtx_status st = rtx_builder_create(&b);
+  if (st != RTX_OK) return st;
+  st = populate_workload(d, b);
+  if (st == RTX_OK) st = rtx_builder_finish_device(b, ctx, nullptr, nullptr);
   if (st != RTX_OK) {
     rtx_builder_destroy(b);
     return st;

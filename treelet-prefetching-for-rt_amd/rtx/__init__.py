@@ -151,6 +151,8 @@ _sig("rtx_builder_destroy", _i32, _vp)
 _sig("rtx_builder_add_blas", _i32, _vp, _vp, _u32, _u32, _P(_u32))
 _sig("rtx_builder_add_instance", _i32, _vp, _u32, _vp, _u32, _u32)
 _sig("rtx_builder_finish", _i32, _vp, _u32, _P(_u64), _P(_u64))
+_sig("rtx_builder_finish_device", _i32, _vp, _vp, _P(_u64), _P(_u64))
+_sig("rtx_workload_scene_device", _i32, _P(_WorkloadDesc), _vp, _P(_vp))
 _sig("rtx_builder_copy_arena", _i32, _vp, _vp, ctypes.c_size_t)
 _sig("rtx_builder_arena_info", _i32, _vp, _P(_u64), _P(_u64))
 _sig("rtx_builder_counts", _i32, _vp, _P(_u64), _P(_u64), _P(_u64), _P(_u64))
@@ -171,7 +173,7 @@ EXPORTED_SYMBOLS = [
     "rtx_set_prefetch_table", "rtx_mlp_create", "rtx_mlp_destroy", "rtx_mlp_param_count", "rtx_mlp_get_params",
     "rtx_mlp_set_params", "rtx_mlp_loss_grad", "rtx_mlp_train", "rtx_mlp_predict", "rtx_mlp_gemm",
     "rtx_builder_create", "rtx_builder_destroy", "rtx_builder_add_blas", "rtx_builder_add_instance",
-    "rtx_builder_finish", "rtx_builder_copy_arena", "rtx_builder_arena_info", "rtx_builder_counts", "rtx_builder_blas_offset",
+    "rtx_builder_finish", "rtx_builder_finish_device", "rtx_workload_scene_device", "rtx_builder_copy_arena", "rtx_builder_arena_info", "rtx_builder_counts", "rtx_builder_blas_offset",
     "rtx_mesh_displaced_sphere", "rtx_workload_scene", "rtx_workload_primary_rays", "rtx_workload_secondary_rays",
     "rtx_format_treelet_trace", "rtx_as_dump_write", "rtx_as_dump_info", "rtx_as_dump_read",
 ]
@@ -272,9 +274,14 @@ class SceneBuilder:
         _check(_lib.rtx_builder_add_instance(self._h, blas_id, _ptr(m), instance_id, hit_group),
                "rtx_builder_add_instance")
 
-    def finish(self, threads=0):
+    def finish(self, threads=0, ctx=None):
+        """Build on the host (SAH), or with `ctx` the BLASes on that context's device."""
         nb, tl = ctypes.c_uint64(0), ctypes.c_uint64(0)
-        _check(_lib.rtx_builder_finish(self._h, threads, ctypes.byref(nb), ctypes.byref(tl)), "rtx_builder_finish")
+        if ctx is None:
+            _check(_lib.rtx_builder_finish(self._h, threads, ctypes.byref(nb), ctypes.byref(tl)), "rtx_builder_finish")
+        else:
+            _check(_lib.rtx_builder_finish_device(self._h, ctx._h, ctypes.byref(nb), ctypes.byref(tl)),
+                   "rtx_builder_finish_device")
         s = Scene(self._h.value)
         s._capture_finished()
         self._h = ctypes.c_void_p(None)
@@ -294,9 +301,13 @@ def workload_desc(kind=WORKLOAD_SPHERE_AND_QUAD, n_instances=2, slices=128, stac
                          scale_max, displacement, ray_flags, 0)
 
 
-def workload_scene(desc, threads=0):
+def workload_scene(desc, threads=0, ctx=None):
+    """Build a workload scene on the host, or with `ctx` its BLASes on that device."""
     h = ctypes.c_void_p()
-    _check(_lib.rtx_workload_scene(ctypes.byref(desc), threads, ctypes.byref(h)), "rtx_workload_scene")
+    if ctx is None:
+        _check(_lib.rtx_workload_scene(ctypes.byref(desc), threads, ctypes.byref(h)), "rtx_workload_scene")
+    else:
+        _check(_lib.rtx_workload_scene_device(ctypes.byref(desc), ctx._h, ctypes.byref(h)), "rtx_workload_scene_device")
     s = Scene(h.value)
     s._capture_finished()
     return s
