@@ -144,6 +144,8 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--width", type=int, default=0, help="override the image width (height follows the aspect)")
     ap.add_argument("--workload", default="diffuse", choices=["diffuse", "large"])
+    ap.add_argument("--build", default="host", choices=["host", "device"],
+                    help="BLAS builder: host SAH (default) or the device radix-tree builder")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="copy MIB MiB device to device once before tracing: a dispatch of known size "
                          "that calibrates the FETCH_SIZE / WRITE_SIZE counters")
@@ -162,12 +164,13 @@ def main():
     else:
         w = args.width or 7680
         desc = c5_desc(rtx, w, w * 9 // 16)
-    t0 = time.perf_counter()
-    scene = rtx.workload_scene(desc)
-    log(f"scene: {scene.n_triangles} tris, {scene.n_blas} BLAS, {scene.arena.nbytes / 2**20:.1f} MiB arena, "
-        f"built in {time.perf_counter() - t0:.1f} s")
     ctx = rtx.Context(0)
     info = ctx.device_info()
+    t0 = time.perf_counter()
+    scene = rtx.workload_scene(desc, ctx=ctx if args.build == "device" else None)
+    build_s = time.perf_counter() - t0
+    log(f"scene: {scene.n_triangles} tris, {scene.n_blas} BLAS, {scene.arena.nbytes / 2**20:.1f} MiB arena, "
+        f"built ({args.build}) in {build_s:.2f} s")
     ctx.set_scene(scene.arena, scene.tlas_offset)
     if args.copy_probe:
         a, b = ctx.buffer(args.copy_probe << 20), ctx.buffer(args.copy_probe << 20)
@@ -315,6 +318,8 @@ def main():
             "mode": args.mode,
             "partitions": {str(k): v for k, v in partitions.items()},
             "arena_mib": round(scene.arena.nbytes / 2**20, 2),
+            "bvh_build": args.build,
+            "bvh_build_s": round(build_s, 3),
             "device": info["name"],
             "n_cus": info["n_cus"],
         },
