@@ -62,8 +62,14 @@ struct Stack {
   uint32_t bot, cnt, g;
 };
 
+// LDS and global stack words are reached through pointers typed by address
+// space, so a pop compiles to ds_read or global_load and never to a flat load
+// (which would also wait for every outstanding store of the lane).
+typedef __attribute__((address_space(3))) uint64_t lds_u64;
+typedef __attribute__((address_space(1))) uint64_t glb_u64;
+
 template <int K, uint32_t BS>
-__device__ __forceinline__ bool st_push(Stack& s, uint64_t e, uint64_t* lds, uint64_t* g, uint32_t gcap) {
+__device__ __forceinline__ bool st_push(Stack& s, uint64_t e, lds_u64* lds, glb_u64* g, uint32_t gcap) {
   if (s.cnt == (uint32_t)K) {
     if (s.g >= gcap) return false;
     g[s.g++] = lds[s.bot * BS];
@@ -76,7 +82,7 @@ __device__ __forceinline__ bool st_push(Stack& s, uint64_t e, uint64_t* lds, uin
 }
 
 template <int K, uint32_t BS>
-__device__ __forceinline__ uint64_t st_pop(Stack& s, uint64_t* lds, uint64_t* g) {
+__device__ __forceinline__ uint64_t st_pop(Stack& s, lds_u64* lds, glb_u64* g) {
   if (s.cnt == 0) return g[--s.g];
   s.cnt--;
   return lds[((s.bot + s.cnt) & (K - 1)) * BS];
@@ -114,11 +120,11 @@ __global__ __launch_bounds__(STAGE ? 64 : 256) void trace_kernel(TraceParams p) 
 
   const uint32_t tid = threadIdx.x;
   const uint32_t lane = tid & 63u;
-  uint64_t* ldsA = lds_stack + tid;
-  uint64_t* ldsB = lds_stack + (NS - 1) * K * BS + tid;
+  lds_u64* ldsA = (lds_u64*)(lds_stack + tid);
+  lds_u64* ldsB = (lds_u64*)(lds_stack + (NS - 1) * K * BS + tid);
   const uint64_t lane_slot = (uint64_t)blockIdx.x * BS + tid;
-  uint64_t* gA = p.spill + lane_slot * (uint64_t)p.spill_cap * NS;
-  uint64_t* gB = gA + (NS - 1) * (uint64_t)p.spill_cap;
+  glb_u64* gA = (glb_u64*)(p.spill + lane_slot * (uint64_t)p.spill_cap * NS);
+  glb_u64* gB = gA + (NS - 1) * (uint64_t)p.spill_cap;
   // Scene of the lane's current ray (a batch kernel switches it per ray).
   const uint8_t* __restrict__ A = p.arena;
   uint64_t bytes = p.arena_bytes;
@@ -136,6 +142,7 @@ __global__ __launch_bounds__(STAGE ? 64 : 256) void trace_kernel(TraceParams p) 
 
   Stack sa{0, 0, 0}, sb{0, 0, 0};
   bool active = false, exhausted = false;
+  bool pending = false;  // the lane's completed ray still has to write its hit record
   uint32_t ray_id = 0;
   uint32_t err = 0;
 
@@ -386,11 +393,45 @@ __global__ __launch_bounds__(STAGE ? 64 : 256) void trace_kernel(TraceParams p) 
     return false;
   };
 
+  // Write the hit record of the lane's completed ray (:2990-3038).  Deferred
+  // to the wave's refill so the write path runs once per batch of completed
+  // rays instead of in nearly every iteration.
+  auto write_hit = [&]() {
+    uint4 o0 = make_uint4(0, 0, 0, 0), o1 = make_uint4(0, 0, 0, 0), o2 = make_uint4(0, 0, 0, 0);
+    if (err == 0 && min_thit < rtmax && closest_leaf != kNoSlot) {
+      const Words16 q = load64B(A + ((uint64_t)closest_leaf << 6));
+      const uint32_t inst_id = *reinterpret_cast<const uint32_t*>(A + ((uint64_t)closest_inst << 6) + 72);
+      const F3 v0 = mk3(f_of(q, 4), f_of(q, 5), f_of(q, 6));
+      const F3 v1 = mk3(f_of(q, 7), f_of(q, 8), f_of(q, 9));
+      const F3 v2 = mk3(f_of(q, 10), f_of(q, 11), f_of(q, 12));
+      const F3 ip = add3(ro, mk3(rd.x * min_thit, rd.y * min_thit, rd.z * min_thit));
+      const F3 op =
+          add3(cobj_o, mk3(cobj_d.x * min_thit_obj, cobj_d.y * min_thit_obj, cobj_d.z * min_thit_obj));
+      const F3 bc = barycentric(op, v0, v1, v2);
+      o0 = make_uint4(1u, q.w[2], q.w[1] & 0x0fffffffu, inst_id);
+      o1 = make_uint4(__float_as_uint(min_thit), __float_as_uint(ip.x), __float_as_uint(ip.y),
+                      __float_as_uint(ip.z));
+      o2 = make_uint4(__float_as_uint(bc.x), __float_as_uint(bc.y), __float_as_uint(bc.z), n_rec);
+    } else {
+      o2.w = n_rec;
+    }
+    uint4* hq = reinterpret_cast<uint4*>(hits_out + local_id);
+    hq[0] = o0;
+    hq[1] = o1;
+    hq[2] = o2;
+    hq[3] = make_uint4(n_nodes, fetch_bytes, n_any, max_level);
+    if (err) atomicOr(p.error, err);
+  };
+
   while (true) {
     // ---- refill idle lanes from the global ray counter (wave converged) ----
     const uint64_t idle = __ballot(!active);
     const uint32_t nidle = (uint32_t)__popcll(idle);
     if (!exhausted && nidle > 0 && (nidle == 64u || nidle >= p.refill)) {
+      if (pending) {
+        write_hit();
+        pending = false;
+      }
       const uint32_t leader = (uint32_t)__ffsll((long long)idle) - 1u;
       uint32_t base = 0;
       if (lane == leader) base = atomicAdd(p.ray_counter, nidle);
@@ -489,31 +530,8 @@ __global__ __launch_bounds__(STAGE ? 64 : 256) void trace_kernel(TraceParams p) 
         has_e = true;
         if (++steps > kMaxStepsPerRay) err |= kErrSteps;
       } else {
-        // ray complete: write the hit record (:2990-3038)
-        uint4 o0 = make_uint4(0, 0, 0, 0), o1 = make_uint4(0, 0, 0, 0), o2 = make_uint4(0, 0, 0, 0);
-        if (err == 0 && min_thit < rtmax && closest_leaf != kNoSlot) {
-          const Words16 q = load64B(A + ((uint64_t)closest_leaf << 6));
-          const uint32_t inst_id = *reinterpret_cast<const uint32_t*>(A + ((uint64_t)closest_inst << 6) + 72);
-          const F3 v0 = mk3(f_of(q, 4), f_of(q, 5), f_of(q, 6));
-          const F3 v1 = mk3(f_of(q, 7), f_of(q, 8), f_of(q, 9));
-          const F3 v2 = mk3(f_of(q, 10), f_of(q, 11), f_of(q, 12));
-          const F3 ip = add3(ro, mk3(rd.x * min_thit, rd.y * min_thit, rd.z * min_thit));
-          const F3 op =
-              add3(cobj_o, mk3(cobj_d.x * min_thit_obj, cobj_d.y * min_thit_obj, cobj_d.z * min_thit_obj));
-          const F3 bc = barycentric(op, v0, v1, v2);
-          o0 = make_uint4(1u, q.w[2], q.w[1] & 0x0fffffffu, inst_id);
-          o1 = make_uint4(__float_as_uint(min_thit), __float_as_uint(ip.x), __float_as_uint(ip.y),
-                          __float_as_uint(ip.z));
-          o2 = make_uint4(__float_as_uint(bc.x), __float_as_uint(bc.y), __float_as_uint(bc.z), n_rec);
-        } else {
-          o2.w = n_rec;
-        }
-        uint4* hq = reinterpret_cast<uint4*>(hits_out + local_id);
-        hq[0] = o0;
-        hq[1] = o1;
-        hq[2] = o2;
-        hq[3] = make_uint4(n_nodes, fetch_bytes, n_any, max_level);
-        if (err) atomicOr(p.error, err);
+        // ray complete; its hit record is written at the wave's next refill
+        pending = true;
         active = false;
       }
     }
@@ -782,6 +800,8 @@ __global__ __launch_bounds__(STAGE ? 64 : 256) void trace_kernel(TraceParams p) 
       }
     }
   }
+
+  if (pending) write_hit();
 
   if constexpr (STAGE > 0) {
     evict_all();
