@@ -433,9 +433,9 @@ rtx_status rtx_trace(rtx_ctx* c, const rtx_ray* rays, uint32_t n, const rtx_trac
     HIP_TRY(hipStreamSynchronize(c->stream));
     return RTX_OK;
   }
-  // 16 LDS entries per lane for plain order (one stack, 32 KiB per block);
-  // treelet order keeps two stacks, so 8 (the same LDS per block)
-  uint32_t lds_stack = opts->lds_stack ? opts->lds_stack : (opts->mode == RTX_MODE_PLAIN ? 16 : 8);
+  // 8 LDS entries per lane: 16 KiB per 256-lane block for plain order, so the
+  // register budget, not LDS, bounds how many waves share a CU
+  uint32_t lds_stack = opts->lds_stack ? opts->lds_stack : 8;
   if (lds_stack != 4 && lds_stack != 8 && lds_stack != 16) return RTX_ERR_INVALID_ARGUMENT;
   // treelet staging: one wave per workgroup with stage_bytes of LDS for the image
   uint32_t stage = 0;
@@ -454,8 +454,10 @@ rtx_status rtx_trace(rtx_ctx* c, const rtx_ray* rays, uint32_t n, const rtx_trac
     lds_stack = 8;
   }
   const uint32_t bs = trace_block_size(stage);
-  const uint32_t waves_per_cu = opts->waves_per_cu ? opts->waves_per_cu : 16;
-  const uint32_t blocks_per_cu = std::max(1u, waves_per_cu * 64 / bs);
+  // persistent blocks: by default as many as fit on a CU at once
+  uint32_t blocks_per_cu = opts->waves_per_cu ? std::max(1u, opts->waves_per_cu * 64 / bs)
+                                              : trace_blocks_per_cu(opts->mode, false, lds_stack, stage);
+  if (blocks_per_cu == 0) blocks_per_cu = 16 * 64 / bs;
   uint32_t grid = (uint32_t)c->n_cus * blocks_per_cu;
   grid = std::min<uint32_t>(grid, (n + bs - 1) / bs);
   grid = std::max(grid, 1u);
@@ -628,8 +630,9 @@ rtx_status batch_shape(const rtx_ctx* c, const rtx_trace_options* o, uint64_t to
   if (s->lds_stack != 4 && s->lds_stack != 8 && s->lds_stack != 16) return RTX_ERR_INVALID_ARGUMENT;
   if (o->refill > 64) return RTX_ERR_INVALID_ARGUMENT;
   s->refill = o->refill ? o->refill : 32;
-  const uint32_t waves = o->waves_per_cu ? o->waves_per_cu : 16;
-  uint32_t grid = (uint32_t)c->n_cus * std::max(1u, waves / 4);
+  uint32_t per_cu = o->waves_per_cu ? std::max(1u, o->waves_per_cu / 4) : trace_batch_blocks_per_cu(o->mode, s->lds_stack);
+  if (per_cu == 0) per_cu = 4;
+  uint32_t grid = (uint32_t)c->n_cus * per_cu;
   grid = (uint32_t)std::min<uint64_t>(grid, (total + kBlock - 1) / kBlock);
   s->grid = std::max(grid, 1u);
   s->ns = o->mode == RTX_MODE_TREELET ? 2 : 1;
@@ -788,9 +791,9 @@ rtx_status rtx_batch_graph_destroy(rtx_batch_graph* g) {
 rtx_status rtx_autotune_batch(rtx_ctx* c, const rtx_batch_job* jobs, uint32_t n_jobs, rtx_trace_options* best,
                               float* best_ms) {
   if (!c || !best) return RTX_ERR_INVALID_ARGUMENT;
-  const uint32_t waves[] = {8, 16, 24, 32};
+  const uint32_t waves[] = {0, 8, 12, 16, 20};  // 0: as many as fit on a CU
   const uint32_t stacks[] = {4, 8, 16};
-  const uint32_t refills[] = {32, 48, 64};
+  const uint32_t refills[] = {24, 32, 48};
   rtx_trace_options o = *best;
   o.prefetch = RTX_PREFETCH_NONE;
   o.emit_records = 0;

@@ -151,12 +151,11 @@ __global__ __launch_bounds__(STAGE ? 64 : 256) void trace_kernel(TraceParams p) 
   uint32_t rflags = 0;
   float min_thit = 0.f, min_thit_obj = 0.f;
   uint32_t closest_leaf = kNoSlot, closest_inst = kNoSlot;
-  F3 cobj_o{}, cobj_d{};
   ObjRay cur{};
   uint32_t cur_inst = kNoSlot;
   // the world ray / current object ray can take the v_min/v_max slab tests
   bool ray_ok = false, cur_ok = false;
-  uint32_t n_rec = 0, n_nodes = 0, fetch_bytes = 0, n_any = 0, max_level = 0, steps = 0;
+  uint32_t n_rec = 0, n_nodes = 0, fetch_bytes = 0, n_any = 0, max_level = 0;
   uint64_t rec_base = 0;
   bool phase2 = false;
 
@@ -405,8 +404,15 @@ __global__ __launch_bounds__(STAGE ? 64 : 256) void trace_kernel(TraceParams p) 
       const F3 v1 = mk3(f_of(q, 7), f_of(q, 8), f_of(q, 9));
       const F3 v2 = mk3(f_of(q, 10), f_of(q, 11), f_of(q, 12));
       const F3 ip = add3(ro, mk3(rd.x * min_thit, rd.y * min_thit, rd.z * min_thit));
-      const F3 op =
-          add3(cobj_o, mk3(cobj_d.x * min_thit_obj, cobj_d.y * min_thit_obj, cobj_d.z * min_thit_obj));
+      // the object-space ray of the closest hit's instance, recomputed from the
+      // instance leaf (the same inputs as when the hit was accepted)
+      const uint4* iq = reinterpret_cast<const uint4*>(A + ((uint64_t)closest_inst << 6) + 16);
+      const uint4 ia = iq[0], ib = iq[1], ic = iq[2];
+      const float im[12] = {__uint_as_float(ia.x), __uint_as_float(ia.y), __uint_as_float(ia.z), __uint_as_float(ia.w),
+                            __uint_as_float(ib.x), __uint_as_float(ib.y), __uint_as_float(ib.z), __uint_as_float(ib.w),
+                            __uint_as_float(ic.x), __uint_as_float(ic.y), __uint_as_float(ic.z), __uint_as_float(ic.w)};
+      const ObjRay hr = transform_ray(im, ro, rd, rtmin, rtmax);
+      const F3 op = add3(hr.o, mk3(hr.d.x * min_thit_obj, hr.d.y * min_thit_obj, hr.d.z * min_thit_obj));
       const F3 bc = barycentric(op, v0, v1, v2);
       o0 = make_uint4(1u, q.w[2], q.w[1] & 0x0fffffffu, inst_id);
       o1 = make_uint4(__float_as_uint(min_thit), __float_as_uint(ip.x), __float_as_uint(ip.y),
@@ -423,6 +429,13 @@ __global__ __launch_bounds__(STAGE ? 64 : 256) void trace_kernel(TraceParams p) 
     if (err) atomicOr(p.error, err);
   };
 
+  // the TLAS header of a single-scene launch, read once
+  uint4 tl_h0 = make_uint4(0, 0, 0, 0), tl_h1 = make_uint4(0, 0, 0, 0);
+  if (!MULTI && tlas + 64 <= bytes) {
+    const uint4* hq = reinterpret_cast<const uint4*>(A + tlas);
+    tl_h0 = hq[0];
+    tl_h1 = hq[1];
+  }
   while (true) {
     // ---- refill idle lanes from the global ray counter (wave converged) ----
     const uint64_t idle = __ballot(!active);
@@ -478,7 +491,7 @@ __global__ __launch_bounds__(STAGE ? 64 : 256) void trace_kernel(TraceParams p) 
           closest_leaf = kNoSlot;
           closest_inst = kNoSlot;
           cur_inst = kNoSlot;
-          n_rec = n_nodes = fetch_bytes = n_any = steps = 0;
+          n_rec = n_nodes = fetch_bytes = n_any = 0;
           max_level = 1;  // tree_level_map[topRootAddr] = 1
           rec_base = REC ? p.record_offsets[r] : 0;
           sa = Stack{0, 0, 0};
@@ -491,8 +504,15 @@ __global__ __launch_bounds__(STAGE ? 64 : 256) void trace_kernel(TraceParams p) 
             err |= kErrRange;
           } else {
             emit(t, 64, RTX_TX_BVH_STRUCTURE, 0);
-            const uint4* hq = reinterpret_cast<const uint4*>(A + t);
-            uint4 h0 = hq[0], h1 = hq[1];
+            uint4 h0, h1;
+            if constexpr (MULTI) {
+              const uint4* hq = reinterpret_cast<const uint4*>(A + t);
+              h0 = hq[0];
+              h1 = hq[1];
+            } else {
+              h0 = tl_h0;
+              h1 = tl_h1;
+            }
             const uint64_t root = t + ((uint64_t)h0.x | ((uint64_t)h0.y << 32));
             F3 lo = mk3(__uint_as_float(h0.z), __uint_as_float(h0.w), __uint_as_float(h1.x));
             F3 hi = mk3(__uint_as_float(h1.y), __uint_as_float(h1.z), __uint_as_float(h1.w));
@@ -528,7 +548,8 @@ __global__ __launch_bounds__(STAGE ? 64 : 256) void trace_kernel(TraceParams p) 
           e = st_pop<K, BS>(sb, ldsB, gB);
         }
         has_e = true;
-        if (++steps > kMaxStepsPerRay) err |= kErrSteps;
+        // every processed entry emits at least one record
+        if (n_rec > kMaxStepsPerRay) err |= kErrSteps;
       } else {
         // ray complete; its hit record is written at the wave's next refill
         pending = true;
@@ -655,9 +676,7 @@ __global__ __launch_bounds__(STAGE ? 64 : 256) void trace_kernel(TraceParams p) 
             tmax = cur.tmax;
             cull = min_thit * cur.tmult;
           }
-          uint64_t child = off + (uint64_t)((int64_t)(int32_t)nd.w[3] * 64);
-          uint64_t coff[6];
-          uint32_t ctype[6];
+          const uint64_t first_child = off + (uint64_t)((int64_t)(int32_t)nd.w[3] * 64);
           uint32_t hitmask = 0;
           // v_min/v_max slab tests when no lane can produce a NaN (finite
           // ray, non-NaN node origin); compare-selects otherwise
@@ -668,9 +687,6 @@ __global__ __launch_bounds__(STAGE ? 64 : 256) void trace_kernel(TraceParams p) 
             for (int i = 0; i < 6; ++i) {
               const uint32_t info = byte_of(nd, 22 + i) & 0x3fu;
               const uint32_t size = info & 3u;
-              ctype[i] = info >> 2;
-              coff[i] = child;
-              child += (uint64_t)size * 64;
               if (size > 0) {
                 F3 lo, hi;
                 child_box(nd, i, ex, ey, ez, lo, hi);
@@ -691,12 +707,17 @@ __global__ __launch_bounds__(STAGE ? 64 : 256) void trace_kernel(TraceParams p) 
             const uint32_t inst = top ? 0u : e_inst(e);
             const uint32_t base_flags = top ? kFlTop : 0u;
             int deferred = -1;
+            uint32_t deferred_cs = 0;
+            uint64_t child = first_child;
 #pragma unroll
             for (int i = 0; i < 6; ++i) {
+              const uint32_t info = byte_of(nd, 22 + i) & 0x3fu;
+              const uint64_t c = child;
+              child += (uint64_t)(info & 3u) * 64;
               if (!(hitmask & (1u << i))) continue;
-              const uint64_t c = coff[i];
-              const bool leaf = ctype[i] != 0;
-              if (top && leaf && ctype[i] != 1u) err |= kErrType;
+              const uint32_t ctype = info >> 2;
+              const bool leaf = ctype != 0;
+              if (top && leaf && ctype != 1u) err |= kErrType;
               if ((c & 63) || c + 64 > bytes) {
                 err |= kErrRange;
                 continue;
@@ -704,13 +725,13 @@ __global__ __launch_bounds__(STAGE ? 64 : 256) void trace_kernel(TraceParams p) 
               const uint32_t cs = (uint32_t)(c >> 6);
               if (MODE == 0 && !leaf && deferred < 0) {
                 deferred = i;
+                deferred_cs = cs;
                 continue;
               }
               push_entry(make_entry(cs, base_flags | (leaf ? kFlLeaf : 0u), lvl, inst), goes_current(cs));
             }
             if (MODE == 0 && deferred >= 0) {
-              const uint32_t cs = (uint32_t)(coff[deferred] >> 6);
-              push_entry(make_entry(cs, base_flags, lvl, inst), true);
+              push_entry(make_entry(deferred_cs, base_flags, lvl, inst), true);
             }
           }
         }
@@ -779,8 +800,6 @@ __global__ __launch_bounds__(STAGE ? 64 : 256) void trace_kernel(TraceParams p) 
               min_thit_obj = th;
               closest_leaf = slot;
               closest_inst = cur_inst;
-              cobj_o = cur.o;
-              cobj_d = cur.d;
               emit(off, 64, RTX_TX_BVH_QUAD_LEAF_HIT, level);
               n_nodes++;
               if (MODE == 0 && !(rflags & RTX_RAY_FLAG_OPAQUE)) n_any++;
@@ -885,37 +904,57 @@ __global__ __launch_bounds__(kScanBlock) void scan_add(uint64_t* out, uint32_t n
     if (base + k < n) out[base + k] += add;
 }
 
-template <int MODE, bool REC, int K, int STAGE>
-hipError_t launch_one(const TraceParams& p, uint32_t grid, hipStream_t s) {
-  constexpr uint32_t BS = STAGE ? 64u : 256u;
-  hipLaunchKernelGGL((trace_kernel<MODE, REC, K, STAGE>), dim3(grid), dim3(BS), 0, s, p);
-  return hipGetLastError();
-}
+using TraceFn = void (*)(TraceParams);
 
 template <int MODE, bool REC>
-hipError_t launch_k(const TraceParams& p, uint32_t lds_stack, uint32_t stage, uint32_t grid, hipStream_t s) {
+TraceFn pick_k(uint32_t lds_stack, uint32_t stage) {
   if (stage == 0) {
     switch (lds_stack) {
-      case 4: return launch_one<MODE, REC, 4, 0>(p, grid, s);
-      case 8: return launch_one<MODE, REC, 8, 0>(p, grid, s);
-      case 16: return launch_one<MODE, REC, 16, 0>(p, grid, s);
-      default: return hipErrorInvalidValue;
+      case 4: return trace_kernel<MODE, REC, 4, 0>;
+      case 8: return trace_kernel<MODE, REC, 8, 0>;
+      case 16: return trace_kernel<MODE, REC, 16, 0>;
+      default: return nullptr;
     }
   }
   switch (stage) {
-    case 4096: return launch_one<MODE, REC, 8, 4096>(p, grid, s);
-    case 8192: return launch_one<MODE, REC, 8, 8192>(p, grid, s);
-    case 16384: return launch_one<MODE, REC, 8, 16384>(p, grid, s);
-    case 32768: return launch_one<MODE, REC, 8, 32768>(p, grid, s);
-    case 65536: return launch_one<MODE, REC, 8, 65536>(p, grid, s);
-    default: return hipErrorInvalidValue;
+    case 4096: return trace_kernel<MODE, REC, 8, 4096>;
+    case 8192: return trace_kernel<MODE, REC, 8, 8192>;
+    case 16384: return trace_kernel<MODE, REC, 8, 16384>;
+    case 32768: return trace_kernel<MODE, REC, 8, 32768>;
+    case 65536: return trace_kernel<MODE, REC, 8, 65536>;
+    default: return nullptr;
   }
 }
 
-template <int MODE, int K>
-hipError_t launch_batch_one(const TraceParams& p, uint32_t grid, hipStream_t s) {
-  hipLaunchKernelGGL((trace_kernel<MODE, false, K, 0, true>), dim3(grid), dim3(256), 0, s, p);
-  return hipGetLastError();
+TraceFn pick_trace(uint32_t mode, bool records, uint32_t lds_stack, uint32_t stage) {
+  if (mode == RTX_MODE_PLAIN)
+    return records ? pick_k<0, true>(lds_stack, stage) : pick_k<0, false>(lds_stack, stage);
+  return records ? pick_k<1, true>(lds_stack, stage) : pick_k<1, false>(lds_stack, stage);
+}
+
+TraceFn pick_batch(uint32_t mode, uint32_t lds_stack) {
+  const bool plain = mode == RTX_MODE_PLAIN;
+  switch (lds_stack) {
+    case 4: return plain ? trace_kernel<0, false, 4, 0, true> : trace_kernel<1, false, 4, 0, true>;
+    case 8: return plain ? trace_kernel<0, false, 8, 0, true> : trace_kernel<1, false, 8, 0, true>;
+    case 16: return plain ? trace_kernel<0, false, 16, 0, true> : trace_kernel<1, false, 16, 0, true>;
+    default: return nullptr;
+  }
+}
+
+hipError_t launch_fn(TraceFn f, const TraceParams& p, uint32_t grid, uint32_t bs, hipStream_t s) {
+  if (!f) return hipErrorInvalidValue;
+  TraceParams arg = p;
+  void* args[] = {&arg};
+  return hipLaunchKernel(reinterpret_cast<const void*>(f), dim3(grid), dim3(bs), args, 0, s);
+}
+
+uint32_t blocks_per_cu(TraceFn f, uint32_t bs) {
+  int nb = 0;
+  if (!f || hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, reinterpret_cast<const void*>(f), (int)bs, 0) !=
+                hipSuccess)
+    return 0;
+  return nb > 0 ? (uint32_t)nb : 0u;
 }
 
 __global__ void fill_zero_kernel(uint32_t* dst, uint32_t words) {
@@ -927,20 +966,11 @@ __global__ void fill_zero_kernel(uint32_t* dst, uint32_t words) {
 hipError_t launch_trace_batch(const TraceParams& p, uint32_t mode, uint32_t lds_stack, uint32_t grid,
                               hipStream_t s) {
   if (p.n_jobs == 0 || p.n_jobs > kMaxBatchJobs) return hipErrorInvalidValue;
-  if (mode == RTX_MODE_PLAIN) {
-    switch (lds_stack) {
-      case 4: return launch_batch_one<0, 4>(p, grid, s);
-      case 8: return launch_batch_one<0, 8>(p, grid, s);
-      case 16: return launch_batch_one<0, 16>(p, grid, s);
-      default: return hipErrorInvalidValue;
-    }
-  }
-  switch (lds_stack) {
-    case 4: return launch_batch_one<1, 4>(p, grid, s);
-    case 8: return launch_batch_one<1, 8>(p, grid, s);
-    case 16: return launch_batch_one<1, 16>(p, grid, s);
-    default: return hipErrorInvalidValue;
-  }
+  return launch_fn(pick_batch(mode, lds_stack), p, grid, 256, s);
+}
+
+uint32_t trace_batch_blocks_per_cu(uint32_t mode, uint32_t lds_stack) {
+  return blocks_per_cu(pick_batch(mode, lds_stack), 256);
 }
 
 hipError_t launch_fill_zero(uint32_t* dst, uint32_t words, hipStream_t s) {
@@ -952,11 +982,11 @@ uint32_t trace_block_size(uint32_t stage_bytes) { return stage_bytes ? 64u : 256
 
 hipError_t launch_trace(const TraceParams& p, uint32_t mode, bool records, uint32_t lds_stack, uint32_t stage_bytes,
                         uint32_t grid, hipStream_t s) {
-  if (mode == RTX_MODE_PLAIN)
-    return records ? launch_k<0, true>(p, lds_stack, stage_bytes, grid, s)
-                   : launch_k<0, false>(p, lds_stack, stage_bytes, grid, s);
-  return records ? launch_k<1, true>(p, lds_stack, stage_bytes, grid, s)
-                 : launch_k<1, false>(p, lds_stack, stage_bytes, grid, s);
+  return launch_fn(pick_trace(mode, records, lds_stack, stage_bytes), p, grid, trace_block_size(stage_bytes), s);
+}
+
+uint32_t trace_blocks_per_cu(uint32_t mode, bool records, uint32_t lds_stack, uint32_t stage_bytes) {
+  return blocks_per_cu(pick_trace(mode, records, lds_stack, stage_bytes), trace_block_size(stage_bytes));
 }
 
 hipError_t launch_scan_records(const rtx_hit* hits, uint32_t n, uint64_t* offsets, uint64_t* tile_sums,
