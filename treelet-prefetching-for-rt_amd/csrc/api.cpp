@@ -629,7 +629,9 @@ rtx_status batch_shape(const rtx_ctx* c, const rtx_trace_options* o, uint64_t to
   s->lds_stack = o->lds_stack ? o->lds_stack : 8;
   if (s->lds_stack != 4 && s->lds_stack != 8 && s->lds_stack != 16) return RTX_ERR_INVALID_ARGUMENT;
   if (o->refill > 64) return RTX_ERR_INVALID_ARGUMENT;
-  s->refill = o->refill ? o->refill : 32;
+  // batches mix camera, shadow and bounce rays; larger refills keep the
+  // coherent ones together
+  s->refill = o->refill ? o->refill : 48;
   uint32_t per_cu = o->waves_per_cu ? std::max(1u, o->waves_per_cu / 4) : trace_batch_blocks_per_cu(o->mode, s->lds_stack);
   if (per_cu == 0) per_cu = 4;
   uint32_t grid = (uint32_t)c->n_cus * per_cu;

@@ -122,7 +122,7 @@ struct ObjRay {
 
 __device__ __forceinline__ ObjRay transform_ray(const float m[12], F3 o, F3 d, float tmin, float tmax) {
   // column i of the product (i = 0..2) and w (column 3)
-  float ox = 0.0f, oy = 0.0f, oz = 0.0f, ow = 0.0f;
+  float ox = 0.0f, oy = 0.0f, oz = 0.0f;
   ox += m[0] * o.x;
   ox += m[3] * o.y;
   ox += m[6] * o.z;
@@ -135,10 +135,6 @@ __device__ __forceinline__ ObjRay transform_ray(const float m[12], F3 o, F3 d, f
   oz += m[5] * o.y;
   oz += m[8] * o.z;
   oz += m[11] * 1.0f;
-  ow += 0.0f * o.x;
-  ow += 0.0f * o.y;
-  ow += 0.0f * o.z;
-  ow += 1.0f * 1.0f;
   float dx = 0.0f, dy = 0.0f, dz = 0.0f;
   dx += m[0] * d.x;
   dx += m[3] * d.y;
@@ -153,7 +149,18 @@ __device__ __forceinline__ ObjRay transform_ray(const float m[12], F3 o, F3 d, f
   dz += m[8] * d.z;
   dz += m[11] * 0.0f;
   ObjRay r;
-  r.o = mk3(ox / ow, oy / ow, oz / ow);
+  // w = (((0 + 0*o.x) + 0*o.y) + 0*o.z) + 1 is exactly 1 for a finite origin,
+  // and x / 1 = x: only a non-finite origin takes the divisions
+  if (finite3(o)) {
+    r.o = mk3(ox, oy, oz);
+  } else {
+    float ow = 0.0f;
+    ow += 0.0f * o.x;
+    ow += 0.0f * o.y;
+    ow += 0.0f * o.z;
+    ow += 1.0f * 1.0f;
+    r.o = mk3(ox / ow, oy / ow, oz / ow);
+  }
   float norm = sqrtf(dx * dx + dy * dy + dz * dz);
   r.tmult = norm;
   r.d = mk3(dx / norm, dy / norm, dz / norm);

@@ -101,8 +101,11 @@ __device__ __forceinline__ uint32_t wave_sum(uint32_t v) {
   return v;
 }
 
+// Non-staged kernels are compiled for five waves per SIMD (at most 96 VGPRs);
+// staged kernels are bounded by their LDS image instead.
 template <int MODE, bool REC, int K, int STAGE, bool MULTI = false>
-__global__ __launch_bounds__(STAGE ? 64 : 256) void trace_kernel(TraceParams p) {
+__global__ __launch_bounds__(STAGE ? 64 : 256) __attribute__((amdgpu_waves_per_eu(STAGE || REC ? 1 : 5)))
+void trace_kernel(TraceParams p) {
   static_assert(!MULTI || (!REC && STAGE == 0), "batch kernels write hits only");
   constexpr uint32_t BS = STAGE ? 64u : 256u;
   constexpr int NS = MODE == 1 ? 2 : 1;
