@@ -111,6 +111,10 @@ void trace_kernel(TraceParams p) {
   constexpr int NS = MODE == 1 ? 2 : 1;
   constexpr uint32_t UNITS = STAGE / 64;
   constexpr uint32_t HSIZE = STAGE ? 2 * UNITS : 1;
+  // Unstaged kernels let a lane go on from a leaf to its next internal node
+  // within one iteration (staged kernels keep one entry per iteration, so
+  // every read passes the staging decision).
+  constexpr bool FUSE = STAGE == 0;
   __shared__ uint64_t lds_stack[NS * K * BS];
   __shared__ uint4 stage_img[STAGE ? STAGE / 16 : 1];
   __shared__ uint32_t hkey[HSIZE];
@@ -398,6 +402,22 @@ void trace_kernel(TraceParams p) {
   // Write the hit record of the lane's completed ray (:2990-3038).  Deferred
   // to the wave's refill so the write path runs once per batch of completed
   // rays instead of in nearly every iteration.
+  // Pop the lane's next entry into `e`; false when its stacks are empty.
+  auto pop_next = [&]() -> bool {
+    if (MODE == 0 ? st_empty(sa) : (st_empty(sa) && st_empty(sb))) return false;
+    if (MODE == 0) {
+      e = st_pop<K, BS>(sa, ldsA, gA);
+    } else if (!st_empty(sa)) {
+      e = st_pop<K, BS>(sa, ldsA, gA);
+    } else {
+      phase2 = true;  // current_treelet_root := host address of the moved node (:1748-1754)
+      e = st_pop<K, BS>(sb, ldsB, gB);
+    }
+    // every processed entry emits at least one record
+    if (n_rec > kMaxStepsPerRay) err |= kErrSteps;
+    return true;
+  };
+
   auto write_hit = [&]() {
     uint4 o0 = make_uint4(0, 0, 0, 0), o1 = make_uint4(0, 0, 0, 0), o2 = make_uint4(0, 0, 0, 0);
     if (err == 0 && min_thit < rtmax && closest_leaf != kNoSlot) {
@@ -540,19 +560,8 @@ void trace_kernel(TraceParams p) {
     // ---- pop the lane's next entry, or complete the ray ---------------------
     if (active && has_e && err != 0) has_e = false;
     if (active && !has_e) {
-      const bool done = MODE == 0 ? st_empty(sa) : (st_empty(sa) && st_empty(sb));
-      if (!done && err == 0) {
-        if (MODE == 0) {
-          e = st_pop<K, BS>(sa, ldsA, gA);
-        } else if (!st_empty(sa)) {
-          e = st_pop<K, BS>(sa, ldsA, gA);
-        } else {
-          phase2 = true;  // current_treelet_root := host address of the moved node (:1748-1754)
-          e = st_pop<K, BS>(sb, ldsB, gB);
-        }
+      if (err == 0 && pop_next()) {
         has_e = true;
-        // every processed entry emits at least one record
-        if (n_rec > kMaxStepsPerRay) err |= kErrSteps;
       } else {
         // ray complete; its hit record is written at the wave's next refill
         pending = true;
@@ -650,94 +659,11 @@ void trace_kernel(TraceParams p) {
     if (step) {
       has_e = false;
       if (MODE == 1 && !(flags & kFlTop) && e_inst(e) != cur_inst) load_instance(e_inst(e));
-      if (err != 0) {
-        // completes at the next iteration
-      } else if (!(flags & kFlLeaf)) {
-        // ---- internal node (top :2499-2599 / bottom :2686-2786) ----------
-        if (off + 64 > bytes) {
-          err |= kErrRange;
-        } else {
-          const Words16 nd = read_node(off, pos);
-          emit(off, 64, RTX_TX_BVH_INTERNAL_NODE, level);
-          n_nodes++;
-          const bool top = (flags & kFlTop) != 0;
-          const int ex = (int)(int8_t)byte_of(nd, 18);
-          const int ey = (int)(int8_t)byte_of(nd, 19);
-          const int ez = (int)(int8_t)byte_of(nd, 20);
-          F3 o, idir;
-          float tmin, tmax, cull;
-          if (top) {
-            o = ro;
-            idir = ridir;
-            tmin = rtmin;
-            tmax = rtmax;
-            cull = min_thit;
-          } else {
-            o = cur.o;
-            idir = cur.idir;
-            tmin = cur.tmin;
-            tmax = cur.tmax;
-            cull = min_thit * cur.tmult;
-          }
-          const uint64_t first_child = off + (uint64_t)((int64_t)(int32_t)nd.w[3] * 64);
-          uint32_t hitmask = 0;
-          // v_min/v_max slab tests when no lane can produce a NaN (finite
-          // ray, non-NaN node origin); compare-selects otherwise
-          const bool fast = (top ? ray_ok : cur_ok) && !__builtin_isnan(f_of(nd, 0) + f_of(nd, 1) + f_of(nd, 2));
-          auto test_children = [&](auto fast_tag) {
-            constexpr bool FAST = decltype(fast_tag)::value;
-#pragma unroll
-            for (int i = 0; i < 6; ++i) {
-              const uint32_t info = byte_of(nd, 22 + i) & 0x3fu;
-              const uint32_t size = info & 3u;
-              if (size > 0) {
-                F3 lo, hi;
-                child_box(nd, i, ex, ey, ez, lo, hi);
-                float th;
-                bool h = FAST ? ray_box_fast(lo, hi, idir, o, tmin, tmax, th) : ray_box(lo, hi, idir, o, tmin, tmax, th);
-                if (h && th >= cull) h = false;
-                if (h) hitmask |= 1u << i;
-              }
-            }
-          };
-          if (__ballot(!fast) == 0)
-            test_children(std::true_type{});
-          else
-            test_children(std::false_type{});
-          if (hitmask) {
-            const uint32_t lvl = level + 1;
-            if (lvl > max_level) max_level = lvl;
-            const uint32_t inst = top ? 0u : e_inst(e);
-            const uint32_t base_flags = top ? kFlTop : 0u;
-            int deferred = -1;
-            uint32_t deferred_cs = 0;
-            uint64_t child = first_child;
-#pragma unroll
-            for (int i = 0; i < 6; ++i) {
-              const uint32_t info = byte_of(nd, 22 + i) & 0x3fu;
-              const uint64_t c = child;
-              child += (uint64_t)(info & 3u) * 64;
-              if (!(hitmask & (1u << i))) continue;
-              const uint32_t ctype = info >> 2;
-              const bool leaf = ctype != 0;
-              if (top && leaf && ctype != 1u) err |= kErrType;
-              if ((c & 63) || c + 64 > bytes) {
-                err |= kErrRange;
-                continue;
-              }
-              const uint32_t cs = (uint32_t)(c >> 6);
-              if (MODE == 0 && !leaf && deferred < 0) {
-                deferred = i;
-                deferred_cs = cs;
-                continue;
-              }
-              push_entry(make_entry(cs, base_flags | (leaf ? kFlLeaf : 0u), lvl, inst), goes_current(cs));
-            }
-            if (MODE == 0 && deferred >= 0) {
-              push_entry(make_entry(deferred_cs, base_flags, lvl, inst), true);
-            }
-          }
-        }
+      // lanes on internal nodes (their own entry, or one reached through a
+      // leaf in this iteration) share one internal-node pass at the end
+      bool inner = err == 0 && !(flags & kFlLeaf);
+      if (err != 0 || !(flags & kFlLeaf)) {
+        // internal node: below; an error completes the ray next iteration
       } else if (flags & kFlTop) {
         // ---- instance leaf (:2602-2668 / :1872-1955) ----------------------
         if (off + 128 > bytes) {
@@ -772,7 +698,16 @@ void trace_kernel(TraceParams p) {
             } else {
               const uint32_t rs = (uint32_t)(root >> 6);
               if (level > max_level) max_level = level;
-              push_entry(make_entry(rs, 0u, level, slot), goes_current(rs));
+              if (MODE == 0 && FUSE) {
+                // plain order pops the BLAS root next: test it in this
+                // iteration's internal-node pass instead
+                if (err == 0) {
+                  e = make_entry(rs, 0u, level, slot);
+                  inner = true;
+                }
+              } else {
+                push_entry(make_entry(rs, 0u, level, slot), goes_current(rs));
+              }
             }
           }
         }
@@ -817,6 +752,110 @@ void trace_kernel(TraceParams p) {
           } else {
             emit(off, 64, RTX_TX_BVH_PROCEDURAL_LEAF, level);
             n_nodes++;
+          }
+        }
+        if constexpr (FUSE) {
+          // the lane pops its next entry now; an internal node is tested in
+          // this iteration's internal-node pass
+          if (err == 0 && pop_next()) {
+            if (err == 0 && !(e_flags(e) & kFlLeaf))
+              inner = true;
+            else
+              has_e = true;
+          }
+        }
+      }
+      if (inner) {
+        const uint32_t slot = e_slot(e);
+        const uint32_t flags = e_flags(e);
+        const uint32_t level = e_level(e);
+        const uint64_t off = (uint64_t)slot << 6;
+        if (MODE == 1 && !(flags & kFlTop) && e_inst(e) != cur_inst) load_instance(e_inst(e));
+        if (err == 0) {
+          // ---- internal node (top :2499-2599 / bottom :2686-2786) ----------
+          if (off + 64 > bytes) {
+            err |= kErrRange;
+          } else {
+            const Words16 nd = read_node(off, pos);
+            emit(off, 64, RTX_TX_BVH_INTERNAL_NODE, level);
+            n_nodes++;
+            const bool top = (flags & kFlTop) != 0;
+            const int ex = (int)(int8_t)byte_of(nd, 18);
+            const int ey = (int)(int8_t)byte_of(nd, 19);
+            const int ez = (int)(int8_t)byte_of(nd, 20);
+            F3 o, idir;
+            float tmin, tmax, cull;
+            if (top) {
+              o = ro;
+              idir = ridir;
+              tmin = rtmin;
+              tmax = rtmax;
+              cull = min_thit;
+            } else {
+              o = cur.o;
+              idir = cur.idir;
+              tmin = cur.tmin;
+              tmax = cur.tmax;
+              cull = min_thit * cur.tmult;
+            }
+            const uint64_t first_child = off + (uint64_t)((int64_t)(int32_t)nd.w[3] * 64);
+            uint32_t hitmask = 0;
+            // v_min/v_max slab tests when no lane can produce a NaN (finite
+            // ray, non-NaN node origin); compare-selects otherwise
+            const bool fast = (top ? ray_ok : cur_ok) && !__builtin_isnan(f_of(nd, 0) + f_of(nd, 1) + f_of(nd, 2));
+            auto test_children = [&](auto fast_tag) {
+              constexpr bool FAST = decltype(fast_tag)::value;
+#pragma unroll
+              for (int i = 0; i < 6; ++i) {
+                const uint32_t info = byte_of(nd, 22 + i) & 0x3fu;
+                const uint32_t size = info & 3u;
+                if (size > 0) {
+                  F3 lo, hi;
+                  child_box(nd, i, ex, ey, ez, lo, hi);
+                  float th;
+                  bool h = FAST ? ray_box_fast(lo, hi, idir, o, tmin, tmax, th) : ray_box(lo, hi, idir, o, tmin, tmax, th);
+                  if (h && th >= cull) h = false;
+                  if (h) hitmask |= 1u << i;
+                }
+              }
+            };
+            if (__ballot(!fast) == 0)
+              test_children(std::true_type{});
+            else
+              test_children(std::false_type{});
+            if (hitmask) {
+              const uint32_t lvl = level + 1;
+              if (lvl > max_level) max_level = lvl;
+              const uint32_t inst = top ? 0u : e_inst(e);
+              const uint32_t base_flags = top ? kFlTop : 0u;
+              int deferred = -1;
+              uint32_t deferred_cs = 0;
+              uint64_t child = first_child;
+#pragma unroll
+              for (int i = 0; i < 6; ++i) {
+                const uint32_t info = byte_of(nd, 22 + i) & 0x3fu;
+                const uint64_t c = child;
+                child += (uint64_t)(info & 3u) * 64;
+                if (!(hitmask & (1u << i))) continue;
+                const uint32_t ctype = info >> 2;
+                const bool leaf = ctype != 0;
+                if (top && leaf && ctype != 1u) err |= kErrType;
+                if ((c & 63) || c + 64 > bytes) {
+                  err |= kErrRange;
+                  continue;
+                }
+                const uint32_t cs = (uint32_t)(c >> 6);
+                if (MODE == 0 && !leaf && deferred < 0) {
+                  deferred = i;
+                  deferred_cs = cs;
+                  continue;
+                }
+                push_entry(make_entry(cs, base_flags | (leaf ? kFlLeaf : 0u), lvl, inst), goes_current(cs));
+              }
+              if (MODE == 0 && deferred >= 0) {
+                push_entry(make_entry(deferred_cs, base_flags, lvl, inst), true);
+              }
+            }
           }
         }
       }
