@@ -190,14 +190,22 @@ __device__ __forceinline__ uint32_t byte_of(const Words16& n, int b) { return (n
 __device__ __forceinline__ float f_of(const Words16& n, int word) { return __uint_as_float(n.w[word]); }
 
 // set_child_bounds (util.h:499-508) on an internal node held in registers.
-__device__ __forceinline__ void child_box(const Words16& n, int i, int ex, int ey, int ez, F3& lo, F3& hi) {
+// Per-axis scales 2^(e-8) of a node's quantized child bounds.
+__device__ __forceinline__ F3 child_scales(int ex, int ey, int ez) {
+  return mk3(ldexpf(1.0f, ex - 8), ldexpf(1.0f, ey - 8), ldexpf(1.0f, ez - 8));
+}
+
+// origin + ldexpf(q, e - 8) per bound.  q * 2^(e-8) is exact for an 8-bit q
+// and e in [-128, 127] (at most 255 * 2^119, at least one unit of 2^-136), so
+// the single rounding of fma(q, 2^(e-8), origin) is the rounding of the sum.
+__device__ __forceinline__ void child_box(const Words16& n, int i, F3 sc, F3& lo, F3& hi) {
   const float ox = f_of(n, 0), oy = f_of(n, 1), oz = f_of(n, 2);
-  lo.x = ox + ldexpf((float)byte_of(n, 28 + i), ex - 8);
-  lo.y = oy + ldexpf((float)byte_of(n, 40 + i), ey - 8);
-  lo.z = oz + ldexpf((float)byte_of(n, 52 + i), ez - 8);
-  hi.x = ox + ldexpf((float)byte_of(n, 34 + i), ex - 8);
-  hi.y = oy + ldexpf((float)byte_of(n, 46 + i), ey - 8);
-  hi.z = oz + ldexpf((float)byte_of(n, 58 + i), ez - 8);
+  lo.x = __builtin_fmaf((float)byte_of(n, 28 + i), sc.x, ox);
+  lo.y = __builtin_fmaf((float)byte_of(n, 40 + i), sc.y, oy);
+  lo.z = __builtin_fmaf((float)byte_of(n, 52 + i), sc.z, oz);
+  hi.x = __builtin_fmaf((float)byte_of(n, 34 + i), sc.x, ox);
+  hi.y = __builtin_fmaf((float)byte_of(n, 46 + i), sc.y, oy);
+  hi.z = __builtin_fmaf((float)byte_of(n, 58 + i), sc.z, oz);
 }
 
 }  // namespace rtxd

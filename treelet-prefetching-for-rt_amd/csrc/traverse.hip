@@ -799,6 +799,7 @@ void trace_kernel(TraceParams p) {
               cull = min_thit * cur.tmult;
             }
             const uint64_t first_child = off + (uint64_t)((int64_t)(int32_t)nd.w[3] * 64);
+            const F3 sc = child_scales(ex, ey, ez);
             uint32_t hitmask = 0;
             // v_min/v_max slab tests when no lane can produce a NaN (finite
             // ray, non-NaN node origin); compare-selects otherwise
@@ -811,7 +812,7 @@ void trace_kernel(TraceParams p) {
                 const uint32_t size = info & 3u;
                 if (size > 0) {
                   F3 lo, hi;
-                  child_box(nd, i, ex, ey, ez, lo, hi);
+                  child_box(nd, i, sc, lo, hi);
                   float th;
                   bool h = FAST ? ray_box_fast(lo, hi, idir, o, tmin, tmax, th) : ray_box(lo, hi, idir, o, tmin, tmax, th);
                   if (h && th >= cull) h = false;
