@@ -181,9 +181,12 @@ __global__ __launch_bounds__(256) void softmax_ce(float* L, int n, int C, const 
   mx = block_reduce(mx, sm, true);
   float se = 0.f;
   for (int c = threadIdx.x; c < C; c += blockDim.x) se += expf(row[c] - mx);
-  se = block_reduce(se, sm, false);
+  // the target logit is read before the barrier inside the next reduction:
+  // after it, the thread owning column t may overwrite it with its gradient
   const uint32_t t = s[r].target;
-  if (threadIdx.x == 0 && t < (uint32_t)C) atomicAdd(loss, (logf(se) - (row[t] - mx)) * inv_n);
+  const float lt = (threadIdx.x == 0 && t < (uint32_t)C) ? row[t] : 0.f;
+  se = block_reduce(se, sm, false);
+  if (threadIdx.x == 0 && t < (uint32_t)C) atomicAdd(loss, (logf(se) - (lt - mx)) * inv_n);
   if (want_grad)
     for (int c = threadIdx.x; c < C; c += blockDim.x)
       row[c] = (expf(row[c] - mx) / se - ((uint32_t)c == t ? 1.f : 0.f)) * inv_n;
