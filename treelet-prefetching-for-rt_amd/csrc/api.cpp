@@ -433,9 +433,10 @@ rtx_status rtx_trace(rtx_ctx* c, const rtx_ray* rays, uint32_t n, const rtx_trac
     HIP_TRY(hipStreamSynchronize(c->stream));
     return RTX_OK;
   }
-  // 8 LDS entries per lane: 16 KiB per 256-lane block for plain order, so the
-  // register budget, not LDS, bounds how many waves share a CU
-  uint32_t lds_stack = opts->lds_stack ? opts->lds_stack : 8;
+  // 16 KiB of LDS stack per 256-lane block (8 entries per lane in plain order,
+  // 4 per list in treelet order), so the register budget, not LDS, bounds how
+  // many waves share a CU
+  uint32_t lds_stack = opts->lds_stack ? opts->lds_stack : (opts->mode == RTX_MODE_PLAIN ? 8 : 4);
   if (lds_stack != 4 && lds_stack != 8 && lds_stack != 16) return RTX_ERR_INVALID_ARGUMENT;
   // treelet staging: one wave per workgroup with stage_bytes of LDS for the image
   uint32_t stage = 0;

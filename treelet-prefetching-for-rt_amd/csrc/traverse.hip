@@ -698,15 +698,18 @@ void trace_kernel(TraceParams p) {
             } else {
               const uint32_t rs = (uint32_t)(root >> 6);
               if (level > max_level) max_level = level;
-              if (MODE == 0 && FUSE) {
-                // plain order pops the BLAS root next: test it in this
-                // iteration's internal-node pass instead
-                if (err == 0) {
-                  e = make_entry(rs, 0u, level, slot);
-                  inner = true;
-                }
+              const bool to_a = goes_current(rs);
+              // The BLAS root is popped next when it goes on top of the list
+              // the next pop takes: always in plain order; in treelet order
+              // when it joins the current list, or the other list while the
+              // current one is empty (that pop starts phase 2).  Such a root is
+              // tested in this iteration's internal-node pass instead.
+              if (FUSE && err == 0 && (MODE == 0 || to_a || st_empty(sa))) {
+                if (MODE == 1 && !to_a) phase2 = true;  // current_treelet_root := host address (:1748-1754)
+                e = make_entry(rs, 0u, level, slot);
+                inner = true;
               } else {
-                push_entry(make_entry(rs, 0u, level, slot), goes_current(rs));
+                push_entry(make_entry(rs, 0u, level, slot), to_a);
               }
             }
           }
