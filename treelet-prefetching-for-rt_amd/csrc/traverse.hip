@@ -834,6 +834,23 @@ void trace_kernel(TraceParams p) {
               const uint32_t base_flags = top ? kFlTop : 0u;
               int deferred = -1;
               uint32_t deferred_cs = 0;
+              // Treelet order before the first switch: which hit children the
+              // first treelet owns.  The six owner loads are issued together
+              // (a child that is not pushed reads this node's own entry).
+              uint32_t cur_bits = 0;
+              if (MODE == 1 && !phase2) {
+                uint32_t ow[6];
+                uint64_t c = first_child;
+#pragma unroll
+                for (int i = 0; i < 6; ++i) {
+                  const bool take = (hitmask >> i) & 1u && !(c & 63) && c + 64 <= bytes;
+                  ow[i] = owner[take ? (uint32_t)(c >> 6) : slot];
+                  c += (uint64_t)(byte_of(nd, 22 + i) & 3u) * 64;
+                }
+#pragma unroll
+                for (int i = 0; i < 6; ++i)
+                  if (ow[i] == first_treelet) cur_bits |= 1u << i;
+              }
               uint64_t child = first_child;
 #pragma unroll
               for (int i = 0; i < 6; ++i) {
@@ -854,7 +871,7 @@ void trace_kernel(TraceParams p) {
                   deferred_cs = cs;
                   continue;
                 }
-                push_entry(make_entry(cs, base_flags | (leaf ? kFlLeaf : 0u), lvl, inst), goes_current(cs));
+                push_entry(make_entry(cs, base_flags | (leaf ? kFlLeaf : 0u), lvl, inst), MODE == 0 || ((cur_bits >> i) & 1u));
               }
               if (MODE == 0 && deferred >= 0) {
                 push_entry(make_entry(deferred_cs, base_flags, lvl, inst), true);
