@@ -22,6 +22,25 @@ def test_plain_c1_hits_and_records(gpu, oracle, c1, lds_stack):
     assert h["hit"].sum() > 0.5 * len(rays)
 
 
+@pytest.mark.parametrize("mode", [0, 1])
+@pytest.mark.parametrize("shape", [dict(waves_per_cu=4), dict(waves_per_cu=64), dict(refill=1), dict(refill=64),
+                                   dict(waves_per_cu=4, refill=1, lds_stack=4)])
+def test_launch_shapes_bit_exact(gpu, oracle, c1, mode, shape):
+    """Persistent-wave corner cases: a few waves, more workgroups than fit on
+    the GPU at once (later ones start after earlier ones exit), a refill on
+    every idle lane, and a refill only when the whole wave is idle."""
+    scene, rays = c1
+    _scene_on(gpu, scene)
+    part = None
+    if mode == 1:
+        gpu.build_treelets(8192)
+        part = oracle.Partition(scene.arena, scene.tlas_offset, 8192)
+    h_ref, o_ref, r_ref = oracle.trace(scene.arena, scene.tlas_offset, rays, mode=mode, partition=part)
+    h, o, r = gpu.trace(rays, mode=mode, records=True, **shape)
+    assert_hits_equal(h, h_ref, f"mode {mode} {shape}")
+    assert_records_equal(o, r, o_ref, r_ref, f"mode {mode} {shape}")
+
+
 @pytest.mark.parametrize("budget", [512, 8192, 65536])
 def test_partition_c1(gpu, oracle, c1, budget):
     scene, _ = c1
