@@ -111,10 +111,6 @@ void trace_kernel(TraceParams p) {
   constexpr int NS = MODE == 1 ? 2 : 1;
   constexpr uint32_t UNITS = STAGE / 64;
   constexpr uint32_t HSIZE = STAGE ? 2 * UNITS : 1;
-  // Unstaged kernels let a lane go on from a leaf to its next internal node
-  // within one iteration (staged kernels keep one entry per iteration, so
-  // every read passes the staging decision).
-  constexpr bool FUSE = STAGE == 0;
   __shared__ uint64_t lds_stack[NS * K * BS];
   __shared__ uint4 stage_img[STAGE ? STAGE / 16 : 1];
   __shared__ uint32_t hkey[HSIZE];
@@ -704,7 +700,7 @@ void trace_kernel(TraceParams p) {
               // when it joins the current list, or the other list while the
               // current one is empty (that pop starts phase 2).  Such a root is
               // tested in this iteration's internal-node pass instead.
-              if (FUSE && err == 0 && (MODE == 0 || to_a || st_empty(sa))) {
+              if (err == 0 && (MODE == 0 || to_a || st_empty(sa))) {
                 if (MODE == 1 && !to_a) phase2 = true;  // current_treelet_root := host address (:1748-1754)
                 e = make_entry(rs, 0u, level, slot);
                 inner = true;
@@ -757,15 +753,14 @@ void trace_kernel(TraceParams p) {
             n_nodes++;
           }
         }
-        if constexpr (FUSE) {
-          // the lane pops its next entry now; an internal node is tested in
-          // this iteration's internal-node pass
-          if (err == 0 && pop_next()) {
-            if (err == 0 && !(e_flags(e) & kFlLeaf))
-              inner = true;
-            else
-              has_e = true;
-          }
+        // the lane pops its next entry now; an internal node is tested in this
+        // iteration's internal-node pass (in staged kernels its read still
+        // probes the staged treelets; it only misses the iteration's vote)
+        if (err == 0 && pop_next()) {
+          if (err == 0 && !(e_flags(e) & kFlLeaf))
+            inner = true;
+          else
+            has_e = true;
         }
       }
       if (inner) {
@@ -779,7 +774,7 @@ void trace_kernel(TraceParams p) {
           if (off + 64 > bytes) {
             err |= kErrRange;
           } else {
-            const Words16 nd = read_node(off, pos);
+            const Words16 nd = read_node(off, STAGE > 0 ? staged_pos(slot) : pos);
             emit(off, 64, RTX_TX_BVH_INTERNAL_NODE, level);
             n_nodes++;
             const bool top = (flags & kFlTop) != 0;
