@@ -21,6 +21,7 @@ using namespace rtxk;
 
 // bytes of the control block cleared before and read after a traversal
 static constexpr size_t kCtlBytes = 16 * sizeof(uint32_t) + 16 * sizeof(unsigned long long);
+static constexpr uint32_t kSlots = 64;
 
 struct rtx_ctx {
   int device = 0;
@@ -47,6 +48,13 @@ struct rtx_ctx {
   uint32_t* counters = nullptr;
   uint32_t* ctl_host = nullptr;
   unsigned long long* stats = nullptr;
+  // rtx_trace's rotating control slots (launch k uses slot k % kSlots and
+  // zeroes the next one) and the mapped host word its kernels set on an error
+  uint32_t* slots = nullptr;
+  uint32_t slot = 0;
+  bool slots_clean = true;
+  uint32_t* err_host = nullptr;
+  uint32_t* err_host_dev = nullptr;
   uint64_t* scan_tmp = nullptr;
   size_t scan_tmp_entries = 0;
   rtx_prefetch_stats last_stats{};
@@ -219,6 +227,12 @@ rtx_status rtx_create(int device, rtx_ctx** out) {
   if (hipHostMalloc(&c->ctl_host, 64 * sizeof(uint32_t), hipHostMallocDefault) != hipSuccess)
     return fail(RTX_ERR_OUT_OF_MEMORY);
   if (hipMemset(c->counters, 0, 64 * sizeof(uint32_t)) != hipSuccess) return fail(RTX_ERR_HIP);
+  if (hipMalloc(&c->slots, kSlots * kSlotWords * sizeof(uint32_t)) != hipSuccess) return fail(RTX_ERR_OUT_OF_MEMORY);
+  if (hipMemset(c->slots, 0, kSlots * kSlotWords * sizeof(uint32_t)) != hipSuccess) return fail(RTX_ERR_HIP);
+  if (hipHostMalloc(&c->err_host, 64, hipHostMallocMapped | hipHostMallocCoherent) != hipSuccess)
+    return fail(RTX_ERR_OUT_OF_MEMORY);
+  if (hipHostGetDevicePointer(reinterpret_cast<void**>(&c->err_host_dev), c->err_host, 0) != hipSuccess)
+    return fail(RTX_ERR_HIP);
   *out = c;
   return RTX_OK;
 }
@@ -233,6 +247,8 @@ rtx_status rtx_destroy(rtx_ctx* c) {
   if (c->spill) hipFree(c->spill);
   if (c->counters) hipFree(c->counters);
   if (c->ctl_host) hipHostFree(c->ctl_host);
+  if (c->slots) hipFree(c->slots);
+  if (c->err_host) hipHostFree(c->err_host);
   if (c->scan_tmp) hipFree(c->scan_tmp);
   for (auto& sc : c->scenes) {
     if (sc.arena) hipFree(sc.arena);
@@ -483,9 +499,6 @@ rtx_status rtx_trace(rtx_ctx* c, const rtx_ray* rays, uint32_t n, const rtx_trac
   p.first_treelet = c->part.valid ? c->part.first_treelet : 0;
   p.spill = c->spill;
   p.spill_cap = spill_cap;
-  p.ray_counter = c->counters;
-  p.error = c->counters + 1;
-  p.stats = c->stats;
   p.prefetch = opts->prefetch;
   p.threshold = opts->threshold;
   if (c->part.valid) {
@@ -505,8 +518,32 @@ rtx_status rtx_trace(rtx_ctx* c, const rtx_ray* rays, uint32_t n, const rtx_trac
     p.pred_treelets = c->pred_treelets;
   }
 
-  HIP_TRY(hipMemsetAsync(c->counters, 0, kCtlBytes, c->stream));
+  // Rotating control slots: each launch uses one slot and its first thread
+  // zeroes the next, so no memset runs between launches; any ray error also
+  // sets a mapped host word, so an error-free launch needs no read-back.
+  volatile uint32_t* herr = c->err_host;
+  auto use_slot = [&]() -> rtx_status {
+    if (!c->slots_clean) HIP_TRY(hipMemsetAsync(c->slots, 0, kSlots * kSlotWords * sizeof(uint32_t), c->stream));
+    c->slots_clean = false;  // until the launch is known to have run
+    uint32_t* cur = c->slots + (size_t)c->slot * kSlotWords;
+    c->slot = (c->slot + 1) % kSlots;
+    p.ray_counter = cur;
+    p.error = cur + 1;
+    p.stats = reinterpret_cast<unsigned long long*>(cur + 16);
+    p.next_slot = c->slots + (size_t)c->slot * kSlotWords;
+    p.host_err = c->err_host_dev;
+    *herr = 0;
+    return RTX_OK;
+  };
+  auto launch_error = [&]() -> rtx_status {  // after a synchronize
+    if (*herr == 0) return RTX_OK;
+    uint32_t err = 0;
+    HIP_TRY(hipMemcpy(&err, p.error, sizeof(uint32_t), hipMemcpyDeviceToHost));
+    return kernel_error_status(err);
+  };
   c->kernel_timed = false;
+  st = use_slot();
+  if (st != RTX_OK) return st;
   HIP_TRY(hipEventRecord(c->evk0, c->stream));
   HIP_TRY(launch_trace(p, opts->mode, false, lds_stack, stage, grid, c->stream));
   HIP_TRY(hipEventRecord(c->evk1, c->stream));
@@ -517,22 +554,23 @@ rtx_status rtx_trace(rtx_ctx* c, const rtx_ray* rays, uint32_t n, const rtx_trac
     if (st != RTX_OK) return st;
     HIP_TRY(launch_scan_records(hits, n, offsets, c->scan_tmp, c->stream));
     uint64_t total = 0;
-    uint32_t err = 0;
     HIP_TRY(hipMemcpyAsync(&total, offsets + n, sizeof(uint64_t), hipMemcpyDeviceToHost, c->stream));
-    HIP_TRY(hipMemcpyAsync(&err, c->counters + 1, sizeof(uint32_t), hipMemcpyDeviceToHost, c->stream));
     HIP_TRY(hipStreamSynchronize(c->stream));
-    if (err) return kernel_error_status(err);
+    c->slots_clean = true;
+    st = launch_error();
+    if (st != RTX_OK) return st;
     if (total > capacity || (total > 0 && !records)) return RTX_ERR_TRACE_OVERFLOW;
     p.records = records;
     p.record_capacity = capacity;
     p.record_offsets = offsets;
-    HIP_TRY(hipMemsetAsync(c->counters, 0, kCtlBytes, c->stream));
+    st = use_slot();
+    if (st != RTX_OK) return st;
     HIP_TRY(launch_trace(p, opts->mode, true, lds_stack, stage, grid, c->stream));
   }
-  HIP_TRY(hipMemcpyAsync(c->ctl_host, c->counters, kCtlBytes, hipMemcpyDeviceToHost, c->stream));
+  unsigned long long stats[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  if (stage) HIP_TRY(hipMemcpyAsync(stats, p.stats, sizeof(stats), hipMemcpyDeviceToHost, c->stream));
   HIP_TRY(hipStreamSynchronize(c->stream));
-  const uint32_t err = c->ctl_host[1];
-  const unsigned long long* stats = reinterpret_cast<const unsigned long long*>(c->ctl_host + 16);
+  c->slots_clean = true;
   c->last_stats.node_reads = stats[0];
   c->last_stats.lds_hits = stats[1];
   c->last_stats.staged_treelets = stats[2];
@@ -541,7 +579,7 @@ rtx_status rtx_trace(rtx_ctx* c, const rtx_ray* rays, uint32_t n, const rtx_trac
   c->last_stats.predictions = stats[5];
   c->last_stats.prediction_hits = stats[6];
   c->last_stats.fallbacks = stats[7];
-  return kernel_error_status(err);
+  return launch_error();
 }
 
 rtx_status rtx_set_prefetch_table(rtx_ctx* c, const uint32_t* table, uint64_t n_rows) {

@@ -20,6 +20,7 @@ enum : uint32_t {
 };
 
 constexpr uint32_t kBlock = 256;        // threads per block (4 waves of 64)
+constexpr uint32_t kSlotWords = 32;     // control slot: [0] ray counter, [1] error, stats from word 16
 constexpr uint32_t kMaxStepsPerRay = 1u << 24;
 constexpr uint32_t kMaxBatchJobs = 1024;
 
@@ -55,6 +56,11 @@ struct TraceParams {
   uint32_t* ray_counter;
   uint32_t* error;
   unsigned long long* stats;  // rtx_prefetch_stats counters
+  // single-scene launches: the control slot the next launch uses (zeroed by
+  // this launch's first thread, so no memset runs between launches) and a
+  // mapped host word set to 1 when any ray reports an error
+  uint32_t* next_slot;
+  uint32_t* host_err;
   // treelet staging
   uint32_t prefetch;                 // RTX_PREFETCH_*
   float threshold;

@@ -143,6 +143,8 @@ void trace_kernel(TraceParams p) {
     __syncthreads();
   }
 
+  if (p.next_slot && blockIdx.x == 0 && threadIdx.x < kSlotWords) p.next_slot[threadIdx.x] = 0u;
+
   Stack sa{0, 0, 0}, sb{0, 0, 0};
   bool active = false, exhausted = false;
   bool pending = false;  // the lane's completed ray still has to write its hit record
@@ -445,7 +447,10 @@ void trace_kernel(TraceParams p) {
     hq[1] = o1;
     hq[2] = o2;
     hq[3] = make_uint4(n_nodes, fetch_bytes, n_any, max_level);
-    if (err) atomicOr(p.error, err);
+    if (err) {
+      atomicOr(p.error, err);
+      if (p.host_err) *reinterpret_cast<volatile uint32_t*>(p.host_err) = 1u;
+    }
   };
 
   // the TLAS header of a single-scene launch, read once
