@@ -11,6 +11,7 @@
 #include <cstdio>
 #include <cstring>
 #include <new>
+#include <unordered_map>
 #include <vector>
 
 #include "kernels.h"
@@ -55,6 +56,8 @@ struct rtx_ctx {
   bool slots_clean = true;
   uint32_t* err_host = nullptr;
   uint32_t* err_host_dev = nullptr;
+  // workgroups per CU of each trace kernel, from the occupancy API on first use
+  std::unordered_map<uint64_t, uint32_t> occupancy;
   uint64_t* scan_tmp = nullptr;
   size_t scan_tmp_entries = 0;
   rtx_prefetch_stats last_stats{};
@@ -128,6 +131,15 @@ rtx_status ensure_spill(rtx_ctx* c, size_t entries) {
   HIP_TRY(hipMalloc(&c->spill, entries * sizeof(uint64_t)));
   c->spill_entries = entries;
   return RTX_OK;
+}
+
+uint32_t cached_blocks_per_cu(rtx_ctx* c, uint32_t mode, uint32_t lds_stack, uint32_t stage) {
+  const uint64_t key = ((uint64_t)mode << 48) | ((uint64_t)lds_stack << 32) | stage;
+  auto it = c->occupancy.find(key);
+  if (it != c->occupancy.end()) return it->second;
+  const uint32_t b = trace_blocks_per_cu(mode, false, lds_stack, stage);
+  c->occupancy[key] = b;
+  return b;
 }
 
 rtx_status ensure_scan_tmp(rtx_ctx* c, size_t entries) {
@@ -473,7 +485,7 @@ rtx_status rtx_trace(rtx_ctx* c, const rtx_ray* rays, uint32_t n, const rtx_trac
   const uint32_t bs = trace_block_size(stage);
   // persistent blocks: by default as many as fit on a CU at once
   uint32_t blocks_per_cu = opts->waves_per_cu ? std::max(1u, opts->waves_per_cu * 64 / bs)
-                                              : trace_blocks_per_cu(opts->mode, false, lds_stack, stage);
+                                              : cached_blocks_per_cu(c, opts->mode, lds_stack, stage);
   if (blocks_per_cu == 0) blocks_per_cu = 16 * 64 / bs;
   uint32_t grid = (uint32_t)c->n_cus * blocks_per_cu;
   grid = std::min<uint32_t>(grid, (n + bs - 1) / bs);
