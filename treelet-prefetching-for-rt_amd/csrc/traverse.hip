@@ -23,6 +23,14 @@
 // Stacks: a ring of K entries per lane in LDS, spilling the oldest entries to
 // a per-lane region in global memory (exact LIFO order).
 //
+// One wave iteration runs three passes, each only for the lanes that need it:
+// instance leaves, triangle leaves, then internal nodes.  A lane that finishes
+// a leaf pops its next entry at once and joins the internal-node pass if that
+// entry is an internal node; an instance leaf hands over its BLAS root the
+// same way when that root is the next entry to be popped.  Completed rays
+// keep their state until the wave's next refill, which writes their hit
+// records together.
+//
 // Treelet staging (STAGE > 0, one wave per workgroup): the wave keeps up to
 // stage/slot treelet images in LDS slots, copied from the treelet-contiguous
 // image buffer with coalesced 16-byte loads, and an LDS hash table from node
@@ -397,9 +405,6 @@ void trace_kernel(TraceParams p) {
     return false;
   };
 
-  // Write the hit record of the lane's completed ray (:2990-3038).  Deferred
-  // to the wave's refill so the write path runs once per batch of completed
-  // rays instead of in nearly every iteration.
   // Pop the lane's next entry into `e`; false when its stacks are empty.
   auto pop_next = [&]() -> bool {
     if (MODE == 0 ? st_empty(sa) : (st_empty(sa) && st_empty(sb))) return false;
@@ -416,6 +421,9 @@ void trace_kernel(TraceParams p) {
     return true;
   };
 
+  // Write the hit record of the lane's completed ray (:2990-3038).  Deferred
+  // to the wave's refill so the write path runs once per batch of completed
+  // rays instead of in nearly every iteration.
   auto write_hit = [&]() {
     uint4 o0 = make_uint4(0, 0, 0, 0), o1 = make_uint4(0, 0, 0, 0), o2 = make_uint4(0, 0, 0, 0);
     if (err == 0 && min_thit < rtmax && closest_leaf != kNoSlot) {
