@@ -49,8 +49,9 @@ struct rtx_ctx {
   uint32_t* counters = nullptr;
   uint32_t* ctl_host = nullptr;
   unsigned long long* stats = nullptr;
-  // rtx_trace's rotating control slots (launch k uses slot k % kSlots and
-  // zeroes the next one) and the mapped host word its kernels set on an error
+  // rtx_trace's rotating control slots (launch k uses slot k % kSlots; all are
+  // cleared together when the index wraps) and the mapped host word its
+  // kernels set on an error
   uint32_t* slots = nullptr;
   uint32_t slot = 0;
   bool slots_clean = true;
@@ -530,19 +531,21 @@ rtx_status rtx_trace(rtx_ctx* c, const rtx_ray* rays, uint32_t n, const rtx_trac
     p.pred_treelets = c->pred_treelets;
   }
 
-  // Rotating control slots: each launch uses one slot and its first thread
-  // zeroes the next, so no memset runs between launches; any ray error also
-  // sets a mapped host word, so an error-free launch needs no read-back.
+  // Rotating control slots: each launch uses a fresh slot, and one memset
+  // clears all of them every kSlots launches; any ray error also sets a mapped
+  // host word, so an error-free launch needs no read-back.
   volatile uint32_t* herr = c->err_host;
   auto use_slot = [&]() -> rtx_status {
-    if (!c->slots_clean) HIP_TRY(hipMemsetAsync(c->slots, 0, kSlots * kSlotWords * sizeof(uint32_t), c->stream));
+    if (!c->slots_clean || c->slot == 0) {
+      HIP_TRY(hipMemsetAsync(c->slots, 0, kSlots * kSlotWords * sizeof(uint32_t), c->stream));
+      c->slot = 0;
+    }
     c->slots_clean = false;  // until the launch is known to have run
     uint32_t* cur = c->slots + (size_t)c->slot * kSlotWords;
     c->slot = (c->slot + 1) % kSlots;
     p.ray_counter = cur;
     p.error = cur + 1;
     p.stats = reinterpret_cast<unsigned long long*>(cur + 16);
-    p.next_slot = c->slots + (size_t)c->slot * kSlotWords;
     p.host_err = c->err_host_dev;
     *herr = 0;
     return RTX_OK;

@@ -56,10 +56,8 @@ struct TraceParams {
   uint32_t* ray_counter;
   uint32_t* error;
   unsigned long long* stats;  // rtx_prefetch_stats counters
-  // single-scene launches: the control slot the next launch uses (zeroed by
-  // this launch's first thread, so no memset runs between launches) and a
-  // mapped host word set to 1 when any ray reports an error
-  uint32_t* next_slot;
+  // single-scene launches: a mapped host word set to 1 when any ray reports
+  // an error
   uint32_t* host_err;
   // treelet staging
   uint32_t prefetch;                 // RTX_PREFETCH_*

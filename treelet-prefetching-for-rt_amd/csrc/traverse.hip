@@ -151,8 +151,6 @@ void trace_kernel(TraceParams p) {
     __syncthreads();
   }
 
-  if (p.next_slot && blockIdx.x == 0 && threadIdx.x < kSlotWords) p.next_slot[threadIdx.x] = 0u;
-
   Stack sa{0, 0, 0}, sb{0, 0, 0};
   bool active = false, exhausted = false;
   bool pending = false;  // the lane's completed ray still has to write its hit record
