@@ -94,3 +94,27 @@ def test_empty_batch_and_record_overflow(gpu, rtx, c1):
     assert total > 8
     recb = gpu.buffer(total * 16)
     assert gpu.trace_device(rb, n, hb, records_buf=recb, record_capacity=total, offsets_buf=ob) == 0
+
+
+def test_many_calls_and_errors_reuse_control_state(gpu, rtx, oracle, c1):
+    """Successive calls share per-context control state (ray counter, error
+    word, statistics): results stay exact over more calls than it holds slots,
+    and an error in one call does not leak into the next."""
+    scene, rays = c1
+    sub = rays[:2048]
+    ref = oracle.trace(scene.arena, scene.tlas_offset, sub, records=False)
+    bad = tiny_scene.tiny_arena().copy()
+    bad[64 + 12:64 + 16] = np.frombuffer(np.int32(1000).tobytes(), np.uint8)
+    ray = tiny_scene.tiny_ray(rtx.RAY_DTYPE)
+    rb, hb = gpu.to_device(sub), gpu.buffer(len(sub) * 64)
+    for k in range(150):
+        if k in (40, 41, 100):
+            gpu.set_scene(bad, 0)
+            with pytest.raises(rtx.RtxError) as e:
+                gpu.trace(ray, mode=0)
+            assert e.value.status == 3
+        gpu.set_scene(scene.arena, scene.tlas_offset)
+        assert gpu.trace_device(rb, len(sub), hb) == 0
+        if k % 25 == 0 or k in (40, 41, 100):
+            from conftest import assert_hits_equal
+            assert_hits_equal(hb.download(rtx.HIT_DTYPE, len(sub)), ref, f"call {k}")
