@@ -167,10 +167,13 @@ def main():
     ctx = rtx.Context(0)
     info = ctx.device_info()
     t0 = time.perf_counter()
-    scene = rtx.workload_scene(desc, ctx=ctx if args.build == "device" else None)
+    builder = rtx.workload_builder(desc)
+    mesh_s = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    scene = builder.finish(ctx=ctx if args.build == "device" else None)
     build_s = time.perf_counter() - t0
     log(f"scene: {scene.n_triangles} tris, {scene.n_blas} BLAS, {scene.arena.nbytes / 2**20:.1f} MiB arena, "
-        f"built ({args.build}) in {build_s:.2f} s")
+        f"meshes in {mesh_s:.2f} s, BVH built ({args.build}) in {build_s:.2f} s")
     ctx.set_scene(scene.arena, scene.tlas_offset)
     if args.copy_probe:
         a, b = ctx.buffer(args.copy_probe << 20), ctx.buffer(args.copy_probe << 20)
@@ -320,6 +323,7 @@ def main():
             "arena_mib": round(scene.arena.nbytes / 2**20, 2),
             "bvh_build": args.build,
             "bvh_build_s": round(build_s, 3),
+            "mesh_gen_s": round(mesh_s, 3),
             "device": info["name"],
             "n_cus": info["n_cus"],
         },

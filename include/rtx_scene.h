@@ -83,6 +83,10 @@ enum {
 rtx_status rtx_workload_scene(const rtx_workload_desc* d, uint32_t threads, rtx_builder** out);
 /* The same scene with its BLASes built on the device of ctx (rtx_builder_finish_device). */
 rtx_status rtx_workload_scene_device(const rtx_workload_desc* d, rtx_ctx* ctx, rtx_builder** out);
+/* The workload's meshes and instances in a fresh builder, not yet built: finish
+ * it with rtx_builder_finish or rtx_builder_finish_device (so the build can be
+ * timed apart from mesh generation). */
+rtx_status rtx_workload_builder(const rtx_workload_desc* d, rtx_builder** out);
 /* Pinhole primary rays through pixel centres, one per pixel, ordered by 8x8
  * pixel tiles (tiles row-major, pixels row-major inside a tile). */
 rtx_status rtx_workload_primary_rays(const rtx_workload_desc* d, rtx_ray* rays_out);

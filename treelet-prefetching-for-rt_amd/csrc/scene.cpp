@@ -195,6 +195,20 @@ rtx_status rtx_workload_scene_device(const rtx_workload_desc* d, rtx_ctx* ctx, r
   return RTX_OK;
 }
 
+rtx_status rtx_workload_builder(const rtx_workload_desc* d, rtx_builder** out) {
+  if (!d || !out) return RTX_ERR_INVALID_ARGUMENT;
+  rtx_builder* b = nullptr;
+  rtx_status st = rtx_builder_create(&b);
+  if (st != RTX_OK) return st;
+  st = populate_workload(d, b);
+  if (st != RTX_OK) {
+    rtx_builder_destroy(b);
+    return st;
+  }
+  *out = b;
+  return RTX_OK;
+}
+
 rtx_status rtx_workload_primary_rays(const rtx_workload_desc* d, rtx_ray* rays) {
   if (!d || !rays || d->width == 0 || d->height == 0) return RTX_ERR_INVALID_ARGUMENT;
   Camera c = workload_camera(d);

@@ -153,6 +153,7 @@ _sig("rtx_builder_add_instance", _i32, _vp, _u32, _vp, _u32, _u32)
 _sig("rtx_builder_finish", _i32, _vp, _u32, _P(_u64), _P(_u64))
 _sig("rtx_builder_finish_device", _i32, _vp, _vp, _P(_u64), _P(_u64))
 _sig("rtx_workload_scene_device", _i32, _P(_WorkloadDesc), _vp, _P(_vp))
+_sig("rtx_workload_builder", _i32, _P(_WorkloadDesc), _P(_vp))
 _sig("rtx_builder_copy_arena", _i32, _vp, _vp, ctypes.c_size_t)
 _sig("rtx_builder_arena_info", _i32, _vp, _P(_u64), _P(_u64))
 _sig("rtx_builder_counts", _i32, _vp, _P(_u64), _P(_u64), _P(_u64), _P(_u64))
@@ -173,7 +174,7 @@ EXPORTED_SYMBOLS = [
     "rtx_set_prefetch_table", "rtx_mlp_create", "rtx_mlp_destroy", "rtx_mlp_param_count", "rtx_mlp_get_params",
     "rtx_mlp_set_params", "rtx_mlp_loss_grad", "rtx_mlp_train", "rtx_mlp_predict", "rtx_mlp_gemm",
     "rtx_builder_create", "rtx_builder_destroy", "rtx_builder_add_blas", "rtx_builder_add_instance",
-    "rtx_builder_finish", "rtx_builder_finish_device", "rtx_workload_scene_device", "rtx_builder_copy_arena", "rtx_builder_arena_info", "rtx_builder_counts", "rtx_builder_blas_offset",
+    "rtx_builder_finish", "rtx_builder_finish_device", "rtx_workload_scene_device", "rtx_workload_builder", "rtx_builder_copy_arena", "rtx_builder_arena_info", "rtx_builder_counts", "rtx_builder_blas_offset",
     "rtx_mesh_displaced_sphere", "rtx_workload_scene", "rtx_workload_primary_rays", "rtx_workload_secondary_rays",
     "rtx_format_treelet_trace", "rtx_as_dump_write", "rtx_as_dump_info", "rtx_as_dump_read",
 ]
@@ -311,6 +312,15 @@ def workload_scene(desc, threads=0, ctx=None):
     s = Scene(h.value)
     s._capture_finished()
     return s
+
+
+def workload_builder(desc):
+    """The workload's meshes and instances in an unfinished SceneBuilder."""
+    b = SceneBuilder.__new__(SceneBuilder)
+    h = ctypes.c_void_p()
+    _check(_lib.rtx_workload_builder(ctypes.byref(desc), ctypes.byref(h)), "rtx_workload_builder")
+    b._h = h
+    return b
 
 
 def primary_rays(desc):
