@@ -64,6 +64,31 @@ def test_device_blas_layout(gpu, rtx, oracle):
     assert internal <= len(tris)
 
 
+def test_device_blas_layout_mixed_sizes(gpu, rtx, oracle):
+    """BLAS above and below 2,048 triangles in one build: the boxes of the
+    small ones are formed per workgroup in LDS, those of the large one through
+    device-wide counters; both must give the mesh bounds and hold their leaves."""
+    meshes = [rtx.displaced_sphere(64, 40, seed=9), rtx.displaced_sphere(32, 16, seed=5),
+              rtx.displaced_sphere(32, 32, seed=2), rtx.displaced_sphere(3, 2, seed=1)]
+    assert [len(m) for m in meshes] == [5120, 1024, 2048, 12]
+    b = rtx.SceneBuilder()
+    for k, m in enumerate(meshes):
+        b.add_instance(b.add_blas(m, geometry_index=k), np.array([[1, 0, 0, 4 * k], [0, 1, 0, 0], [0, 0, 1, 0]],
+                                                                   np.float32), instance_id=k)
+    s = b.finish(ctx=gpu)
+    A = s.arena
+    assert oracle.validate(A, s.tlas_offset) > sum(len(m) for m in meshes)
+    for k, m in enumerate(meshes):
+        hdr = s.blas_offset(k)
+        root = hdr + struct.unpack_from("<Q", A, hdr)[0]
+        leaves, internal, lo, hi = _walk(A, root, oracle)
+        assert sorted(struct.unpack_from("<I", A, l + 8)[0] for l in leaves) == list(range(len(m)))
+        hb = np.frombuffer(A[hdr + 8:hdr + 32].tobytes(), np.float32)
+        v = m.reshape(-1, 3)
+        assert np.array_equal(hb[:3], v.min(0)) and np.array_equal(hb[3:], v.max(0)), k
+        assert np.array_equal(lo, v.min(0)) and np.array_equal(hi, v.max(0)), k
+
+
 def test_device_blas_single_triangle_and_many_blas(gpu, rtx, oracle):
     b = rtx.SceneBuilder()
     one = b.add_blas(np.array([[0, 0, 0, 1, 0, 0, 0, 1, 0]], np.float32), geometry_index=1)

@@ -461,16 +461,18 @@ rtx_status rtx_builder_add_instance(rtx_builder* b, uint32_t blas_id, const floa
 namespace rtx_internal {
 rtx_status ctx_stream(rtx_ctx* c, hipStream_t* s, int* device);
 rtx_status gpu_build_blas_pool(hipStream_t s, const float* tris_host, const uint32_t* pofs_host,
-                               const uint32_t* geom_host, uint32_t n_blas, std::vector<uint8_t>& pool_out,
+                               const uint32_t* geom_host, uint32_t n_blas, uint8_t** pool_dev, uint64_t* pool_bytes,
                                std::vector<float>& boxes_out);
 }  // namespace rtx_internal
 
 namespace {
 
 // TLAS over the instances and the arena: TLAS at 0, then the bottom levels.
-// With `pool` set, the bottom levels are one pool whose BLAS b header sits at
-// byte 64 b; otherwise each BLAS's blob follows the previous one.
-rtx_status assemble(rtx_builder* b, const std::vector<uint8_t>* pool, uint64_t* arena_bytes, uint64_t* tlas_offset) {
+// With `pool_dev` set, the bottom levels are one pool of pool_bytes in device
+// memory whose BLAS b header sits at byte 64 b, copied straight into the
+// arena on stream s; otherwise each BLAS's blob follows the previous one.
+rtx_status assemble(rtx_builder* b, const uint8_t* pool_dev, uint64_t pool_bytes, hipStream_t s, uint64_t* arena_bytes,
+                    uint64_t* tlas_offset) {
   // Instance world boxes: the 8 transformed corners of the BLAS box, rounded outward.
   std::vector<Box> boxes(b->inst.size());
   for (size_t k = 0; k < b->inst.size(); ++k) {
@@ -526,9 +528,9 @@ rtx_status assemble(rtx_builder* b, const std::vector<uint8_t>* pool, uint64_t* 
     }
   }
   uint64_t cursor = tlas.size();
-  if (pool) {
+  if (pool_dev) {
     for (size_t k = 0; k < b->blas.size(); ++k) b->blas[k].offset = cursor + 64 * k;
-    cursor += pool->size();
+    cursor += pool_bytes;
   } else {
     for (auto& bl : b->blas) {
       bl.offset = cursor;
@@ -541,8 +543,12 @@ rtx_status assemble(rtx_builder* b, const std::vector<uint8_t>* pool, uint64_t* 
     return RTX_ERR_OUT_OF_MEMORY;
   }
   memcpy(b->arena.data(), tlas.data(), tlas.size());
-  if (pool) {
-    memcpy(b->arena.data() + tlas.size(), pool->data(), pool->size());
+  if (pool_dev) {
+    if (hipMemcpyAsync(b->arena.data() + tlas.size(), pool_dev, pool_bytes, hipMemcpyDeviceToHost, s) != hipSuccess ||
+        hipStreamSynchronize(s) != hipSuccess) {
+      (void)hipGetLastError();
+      return RTX_ERR_HIP;
+    }
   } else {
     for (auto& bl : b->blas) {
       memcpy(b->arena.data() + bl.offset, bl.blob.data(), bl.blob.size());
@@ -579,7 +585,7 @@ rtx_status rtx_builder_finish(rtx_builder* b, uint32_t threads, uint64_t* arena_
   for (auto& t : pool) t.join();
   for (auto& bl : b->blas)
     if (!bl.ok) return RTX_ERR_INVALID_ARGUMENT;
-  return assemble(b, nullptr, arena_bytes, tlas_offset);
+  return assemble(b, nullptr, 0, nullptr, arena_bytes, tlas_offset);
 }
 
 rtx_status rtx_builder_finish_device(rtx_builder* b, rtx_ctx* ctx, uint64_t* arena_bytes, uint64_t* tlas_offset) {
@@ -600,18 +606,22 @@ rtx_status rtx_builder_finish_device(rtx_builder* b, rtx_ctx* ctx, uint64_t* are
   }
   if (n >= (1ull << 31)) return RTX_ERR_INVALID_ARGUMENT;
   pofs[B] = (uint32_t)n;
-  std::vector<float> tris;
-  try {
-    tris.resize(n * 9);
-  } catch (...) {
-    return RTX_ERR_OUT_OF_MEMORY;
-  }
-  for (uint32_t k = 0; k < B; ++k) memcpy(tris.data() + 9ull * pofs[k], b->blas[k].tris.data(), b->blas[k].tris.size() * 4);
-  std::vector<uint8_t> pool;
+  // gathered without zero-filling first: every float is written below
+  std::unique_ptr<float[]> tris(new (std::nothrow) float[n * 9]);
+  if (!tris) return RTX_ERR_OUT_OF_MEMORY;
+  for (uint32_t k = 0; k < B; ++k) memcpy(tris.get() + 9ull * pofs[k], b->blas[k].tris.data(), b->blas[k].tris.size() * 4);
+  struct DevPool {
+    uint8_t* p = nullptr;
+    ~DevPool() {
+      if (p) (void)hipFree(p);
+    }
+  } pool;
+  uint64_t pool_bytes = 0;
   std::vector<float> boxes;
-  rtx_status st = rtx_internal::gpu_build_blas_pool(s, tris.data(), pofs.data(), geom.data(), B, pool, boxes);
+  rtx_status st = rtx_internal::gpu_build_blas_pool(s, tris.get(), pofs.data(), geom.data(), B, &pool.p, &pool_bytes, boxes);
+  tris.reset();
   if (st != RTX_OK) return st;
-  const uint64_t slots = pool.size() / 64;
+  const uint64_t slots = pool_bytes / 64;
   for (uint32_t k = 0; k < B; ++k) {
     Blas& bl = b->blas[k];
     for (int a = 0; a < 3; ++a) {
@@ -623,7 +633,7 @@ rtx_status rtx_builder_finish_device(rtx_builder* b, rtx_ctx* ctx, uint64_t* are
   }
   // every slot of the pool is a header, an internal node or a quad leaf
   b->blas[0].n_internal = slots - B - n;
-  return assemble(b, &pool, arena_bytes, tlas_offset);
+  return assemble(b, pool.p, pool_bytes, s, arena_bytes, tlas_offset);
 }
 
 rtx_status rtx_builder_copy_arena(const rtx_builder* b, void* dst, size_t capacity) {

@@ -72,15 +72,48 @@ struct BuildParams {
   uint32_t* flags;    // [n_int]
 };
 
+// BLAS b owns triangles [pofs[b], pofs[b+1]): one workgroup per BLAS writes
+// its id over that range.
+__global__ void k_blas_of(BuildParams p, uint32_t* blas_of) {
+  const uint32_t b = blockIdx.x;
+  for (uint32_t t = p.pofs[b] + threadIdx.x; t < p.pofs[b + 1]; t += blockDim.x) blas_of[t] = b;
+}
+
+// Centroid bounds per BLAS as ordered-int atomics.  Triangles of a BLAS are
+// contiguous, so most waves hold one BLAS: those reduce across the wave first
+// and issue one atomic per bound instead of 64.
 __global__ void k_centroid_bounds(BuildParams p) {
   const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= p.n_tris) return;
-  const float* v = p.tris + 9 * (size_t)t;
-  const uint32_t b = p.blas_of[t];
+  const bool ok = t < p.n_tris;
+  const uint32_t b = ok ? p.blas_of[t] : 0xffffffffu;
+  uint32_t lo[3], hi[3];
   for (int a = 0; a < 3; ++a) {
-    const float c = (v[a] + v[3 + a] + v[6 + a]) * (1.0f / 3.0f);
-    atomicMin(&p.cmin[3 * b + a], fkey(c));
-    atomicMax(&p.cmax[3 * b + a], fkey(c));
+    lo[a] = 0xffffffffu;
+    hi[a] = 0u;
+    if (ok) {
+      const float* v = p.tris + 9 * (size_t)t;
+      lo[a] = hi[a] = fkey((v[a] + v[3 + a] + v[6 + a]) * (1.0f / 3.0f));
+    }
+  }
+  // lane 0 holds the lowest t of the wave, so it is valid whenever any lane is
+  const uint32_t b0 = __shfl(b, 0);
+  if (__all(!ok || b == b0)) {
+    if (b0 == 0xffffffffu) return;
+    for (int off = 32; off >= 1; off >>= 1)
+      for (int a = 0; a < 3; ++a) {
+        lo[a] = min(lo[a], (uint32_t)__shfl_xor((int)lo[a], off));
+        hi[a] = max(hi[a], (uint32_t)__shfl_xor((int)hi[a], off));
+      }
+    if ((threadIdx.x & 63u) == 0u)
+      for (int a = 0; a < 3; ++a) {
+        atomicMin(&p.cmin[3 * b0 + a], lo[a]);
+        atomicMax(&p.cmax[3 * b0 + a], hi[a]);
+      }
+  } else if (ok) {
+    for (int a = 0; a < 3; ++a) {
+      atomicMin(&p.cmin[3 * b + a], lo[a]);
+      atomicMax(&p.cmax[3 * b + a], hi[a]);
+    }
   }
 }
 
@@ -164,15 +197,78 @@ __device__ __forceinline__ float ld_coherent(const float* a) {
                                            __HIP_MEMORY_SCOPE_AGENT));
 }
 
-__global__ void k_boxes(BuildParams p) {
-  const uint32_t pos = blockIdx.x * blockDim.x + threadIdx.x;
-  if (pos >= p.n_tris) return;
+// Box of the triangle at sorted position pos.
+__device__ __forceinline__ void leaf_box(const BuildParams& p, uint32_t pos, float bx[6]) {
   const float* v = p.tris + 9 * (size_t)p.svals[pos];
-  float bx[6];
   for (int a = 0; a < 3; ++a) {
     bx[a] = fminf(fminf(v[a], v[3 + a]), v[6 + a]);
     bx[3 + a] = fmaxf(fmaxf(v[a], v[3 + a]), v[6 + a]);
   }
+}
+
+// Largest BLAS (in triangles) whose boxes k_boxes_lds forms; k_boxes takes
+// the larger ones.
+constexpr uint32_t kLdsBoxLeaves = 2048;
+
+// Boxes of one BLAS per workgroup, for BLAS of at most kLdsBoxLeaves
+// triangles: the bottom-up climb of k_boxes with the arrival counters and the
+// internal boxes in LDS (cap internal nodes: 6 floats each, then one counter
+// each), so every hand-over stays inside the workgroup.  The second child to
+// arrive forms the parent from both children, left then right, as k_boxes
+// does, so both kernels give the same bits; a leaf child's box is recomputed
+// from its triangle.
+__global__ void k_boxes_lds(BuildParams p, uint32_t cap) {
+  extern __shared__ uint32_t lds[];
+  float* sbox = reinterpret_cast<float*>(lds);
+  uint32_t* sflag = lds + 6 * cap;
+  const uint32_t b = blockIdx.x;
+  const uint32_t base = p.pofs[b], n = p.pofs[b + 1] - base, ibase = p.iofs[b];
+  if (n > kLdsBoxLeaves || n - 1 > cap) return;
+  for (uint32_t i = threadIdx.x; i + 1 < n; i += blockDim.x) sflag[i] = 0u;
+  __syncthreads();
+  for (uint32_t l = threadIdx.x; l < n; l += blockDim.x) {
+    const uint32_t pos = base + l;
+    float u[6];
+    leaf_box(p, pos, u);
+    for (int c = 0; c < 6; ++c) p.lbox[6 * (size_t)pos + c] = u[c];
+    uint32_t cur = kLeafFlag | pos, node = p.lparent[pos];
+    while (node != kNoParent) {
+      const uint32_t li = node - ibase;
+      if (__hip_atomic_fetch_add(&sflag[li], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_WORKGROUP) == 0u)
+        break;  // the sibling forms the parent
+      const uint32_t lref = p.left[node], rref = p.right[node];
+      const uint32_t sib = lref == cur ? rref : lref;
+      float s[6];
+      if (sib & kLeafFlag)
+        leaf_box(p, sib & ~kLeafFlag, s);
+      else
+        for (int c = 0; c < 6; ++c) s[c] = sbox[6 * (sib - ibase) + c];
+      const bool own_left = lref == cur;
+      for (int c = 0; c < 6; ++c) {
+        const float l0 = own_left ? u[c] : s[c], r0 = own_left ? s[c] : u[c];
+        u[c] = c < 3 ? fminf(fminf(INFINITY, l0), r0) : fmaxf(fmaxf(-INFINITY, l0), r0);
+      }
+      for (int c = 0; c < 6; ++c) {
+        sbox[6 * li + c] = u[c];
+        p.ibox[6 * (size_t)node + c] = u[c];
+      }
+      cur = node;
+      node = p.iparent[node];
+    }
+  }
+}
+
+// Boxes bottom-up over the leaves of BLAS larger than kLdsBoxLeaves (all
+// BLAS when lds_leaves is 0), through device-wide counters.
+__global__ void k_boxes(BuildParams p, uint32_t lds_leaves) {
+  const uint32_t pos = blockIdx.x * blockDim.x + threadIdx.x;
+  if (pos >= p.n_tris) return;
+  if (lds_leaves) {
+    const uint32_t b = p.blas_of[p.svals[pos]];
+    if (p.pofs[b + 1] - p.pofs[b] <= lds_leaves) return;
+  }
+  float bx[6];
+  leaf_box(p, pos, bx);
   for (int c = 0; c < 6; ++c) p.lbox[6 * (size_t)pos + c] = bx[c];
   uint32_t node = p.lparent[pos];
   while (node != kNoParent) {
@@ -398,20 +494,28 @@ struct DevBuf {
 namespace rtx_internal {
 
 // Builds the bottom levels of B meshes on the device.  tris: all triangles,
-// 9 floats each, BLAS b owning [pofs[b], pofs[b+1]).  Returns the pool bytes
-// (header of BLAS b at byte 64 b) and each BLAS's bounds.
+// 9 floats each, BLAS b owning [pofs[b], pofs[b+1]).  Returns the pool in
+// device memory (*pool_dev, *pool_bytes; header of BLAS b at byte 64 b, freed
+// by the caller with hipFree) and each BLAS's bounds.
 rtx_status gpu_build_blas_pool(hipStream_t s, const float* tris_host, const uint32_t* pofs_host,
-                               const uint32_t* geom_host, uint32_t n_blas, std::vector<uint8_t>& pool_out,
+                               const uint32_t* geom_host, uint32_t n_blas, uint8_t** pool_dev, uint64_t* pool_bytes,
                                std::vector<float>& boxes_out) {
-  if (n_blas == 0) return RTX_ERR_INVALID_ARGUMENT;
+  if (n_blas == 0 || !pool_dev || !pool_bytes) return RTX_ERR_INVALID_ARGUMENT;
+  *pool_dev = nullptr;
+  *pool_bytes = 0;
   const uint32_t N = pofs_host[n_blas];
   for (uint32_t b = 0; b < n_blas; ++b)
     if (pofs_host[b + 1] <= pofs_host[b]) return RTX_ERR_INVALID_ARGUMENT;  // every BLAS holds a triangle
   const uint32_t n_int = N - n_blas;
-  std::vector<uint32_t> blas_of(N), iofs(n_blas + 1);
+  std::vector<uint32_t> iofs(n_blas + 1);
+  // lds_cap: internal nodes of the largest BLAS that k_boxes_lds takes
+  uint32_t lds_cap = 0, max_n = 0, min_n = 0xffffffffu;
   for (uint32_t b = 0; b < n_blas; ++b) {
     iofs[b] = pofs_host[b] - b;
-    for (uint32_t t = pofs_host[b]; t < pofs_host[b + 1]; ++t) blas_of[t] = b;
+    const uint32_t n = pofs_host[b + 1] - pofs_host[b];
+    max_n = std::max(max_n, n);
+    min_n = std::min(min_n, n);
+    if (n <= kLdsBoxLeaves) lds_cap = std::max(lds_cap, n - 1);
   }
   iofs[n_blas] = n_int;
   const uint64_t max_slots = 2ull * n_blas + 2ull * N;
@@ -439,7 +543,6 @@ rtx_status gpu_build_blas_pool(hipStream_t s, const float* tris_host, const uint
     }                             \
   } while (0)
   GB_TRY(hipMemcpyAsync(tris.p, tris_host, (size_t)N * 36, hipMemcpyHostToDevice, s));
-  GB_TRY(hipMemcpyAsync(d_blas_of.p, blas_of.data(), (size_t)N * 4, hipMemcpyHostToDevice, s));
   GB_TRY(hipMemcpyAsync(d_pofs.p, pofs_host, (size_t)(n_blas + 1) * 4, hipMemcpyHostToDevice, s));
   GB_TRY(hipMemcpyAsync(d_iofs.p, iofs.data(), (size_t)(n_blas + 1) * 4, hipMemcpyHostToDevice, s));
   GB_TRY(hipMemcpyAsync(d_geom.p, geom_host, (size_t)n_blas * 4, hipMemcpyHostToDevice, s));
@@ -472,6 +575,7 @@ rtx_status gpu_build_blas_pool(hipStream_t s, const float* tris_host, const uint
   p.lbox = lbox.p;
   p.flags = flags.p;
   const uint32_t bs = 256;
+  hipLaunchKernelGGL(k_blas_of, dim3(n_blas), dim3(bs), 0, s, p, d_blas_of.p);
   hipLaunchKernelGGL(k_centroid_bounds, dim3((N + bs - 1) / bs), dim3(bs), 0, s, p);
   hipLaunchKernelGGL(k_morton, dim3((N + bs - 1) / bs), dim3(bs), 0, s, p);
   GB_TRY(hipGetLastError());
@@ -487,7 +591,10 @@ rtx_status gpu_build_blas_pool(hipStream_t s, const float* tris_host, const uint
   }
   GB_TRY(hipcub::DeviceRadixSort::SortPairs(tmp.p, tmp_bytes, keys.p, skeys.p, vals.p, svals.p, (int)N, 0, end_bit, s));
   if (n_int > 0) hipLaunchKernelGGL(k_karras, dim3((n_int + bs - 1) / bs), dim3(bs), 0, s, p);
-  hipLaunchKernelGGL(k_boxes, dim3((N + bs - 1) / bs), dim3(bs), 0, s, p);
+  if (min_n <= kLdsBoxLeaves)
+    hipLaunchKernelGGL(k_boxes_lds, dim3(n_blas), dim3(bs), (size_t)lds_cap * 28, s, p, lds_cap);
+  if (max_n > kLdsBoxLeaves)
+    hipLaunchKernelGGL(k_boxes, dim3((N + bs - 1) / bs), dim3(bs), 0, s, p, kLdsBoxLeaves);
   hipLaunchKernelGGL(k_headers, dim3((n_blas + bs - 1) / bs), dim3(bs), 0, s, p, pool.p, rboxes.p, f0.p, s0.p);
   GB_TRY(hipGetLastError());
 
@@ -531,13 +638,15 @@ rtx_status gpu_build_blas_pool(hipStream_t s, const float* tris_host, const uint
   }
   uint32_t e = 0;
   GB_TRY(hipMemcpyAsync(&e, err.p, 4, hipMemcpyDeviceToHost, s));
-  pool_out.resize((size_t)cursor * 64);
   boxes_out.resize((size_t)n_blas * 6);
-  GB_TRY(hipMemcpyAsync(pool_out.data(), pool.p, pool_out.size(), hipMemcpyDeviceToHost, s));
   GB_TRY(hipMemcpyAsync(boxes_out.data(), rboxes.p, boxes_out.size() * 4, hipMemcpyDeviceToHost, s));
   GB_TRY(hipStreamSynchronize(s));
 #undef GB_TRY
-  return e ? RTX_ERR_INVALID_ARGUMENT : RTX_OK;
+  if (e) return RTX_ERR_INVALID_ARGUMENT;
+  *pool_dev = pool.p;  // the caller frees it with hipFree
+  pool.p = nullptr;
+  *pool_bytes = (uint64_t)cursor * 64;
+  return RTX_OK;
 }
 
 }  // namespace rtx_internal
