@@ -13,7 +13,8 @@ struct Partition {
   uint32_t n = 0;              // treelets
   uint32_t first_treelet = 0;  // dense index of the treelet rooted at the TLAS header
   uint64_t n_list = 0;
-  uint32_t* owner = nullptr;         // [arena_bytes/64] dense treelet index or RTX_NO_TREELET
+  uint32_t* owner = nullptr;         // [arena_bytes/64] dense treelet index or RTX_NO_TREELET, followed by
+                                     // [arena_bytes/64] bytes of first-treelet child bits (first_bits_kernel)
   uint64_t* roots = nullptr;         // [n] root offsets, ascending
   uint64_t* list_offsets = nullptr;  // [n+1]
   uint64_t* list_nodes = nullptr;    // [n_list] node offsets

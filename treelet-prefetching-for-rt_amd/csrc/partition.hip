@@ -622,8 +622,41 @@ __global__ __launch_bounds__(256) void image_copy(uint32_t n, const uint64_t* lo
 
 }  // namespace
 
+// Children owned by the first treelet, one byte per node slot after the owner
+// words: for an internal node bit i is set when child i is in the arena and
+// owner[child i] == first; for an instance leaf bit 0 is set when its BLAS
+// root is.  Treelet-order traversal tests exactly these ownerships before its
+// first treelet switch, so it reads this byte instead of six owner words.
+__global__ void first_bits_kernel(const uint8_t* A, uint64_t bytes, const uint8_t* kind, const uint32_t* owner,
+                                  uint32_t first, uint64_t slots, uint8_t* out) {
+  for (uint64_t sl = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; sl < slots;
+       sl += (uint64_t)gridDim.x * blockDim.x) {
+    const uint64_t off = sl << 6;
+    uint32_t bits = 0;
+    if (kind[sl] == kSlotInternal) {
+      const uint8_t* nd = A + off;
+      uint64_t c = off + (uint64_t)((int64_t)*reinterpret_cast<const int32_t*>(nd + 12) * 64);
+      for (int i = 0; i < 6; ++i) {
+        const uint32_t size = nd[22 + i] & 3u;
+        if (size != 0 && !(c & 63) && c + 64 <= bytes && owner[c >> 6] == first) bits |= 1u << i;
+        c += (uint64_t)size * 64;
+      }
+    } else if (kind[sl] == kSlotInstance && off + 128 <= bytes) {
+      const uint64_t h = off + *reinterpret_cast<const uint64_t*>(A + off + 64);
+      if (!(h & 63) && h + 64 <= bytes) {
+        const uint64_t r = h + *reinterpret_cast<const uint64_t*>(A + h);
+        if (!(r & 63) && r + 64 <= bytes && owner[r >> 6] == first) bits = 1u;
+      }
+    }
+    out[sl] = (uint8_t)bits;
+  }
+}
+
 rtx_status build_images(const uint8_t* A, uint64_t bytes, hipStream_t s, const uint8_t* kind, Partition& p) {
   const uint64_t slots = bytes / 64;
+  hipLaunchKernelGGL(first_bits_kernel, dim3((uint32_t)std::min<uint64_t>(8192, (slots + 255) / 256)), dim3(256), 0, s, A,
+                     bytes, kind, p.owner, p.first_treelet, slots, reinterpret_cast<uint8_t*>(p.owner + slots));
+  CK(hipGetLastError());
   CS(dmalloc(&p.stage_slot, slots));
   CS(dmalloc(&p.image_bytes, p.n));
   CS(dmalloc(&p.image_offset, (size_t)p.n + 1));
@@ -750,7 +783,7 @@ rtx_status partition_build(const uint8_t* A, uint64_t bytes, uint64_t tlas, uint
   Partition p;
   p.budget = budget;
   p.n = n;
-  CS(dmalloc(&p.owner, slots));
+  CS(dmalloc(&p.owner, slots + (slots + 3) / 4));  // owner words, then the first-treelet bytes
   CS(dmalloc(&p.roots, n));
   CS(dmalloc(&p.list_offsets, (size_t)n + 1));
   hipLaunchKernelGGL(owner_and_roots, dim3(2048), dim3(256), 0, s, slots, root_flag, rank, owner_root1, p.owner,
@@ -846,7 +879,7 @@ rtx_status partition_install(const uint8_t* A, uint64_t bytes, uint64_t tlas, ui
   p.n = n;
   p.n_list = n_list;
   p.first_treelet = ft;
-  CS(dmalloc(&p.owner, slots));
+  CS(dmalloc(&p.owner, slots + (slots + 3) / 4));  // owner words, then the first-treelet bytes
   CS(dmalloc(&p.roots, n));
   CS(dmalloc(&p.list_offsets, (size_t)n + 1));
   CS(dmalloc(&p.list_nodes, n_list));

@@ -16,9 +16,12 @@
 //    to the "current" list while the current treelet is still the one rooted
 //    at the TLAS header and the child's owner is that treelet; after the
 //    first move from the "other" list the reference compares against a host
-//    address (:1752), so every later child goes to "other".  Entries carry the
-//    instance leaf that opened them; the object-space ray is recomputed from
-//    that leaf when it differs from the cached one (same inputs, same floats).
+//    address (:1752), so every later child goes to "other".  Which children
+//    the first treelet owns is precomputed with the partition, one byte per
+//    node slot, so a node costs one extra byte load, not one per child.
+//    Entries carry the instance leaf that opened them; the object-space ray
+//    is recomputed from that leaf when it differs from the cached one (same
+//    inputs, same floats).
 //
 // Stacks: a ring of K entries per lane in LDS, spilling the oldest entries to
 // a per-lane region in global memory (exact LIFO order).
@@ -231,6 +234,12 @@ void trace_kernel(TraceParams p) {
     if (MODE == 0) return true;
     if (phase2) return false;
     return owner[slot] == first_treelet;
+  };
+  // The first-treelet child bits of node slot `slot`: one byte per slot after
+  // the owner words (bit i: child i of an internal node is owned by the first
+  // treelet; bit 0 of an instance leaf: its BLAS root is).
+  auto first_bits = [&](uint32_t slot) -> uint32_t {
+    return reinterpret_cast<const uint8_t*>(owner + (bytes >> 6))[slot];
   };
 
   // Position of `slot` in the staged image, or kNone.
@@ -705,7 +714,7 @@ void trace_kernel(TraceParams p) {
             } else {
               const uint32_t rs = (uint32_t)(root >> 6);
               if (level > max_level) max_level = level;
-              const bool to_a = goes_current(rs);
+              const bool to_a = MODE == 0 || (!phase2 && (first_bits(slot) & 1u));
               // The BLAS root is popped next when it goes on top of the list
               // the next pop takes: always in plain order; in treelet order
               // when it joins the current list, or the other list while the
@@ -786,6 +795,9 @@ void trace_kernel(TraceParams p) {
             err |= kErrRange;
           } else {
             const Words16 nd = read_node(off, STAGE > 0 ? staged_pos(slot) : pos);
+            // Treelet order before the first switch: which children the first
+            // treelet owns (read beside the node, used after its tests)
+            const uint32_t fbits = (MODE == 1 && !phase2) ? first_bits(slot) : 0u;
             emit(off, 64, RTX_TX_BVH_INTERNAL_NODE, level);
             n_nodes++;
             const bool top = (flags & kFlTop) != 0;
@@ -840,23 +852,7 @@ void trace_kernel(TraceParams p) {
               const uint32_t base_flags = top ? kFlTop : 0u;
               int deferred = -1;
               uint32_t deferred_cs = 0;
-              // Treelet order before the first switch: which hit children the
-              // first treelet owns.  The six owner loads are issued together
-              // (a child that is not pushed reads this node's own entry).
-              uint32_t cur_bits = 0;
-              if (MODE == 1 && !phase2) {
-                uint32_t ow[6];
-                uint64_t c = first_child;
-#pragma unroll
-                for (int i = 0; i < 6; ++i) {
-                  const bool take = (hitmask >> i) & 1u && !(c & 63) && c + 64 <= bytes;
-                  ow[i] = owner[take ? (uint32_t)(c >> 6) : slot];
-                  c += (uint64_t)(byte_of(nd, 22 + i) & 3u) * 64;
-                }
-#pragma unroll
-                for (int i = 0; i < 6; ++i)
-                  if (ow[i] == first_treelet) cur_bits |= 1u << i;
-              }
+              const uint32_t cur_bits = fbits;
               uint64_t child = first_child;
 #pragma unroll
               for (int i = 0; i < 6; ++i) {
