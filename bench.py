@@ -16,7 +16,10 @@ around synchronous rtx_trace calls); every mode is measured and reported.
 grid per pixel) and 3 cosine-weighted diffuse bounces per path without Russian
 roulette; a step traces the four segments (camera rays, then each bounce) once.
 
-  python bench.py [--steps K] [--warmup W] [--budget BYTES] [--mode NAME] [--workload diffuse|large]
+--workload c1: the small scene the reference's CPU path runs, a displaced UV sphere
+of 16,384 triangles over a 2-triangle ground quad, 128x128 primary rays.
+
+  python bench.py [--steps K] [--warmup W] [--budget BYTES] [--mode NAME] [--workload diffuse|large|c1]
 """
 import argparse
 import json
@@ -53,6 +56,11 @@ def c5_desc(rtx, width=7680, height=4320):
     return rtx.workload_desc(rtx.WORKLOAD_INSTANCED_SPHERES, n_instances=n_inst, slices=32, stacks=16, shared_blas=0,
                              width=width, height=height, seed=2025, box_size=100.0 * (n_inst / 1024) ** (1 / 3),
                              scale_min=0.5, scale_max=2.0, displacement=0.05)
+
+
+def c1_desc(rtx, width=128, height=128):
+    return rtx.workload_desc(rtx.WORKLOAD_SPHERE_AND_QUAD, slices=128, stacks=64, width=width, height=height, seed=7,
+                             displacement=0.05)
 
 
 def c2_desc(rtx, width=1024, height=1024):
@@ -143,7 +151,7 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--width", type=int, default=0, help="override the image width (height follows the aspect)")
-    ap.add_argument("--workload", default="diffuse", choices=["diffuse", "large"])
+    ap.add_argument("--workload", default="diffuse", choices=["diffuse", "large", "c1"])
     ap.add_argument("--build", default="host", choices=["host", "device"],
                     help="BLAS builder: host SAH (default) or the device radix-tree builder")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
@@ -161,6 +169,9 @@ def main():
     if args.workload == "diffuse":
         w = args.width or 1024
         desc = c2_desc(rtx, w, w)
+    elif args.workload == "c1":
+        w = args.width or 128
+        desc = c1_desc(rtx, w, w)
     else:
         w = args.width or 7680
         desc = c5_desc(rtx, w, w * 9 // 16)
@@ -194,6 +205,12 @@ def main():
         train_rays = training_rays_diffuse(scene, primary, hits)
         workload = (f"1,024 instances x 1,024-tri displaced spheres ({scene.n_triangles} tris), {w}x{w} primary + "
                     f"1 cosine diffuse bounce per hit ({len(segments[0])} rays/step)")
+    elif args.workload == "c1":
+        segments = [primary]
+        train_rays = primary
+        log(f"batch: {len(primary)} primary rays ({int(hits['hit'].sum())} hits)")
+        workload = (f"displaced UV sphere of 16,384 tris over a 2-tri ground quad ({scene.n_triangles} tris), "
+                    f"{w}x{w} primary rays ({len(primary)} rays/step)")
     else:
         # camera rays, then three diffuse bounces from each segment's hits; the
         # training traces come from an earlier frame (other bounce seeds)
@@ -219,7 +236,7 @@ def main():
     del hits
     n = sum(len(x) for x in segments)
     dev = [(ctx.to_device(x), len(x), ctx.buffer(len(x) * rtx.HIT_DTYPE.itemsize)) for x in segments]
-    batch = np.concatenate(segments) if args.workload == "diffuse" else None
+    batch = np.concatenate(segments) if args.workload != "large" else None
 
     modes = [m for m in args.modes.split(",") if m in MODES]
     if args.mode not in modes:

@@ -969,6 +969,346 @@ __global__ __launch_bounds__(kScanBlock) void scan_add(uint64_t* out, uint32_t n
     if (base + k < n) out[base + k] += add;
 }
 
+// ---- cooperative plain-order traversal for small batches -------------------
+//
+// One ray per group of kWideLanes lanes.  Every lane of a group holds the same
+// ray state and takes the same branches; an internal node's six child boxes
+// are tested by lanes 0-5 of the group at once, so a traversal step issues a
+// fraction of the instructions of trace_kernel, whose lanes each test all six.
+// A batch too small to fill the GPU is bound by the step latency of its
+// longest rays, which this shortens.  The per-ray sequence of popped entries,
+// and so every hit record and counter, is that of trace_kernel<0, false, K, 0>:
+// the stack is one LDS array of `cap` = K + spill_cap entries per ray, and a
+// push fails exactly when the per-lane kernel's LDS ring and spill area are
+// both full.
+constexpr uint32_t kWideLanes = 8;
+constexpr uint32_t kWideCap = 144;  // K + spill_cap for K <= 16
+constexpr uint32_t kWideBlock = 256;
+
+// Byte `base + j` (j in 0..5) of a node held in registers, for a lane-varying j:
+// it lies in word base / 4 or the next one.
+__device__ __forceinline__ uint32_t byte_at(const Words16& n, int base, uint32_t j) {
+  const uint32_t b = (uint32_t)base + j;
+  const uint32_t w = (b >> 2) == (uint32_t)(base >> 2) ? n.w[base >> 2] : n.w[(base >> 2) + 1];
+  return (w >> ((b & 3u) * 8u)) & 0xffu;
+}
+
+__global__ __launch_bounds__(kWideBlock) void trace_wide(TraceParams p, uint32_t cap) {
+  __shared__ uint64_t stk[kWideBlock / kWideLanes][kWideCap];
+  const uint32_t tid = threadIdx.x;
+  const uint32_t lane = tid & 63u;
+  const uint32_t j = tid & (kWideLanes - 1);
+  const uint32_t gbase = lane & ~(kWideLanes - 1);  // the group's first lane in the wave
+  lds_u64* st = (lds_u64*)&stk[tid / kWideLanes][0];
+  const uint8_t* __restrict__ A = p.arena;
+  const uint64_t bytes = p.arena_bytes;
+  const uint64_t tlas = p.tlas;
+
+  bool active = false, exhausted = false, pending = false;
+  uint32_t depth = 0, err = 0, local_id = 0;
+  F3 ro{}, rd{}, ridir{};
+  float rtmin = 0.f, rtmax = 0.f;
+  uint32_t rflags = 0;
+  float min_thit = 0.f, min_thit_obj = 0.f;
+  uint32_t closest_leaf = kNoSlot, closest_inst = kNoSlot;
+  ObjRay cur{};
+  uint32_t cur_inst = kNoSlot;
+  bool ray_ok = false, cur_ok = false;
+  uint32_t n_rec = 0, n_nodes = 0, fetch_bytes = 0, n_any = 0, max_level = 0;
+  bool has_e = false;
+  uint64_t e = 0;
+
+  auto emit = [&](uint32_t size, uint32_t type) {
+    if (type != RTX_TX_BVH_PRIMITIVE_LEAF_DESCRIPTOR) fetch_bytes += size;
+    n_rec++;
+  };
+  // lane 0 of the group writes; the group's later pops read it back in order
+  auto push = [&](uint64_t v) {
+    if (depth >= cap) {
+      err |= kErrStack;
+      return;
+    }
+    if (j == 0) st[depth] = v;
+    depth++;
+  };
+  auto pop_next = [&]() -> bool {
+    if (depth == 0) return false;
+    e = st[--depth];
+    if (n_rec > kMaxStepsPerRay) err |= kErrSteps;
+    return true;
+  };
+  // the hit record (as trace_kernel writes it): lanes 0-3 store one 16-byte word each
+  auto write_hit = [&]() {
+    uint4 o0 = make_uint4(0, 0, 0, 0), o1 = make_uint4(0, 0, 0, 0), o2 = make_uint4(0, 0, 0, 0);
+    if (err == 0 && min_thit < rtmax && closest_leaf != kNoSlot) {
+      const Words16 q = load64B(A + ((uint64_t)closest_leaf << 6));
+      const uint32_t inst_id = *reinterpret_cast<const uint32_t*>(A + ((uint64_t)closest_inst << 6) + 72);
+      const F3 v0 = mk3(f_of(q, 4), f_of(q, 5), f_of(q, 6));
+      const F3 v1 = mk3(f_of(q, 7), f_of(q, 8), f_of(q, 9));
+      const F3 v2 = mk3(f_of(q, 10), f_of(q, 11), f_of(q, 12));
+      const F3 ip = add3(ro, mk3(rd.x * min_thit, rd.y * min_thit, rd.z * min_thit));
+      const uint4* iq = reinterpret_cast<const uint4*>(A + ((uint64_t)closest_inst << 6) + 16);
+      const uint4 ia = iq[0], ib = iq[1], ic = iq[2];
+      const float im[12] = {__uint_as_float(ia.x), __uint_as_float(ia.y), __uint_as_float(ia.z), __uint_as_float(ia.w),
+                            __uint_as_float(ib.x), __uint_as_float(ib.y), __uint_as_float(ib.z), __uint_as_float(ib.w),
+                            __uint_as_float(ic.x), __uint_as_float(ic.y), __uint_as_float(ic.z), __uint_as_float(ic.w)};
+      const ObjRay hr = transform_ray(im, ro, rd, rtmin, rtmax);
+      const F3 op = add3(hr.o, mk3(hr.d.x * min_thit_obj, hr.d.y * min_thit_obj, hr.d.z * min_thit_obj));
+      const F3 bc = barycentric(op, v0, v1, v2);
+      o0 = make_uint4(1u, q.w[2], q.w[1] & 0x0fffffffu, inst_id);
+      o1 = make_uint4(__float_as_uint(min_thit), __float_as_uint(ip.x), __float_as_uint(ip.y),
+                      __float_as_uint(ip.z));
+      o2 = make_uint4(__float_as_uint(bc.x), __float_as_uint(bc.y), __float_as_uint(bc.z), n_rec);
+    } else {
+      o2.w = n_rec;
+    }
+    uint4* hq = reinterpret_cast<uint4*>(p.hits + local_id);
+    if (j == 0)
+      hq[0] = o0;
+    else if (j == 1)
+      hq[1] = o1;
+    else if (j == 2)
+      hq[2] = o2;
+    else if (j == 3)
+      hq[3] = make_uint4(n_nodes, fetch_bytes, n_any, max_level);
+    if (err && j == 0) {
+      atomicOr(p.error, err);
+      if (p.host_err) *reinterpret_cast<volatile uint32_t*>(p.host_err) = 1u;
+    }
+  };
+
+  uint4 tl_h0 = make_uint4(0, 0, 0, 0), tl_h1 = make_uint4(0, 0, 0, 0);
+  if (tlas + 64 <= bytes) {
+    const uint4* hq = reinterpret_cast<const uint4*>(A + tlas);
+    tl_h0 = hq[0];
+    tl_h1 = hq[1];
+  }
+  const uint32_t groups = 64u / kWideLanes;
+  while (true) {
+    // ---- refill idle groups from the global ray counter (wave converged) ----
+    const uint64_t idle = __ballot(j == 0 && !active);
+    const uint32_t nidle = (uint32_t)__popcll(idle);
+    if (!exhausted && nidle > 0 && (nidle == groups || nidle * kWideLanes >= p.refill)) {
+      if (pending) {
+        write_hit();
+        pending = false;
+      }
+      const uint32_t leader = (uint32_t)__ffsll((long long)idle) - 1u;
+      uint32_t base = 0;
+      if (lane == leader) base = atomicAdd(p.ray_counter, nidle);
+      base = __shfl(base, (int)leader, 64);
+      if ((uint64_t)base + nidle >= p.n) exhausted = true;
+      if (!active) {
+        const uint32_t r = base + (uint32_t)__popcll(idle & ((1ull << gbase) - 1ull));
+        if (r < p.n) {
+          active = true;
+          local_id = r;
+          const uint4* rq = reinterpret_cast<const uint4*>(p.rays + r);
+          const uint4 r0 = rq[0], r1 = rq[1], r2 = rq[2];
+          ro = mk3(__uint_as_float(r0.x), __uint_as_float(r0.y), __uint_as_float(r0.z));
+          rtmin = __uint_as_float(r0.w);
+          rd = mk3(__uint_as_float(r1.x), __uint_as_float(r1.y), __uint_as_float(r1.z));
+          rtmax = __uint_as_float(r1.w);
+          rflags = r2.x;
+          ridir = calc_idir(rd);
+          ray_ok = slab_inputs_ok(ro, ridir, rtmin, rtmax);
+          min_thit = rtmax;
+          min_thit_obj = 0.f;
+          closest_leaf = kNoSlot;
+          closest_inst = kNoSlot;
+          cur_inst = kNoSlot;
+          n_rec = n_nodes = fetch_bytes = n_any = 0;
+          max_level = 1;
+          depth = 0;
+          err = 0;
+          has_e = false;
+          if (tlas + 64 > bytes) {
+            err |= kErrRange;
+          } else {
+            emit(64, RTX_TX_BVH_STRUCTURE);
+            const uint64_t root = tlas + ((uint64_t)tl_h0.x | ((uint64_t)tl_h0.y << 32));
+            const F3 lo = mk3(__uint_as_float(tl_h0.z), __uint_as_float(tl_h0.w), __uint_as_float(tl_h1.x));
+            const F3 hi = mk3(__uint_as_float(tl_h1.y), __uint_as_float(tl_h1.z), __uint_as_float(tl_h1.w));
+            float th;
+            if (ray_box(lo, hi, ridir, ro, rtmin, rtmax, th)) {
+              if ((root & 63) || root + 64 > bytes)
+                err |= kErrRange;
+              else
+                push(make_entry((uint32_t)(root >> 6), kFlTop, 1, 0));
+            }
+          }
+        }
+      }
+    }
+    if (__ballot(active) == 0) {
+      if (exhausted) break;
+      continue;
+    }
+
+    // ---- pop the group's next entry, or complete the ray ---------------------
+    if (active && has_e && err != 0) has_e = false;
+    if (active && !has_e) {
+      if (err == 0 && pop_next()) {
+        has_e = true;
+      } else {
+        pending = true;
+        active = false;
+      }
+    }
+    const bool step = has_e && err == 0;
+    const uint32_t slot = e_slot(e);
+    const uint32_t flags = e_flags(e);
+    const uint32_t level = e_level(e);
+    const uint64_t off = (uint64_t)slot << 6;
+    bool inner = false;
+    if (step) {
+      has_e = false;
+      if (!(flags & kFlLeaf)) {
+        inner = true;
+      } else if (flags & kFlTop) {
+        // ---- instance leaf: the BLAS root is the group's next entry ----------
+        if (off + 128 > bytes) {
+          err |= kErrRange;
+        } else {
+          const uint4* q4 = reinterpret_cast<const uint4*>(A + off + 16);
+          const uint4 a = q4[0], b = q4[1], c = q4[2], d = q4[3];
+          const float m[12] = {__uint_as_float(a.x), __uint_as_float(a.y), __uint_as_float(a.z), __uint_as_float(a.w),
+                               __uint_as_float(b.x), __uint_as_float(b.y), __uint_as_float(b.z), __uint_as_float(b.w),
+                               __uint_as_float(c.x), __uint_as_float(c.y), __uint_as_float(c.z), __uint_as_float(c.w)};
+          const uint64_t bvh = (uint64_t)d.x | ((uint64_t)d.y << 32);
+          emit(128, RTX_TX_BVH_INSTANCE_LEAF);
+          n_nodes++;
+          const uint64_t blas = off + bvh;
+          if ((blas & 63) || blas + 64 > bytes) {
+            err |= kErrRange;
+          } else {
+            const uint2 ro2 = *reinterpret_cast<const uint2*>(A + blas);
+            emit(64, RTX_TX_BVH_STRUCTURE);
+            cur = transform_ray(m, ro, rd, rtmin, rtmax);
+            cur_ok = slab_inputs_ok(cur.o, cur.idir, cur.tmin, cur.tmax);
+            cur_inst = slot;
+            if (slot >= (1u << 22)) err |= kErrInstSlot;
+            const uint64_t root = blas + ((uint64_t)ro2.x | ((uint64_t)ro2.y << 32));
+            if ((root & 63) || root + 64 > bytes) {
+              err |= kErrRange;
+            } else {
+              if (level > max_level) max_level = level;
+              if (err == 0) {
+                e = make_entry((uint32_t)(root >> 6), 0u, level, slot);
+                has_e = true;
+              } else {
+                push(make_entry((uint32_t)(root >> 6), 0u, level, slot));  // as trace_kernel does
+              }
+            }
+          }
+        }
+      } else {
+        // ---- primitive leaf ------------------------------------------------
+        if (off + 64 > bytes) {
+          err |= kErrRange;
+        } else {
+          emit(8, RTX_TX_BVH_PRIMITIVE_LEAF_DESCRIPTOR);
+          const Words16 q = load64B(A + off);
+          if (((q.w[1] >> 29) & 1u) == 0u) {
+            if ((q.w[3] & 0x1ffffu) != 0u) err |= kErrType;
+            const F3 v0 = mk3(f_of(q, 4), f_of(q, 5), f_of(q, 6));
+            const F3 v1 = mk3(f_of(q, 7), f_of(q, 8), f_of(q, 9));
+            const F3 v2 = mk3(f_of(q, 10), f_of(q, 11), f_of(q, 12));
+            float th = 0.f;
+            const bool h = mt_test(v0, v1, v2, cur.o, cur.d, th);
+            const float wt = th / cur.tmult;
+            const bool accept = h && rtmin <= wt && wt <= rtmax;
+            if (accept && (rflags & RTX_RAY_FLAG_OPAQUE) && wt < min_thit) min_thit = th / cur.tmult;
+            if (accept) {
+              min_thit_obj = th;
+              closest_leaf = slot;
+              closest_inst = cur_inst;
+              emit(64, RTX_TX_BVH_QUAD_LEAF_HIT);
+              n_nodes++;
+              if (!(rflags & RTX_RAY_FLAG_OPAQUE)) n_any++;
+              if (rflags & RTX_RAY_FLAG_TERMINATE_ON_FIRST_HIT) depth = 0;
+            } else {
+              emit(64, RTX_TX_BVH_QUAD_LEAF);
+              n_nodes++;
+            }
+          } else {
+            emit(64, RTX_TX_BVH_PROCEDURAL_LEAF);
+            n_nodes++;
+          }
+        }
+      }
+    }
+    if (inner) {
+      // ---- internal node: lane j of the group tests child j ------------------
+      if (off + 64 > bytes) {
+        err |= kErrRange;
+      } else {
+        const Words16 nd = load64B(A + off);
+        emit(64, RTX_TX_BVH_INTERNAL_NODE);
+        n_nodes++;
+        const bool top = (flags & kFlTop) != 0;
+        const int ex = (int)(int8_t)byte_of(nd, 18);
+        const int ey = (int)(int8_t)byte_of(nd, 19);
+        const int ez = (int)(int8_t)byte_of(nd, 20);
+        const F3 o = top ? ro : cur.o;
+        const F3 idir = top ? ridir : cur.idir;
+        const float tmin = top ? rtmin : cur.tmin;
+        const float tmax = top ? rtmax : cur.tmax;
+        const float cull = top ? min_thit : min_thit * cur.tmult;
+        const bool fast = (top ? ray_ok : cur_ok) && !__builtin_isnan(f_of(nd, 0) + f_of(nd, 1) + f_of(nd, 2));
+        bool h = false;
+        if (j < 6u && (byte_at(nd, 22, j) & 3u) != 0u) {
+          const F3 sc = child_scales(ex, ey, ez);
+          const float ox = f_of(nd, 0), oy = f_of(nd, 1), oz = f_of(nd, 2);
+          F3 lo, hi;
+          lo.x = __builtin_fmaf((float)byte_at(nd, 28, j), sc.x, ox);
+          lo.y = __builtin_fmaf((float)byte_at(nd, 40, j), sc.y, oy);
+          lo.z = __builtin_fmaf((float)byte_at(nd, 52, j), sc.z, oz);
+          hi.x = __builtin_fmaf((float)byte_at(nd, 34, j), sc.x, ox);
+          hi.y = __builtin_fmaf((float)byte_at(nd, 46, j), sc.y, oy);
+          hi.z = __builtin_fmaf((float)byte_at(nd, 58, j), sc.z, oz);
+          float th;
+          h = fast ? ray_box_fast(lo, hi, idir, o, tmin, tmax, th) : ray_box(lo, hi, idir, o, tmin, tmax, th);
+          if (h && th >= cull) h = false;
+        }
+        const uint32_t hitmask = (uint32_t)(__ballot(h) >> gbase) & 0x3fu;
+        if (hitmask) {
+          const uint32_t lvl = level + 1;
+          if (lvl > max_level) max_level = lvl;
+          const uint32_t inst = top ? 0u : e_inst(e);
+          const uint32_t base_flags = top ? kFlTop : 0u;
+          int deferred = -1;
+          uint32_t deferred_cs = 0;
+          uint64_t child = off + (uint64_t)((int64_t)(int32_t)nd.w[3] * 64);
+#pragma unroll
+          for (int i = 0; i < 6; ++i) {
+            const uint32_t info = byte_of(nd, 22 + i) & 0x3fu;
+            const uint64_t c = child;
+            child += (uint64_t)(info & 3u) * 64;
+            if (!(hitmask & (1u << i))) continue;
+            const uint32_t ctype = info >> 2;
+            const bool leaf = ctype != 0;
+            if (top && leaf && ctype != 1u) err |= kErrType;
+            if ((c & 63) || c + 64 > bytes) {
+              err |= kErrRange;
+              continue;
+            }
+            const uint32_t cs = (uint32_t)(c >> 6);
+            if (!leaf && deferred < 0) {
+              deferred = i;
+              deferred_cs = cs;
+              continue;
+            }
+            push(make_entry(cs, base_flags | (leaf ? kFlLeaf : 0u), lvl, inst));
+          }
+          if (deferred >= 0) push(make_entry(deferred_cs, base_flags, lvl, inst));
+        }
+      }
+    }
+  }
+  if (pending) write_hit();
+}
+
 using TraceFn = void (*)(TraceParams);
 
 template <int MODE, bool REC>
@@ -1044,6 +1384,20 @@ hipError_t launch_fill_zero(uint32_t* dst, uint32_t words, hipStream_t s) {
 }
 
 uint32_t trace_block_size(uint32_t stage_bytes) { return stage_bytes ? 64u : 256u; }
+
+hipError_t launch_trace_wide(const TraceParams& p, uint32_t cap, uint32_t grid, hipStream_t s) {
+  if (cap > kWideCap) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(trace_wide, dim3(grid), dim3(kWideBlock), 0, s, p, cap);
+  return hipGetLastError();
+}
+
+uint32_t trace_wide_blocks_per_cu() {
+  int nb = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, reinterpret_cast<const void*>(trace_wide), (int)kWideBlock,
+                                                   0) != hipSuccess)
+    return 0;
+  return nb > 0 ? (uint32_t)nb : 0u;
+}
 
 hipError_t launch_trace(const TraceParams& p, uint32_t mode, bool records, uint32_t lds_stack, uint32_t stage_bytes,
                         uint32_t grid, hipStream_t s) {

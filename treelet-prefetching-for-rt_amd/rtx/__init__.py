@@ -60,7 +60,8 @@ class _TraceOptions(ctypes.Structure):
     _fields_ = [("mode", ctypes.c_uint32), ("prefetch", ctypes.c_uint32), ("threshold", ctypes.c_float),
                 ("emit_records", ctypes.c_uint32), ("waves_per_cu", ctypes.c_uint32),
                 ("lds_stack", ctypes.c_uint32), ("stage_bytes", ctypes.c_uint32), ("refill", ctypes.c_uint32),
-                ("slot_bytes", ctypes.c_uint32), ("min_score", ctypes.c_float), ("pred_fraction", ctypes.c_float)]
+                ("slot_bytes", ctypes.c_uint32), ("min_score", ctypes.c_float), ("pred_fraction", ctypes.c_float),
+                ("lanes_per_ray", ctypes.c_uint32)]
 
 
 class _PrefetchStats(ctypes.Structure):
@@ -581,10 +582,10 @@ class Context:
 
     def trace_device(self, rays_buf, n, hits_buf, mode=MODE_PLAIN, prefetch=PREFETCH_NONE, threshold=0.5,
                      records_buf=None, record_capacity=0, offsets_buf=None, waves_per_cu=0, lds_stack=0,
-                     stage_bytes=0, refill=0, slot_bytes=0, min_score=0.0, pred_fraction=0.0):
+                     stage_bytes=0, refill=0, slot_bytes=0, min_score=0.0, pred_fraction=0.0, lanes_per_ray=0):
         """rtx_trace on device buffers; returns the rtx_status (0 = ok)."""
         o = _TraceOptions(mode, prefetch, threshold, 1 if offsets_buf is not None else 0, waves_per_cu, lds_stack,
-                          stage_bytes, refill, slot_bytes, min_score, pred_fraction)
+                          stage_bytes, refill, slot_bytes, min_score, pred_fraction, lanes_per_ray)
         return _lib.rtx_trace(self._h, rays_buf.ptr, n, ctypes.byref(o), hits_buf.ptr,
                               records_buf.ptr if records_buf is not None else None, record_capacity,
                               offsets_buf.ptr if offsets_buf is not None else None)
