@@ -21,7 +21,6 @@
 using namespace rtxk;
 
 // bytes of the control block cleared before and read after a traversal
-static constexpr size_t kCtlBytes = 16 * sizeof(uint32_t) + 16 * sizeof(unsigned long long);
 static constexpr uint32_t kSlots = 64;
 
 struct rtx_ctx {
@@ -47,7 +46,6 @@ struct rtx_ctx {
   // word 16 (16 x u64).  Cleared with one memset, read back with one copy into
   // pinned host memory.
   uint32_t* counters = nullptr;
-  uint32_t* ctl_host = nullptr;
   unsigned long long* stats = nullptr;
   // rtx_trace's rotating control slots (launch k uses slot k % kSlots; all are
   // cleared together when the index wraps) and the mapped host word its
@@ -241,8 +239,6 @@ rtx_status rtx_create(int device, rtx_ctx** out) {
   if (hipEventCreate(&c->evk0) != hipSuccess || hipEventCreate(&c->evk1) != hipSuccess) return fail(RTX_ERR_HIP);
   if (hipMalloc(&c->counters, 64 * sizeof(uint32_t)) != hipSuccess) return fail(RTX_ERR_OUT_OF_MEMORY);
   c->stats = reinterpret_cast<unsigned long long*>(c->counters + 16);
-  if (hipHostMalloc(&c->ctl_host, 64 * sizeof(uint32_t), hipHostMallocDefault) != hipSuccess)
-    return fail(RTX_ERR_OUT_OF_MEMORY);
   if (hipMemset(c->counters, 0, 64 * sizeof(uint32_t)) != hipSuccess) return fail(RTX_ERR_HIP);
   if (hipMalloc(&c->slots, kSlots * kSlotWords * sizeof(uint32_t)) != hipSuccess) return fail(RTX_ERR_OUT_OF_MEMORY);
   if (hipMemset(c->slots, 0, kSlots * kSlotWords * sizeof(uint32_t)) != hipSuccess) return fail(RTX_ERR_HIP);
@@ -263,7 +259,6 @@ rtx_status rtx_destroy(rtx_ctx* c) {
   drop_table(c);
   if (c->spill) hipFree(c->spill);
   if (c->counters) hipFree(c->counters);
-  if (c->ctl_host) hipHostFree(c->ctl_host);
   if (c->slots) hipFree(c->slots);
   if (c->err_host) hipHostFree(c->err_host);
   if (c->scan_tmp) hipFree(c->scan_tmp);
