@@ -146,7 +146,7 @@ typedef struct rtx_trace_options {
   float pred_fraction;   /* LEARNED: share of a predicted treelet's image staged, root first
                             (0 = default 0.25; 1 = the whole image) */
   uint32_t lanes_per_ray; /* 1: one lane per ray; 8: eight lanes per ray, testing a node's six
-                             children at once (PLAIN order, no records or staging), for batches
+                             children at once (either order, no records or staging), for batches
                              too small to fill the GPU; 0 = 8 when those hold and the batch has
                              at most 128 rays per CU, else 1.  Results are identical.
                              rtx_trace only; batches ignore it. */

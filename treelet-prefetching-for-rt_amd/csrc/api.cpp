@@ -132,14 +132,15 @@ rtx_status ensure_spill(rtx_ctx* c, size_t entries) {
   return RTX_OK;
 }
 
-// occupancy-cache key of the eight-lanes-per-ray kernel (no traversal mode has it)
-constexpr uint32_t kWideOccupancyKey = 0xffffu;
+// occupancy-cache keys of the eight-lanes-per-ray kernels: this plus the
+// traversal mode (no traversal mode reaches it)
+constexpr uint32_t kWideOccupancyKey = 0xff00u;
 
 uint32_t cached_blocks_per_cu(rtx_ctx* c, uint32_t mode, uint32_t lds_stack, uint32_t stage) {
   const uint64_t key = ((uint64_t)mode << 48) | ((uint64_t)lds_stack << 32) | stage;
   auto it = c->occupancy.find(key);
   if (it != c->occupancy.end()) return it->second;
-  const uint32_t b = mode == kWideOccupancyKey ? trace_wide_blocks_per_cu()
+  const uint32_t b = mode >= kWideOccupancyKey ? trace_wide_blocks_per_cu(mode - kWideOccupancyKey)
                                                : trace_blocks_per_cu(mode, false, lds_stack, stage);
   c->occupancy[key] = b;
   return b;
@@ -482,20 +483,20 @@ rtx_status rtx_trace(rtx_ctx* c, const rtx_ray* rays, uint32_t n, const rtx_trac
     if (!(opts->pred_fraction >= 0.f && opts->pred_fraction <= 1.f)) return RTX_ERR_INVALID_ARGUMENT;
     lds_stack = 8;
   }
-  // Eight lanes per ray (plain order, no records or staging) for batches of at
-  // most 128 rays per CU, which leave most of the GPU idle: their time is the
-  // step latency of the longest rays, and a group tests a node's six children
-  // at once.
+  // Eight lanes per ray (no records or staging) for batches of at most 128
+  // rays per CU, which leave most of the GPU idle: their time is the step
+  // latency of the longest rays, and a group tests a node's six children at
+  // once.
   if (opts->lanes_per_ray != 0 && opts->lanes_per_ray != 1 && opts->lanes_per_ray != kWideRayLanes)
     return RTX_ERR_INVALID_ARGUMENT;
-  const bool wide_ok = opts->mode == RTX_MODE_PLAIN && !opts->emit_records && stage == 0;
+  const bool wide_ok = !opts->emit_records && stage == 0;
   if (opts->lanes_per_ray == kWideRayLanes && !wide_ok) return RTX_ERR_INVALID_ARGUMENT;
   const bool wide = wide_ok && (opts->lanes_per_ray == kWideRayLanes ||
                                 (opts->lanes_per_ray == 0 && (uint64_t)n <= 128ull * (uint64_t)c->n_cus));
   const uint32_t bs = trace_block_size(stage);
   // persistent blocks: by default as many as fit on a CU at once
   uint32_t blocks_per_cu = opts->waves_per_cu ? std::max(1u, opts->waves_per_cu * 64 / bs)
-                           : wide             ? cached_blocks_per_cu(c, kWideOccupancyKey, 0, 0)
+                           : wide             ? cached_blocks_per_cu(c, kWideOccupancyKey + opts->mode, 0, 0)
                                               : cached_blocks_per_cu(c, opts->mode, lds_stack, stage);
   if (blocks_per_cu == 0) blocks_per_cu = 16 * 64 / bs;
   uint32_t grid = (uint32_t)c->n_cus * blocks_per_cu;
@@ -572,7 +573,7 @@ rtx_status rtx_trace(rtx_ctx* c, const rtx_ray* rays, uint32_t n, const rtx_trac
   if (st != RTX_OK) return st;
   HIP_TRY(hipEventRecord(c->evk0, c->stream));
   if (wide)
-    HIP_TRY(launch_trace_wide(p, lds_stack + spill_cap, grid, c->stream));
+    HIP_TRY(launch_trace_wide(p, opts->mode, lds_stack + spill_cap, grid, c->stream));
   else
     HIP_TRY(launch_trace(p, opts->mode, false, lds_stack, stage, grid, c->stream));
   HIP_TRY(hipEventRecord(c->evk1, c->stream));

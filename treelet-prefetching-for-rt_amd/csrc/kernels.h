@@ -89,11 +89,12 @@ hipError_t launch_trace_batch(const TraceParams& p, uint32_t mode, uint32_t lds_
 // 0 when the configuration has no kernel.
 uint32_t trace_blocks_per_cu(uint32_t mode, bool records, uint32_t lds_stack, uint32_t stage_bytes);
 uint32_t trace_batch_blocks_per_cu(uint32_t mode, uint32_t lds_stack);
-// Cooperative plain-order kernel for small batches (eight lanes per ray, no
-// records or staging): `cap` stack entries per ray (at most 144), 256-lane blocks.
+// Cooperative kernel for small batches (eight lanes per ray, no records or
+// staging), plain or treelet order: `cap` entries per list and ray (at most
+// 144), 256-lane blocks.
 constexpr uint32_t kWideRayLanes = 8;
-hipError_t launch_trace_wide(const TraceParams& p, uint32_t cap, uint32_t grid, hipStream_t s);
-uint32_t trace_wide_blocks_per_cu();
+hipError_t launch_trace_wide(const TraceParams& p, uint32_t mode, uint32_t cap, uint32_t grid, hipStream_t s);
+uint32_t trace_wide_blocks_per_cu(uint32_t mode);
 // Clears `words` 32-bit words with a kernel (usable inside graph capture).
 hipError_t launch_fill_zero(uint32_t* dst, uint32_t words, hipStream_t s);
 hipError_t launch_scan_records(const rtx_hit* hits, uint32_t n, uint64_t* offsets, uint64_t* block_tmp,

@@ -977,10 +977,10 @@ __global__ __launch_bounds__(kScanBlock) void scan_add(uint64_t* out, uint32_t n
 // fraction of the instructions of trace_kernel, whose lanes each test all six.
 // A batch too small to fill the GPU is bound by the step latency of its
 // longest rays, which this shortens.  The per-ray sequence of popped entries,
-// and so every hit record and counter, is that of trace_kernel<0, false, K, 0>:
-// the stack is one LDS array of `cap` = K + spill_cap entries per ray, and a
-// push fails exactly when the per-lane kernel's LDS ring and spill area are
-// both full.
+// and so every hit record and counter, is that of trace_kernel<MODE, false, K,
+// 0>: each list (one in plain order, two in treelet order) is an LDS array of
+// `cap` = K + spill_cap entries per ray, and a push fails exactly when the
+// per-lane kernel's LDS ring and spill area for that list are both full.
 constexpr uint32_t kWideLanes = 8;
 constexpr uint32_t kWideCap = 144;  // K + spill_cap for K <= 16
 constexpr uint32_t kWideBlock = 256;
@@ -993,19 +993,25 @@ __device__ __forceinline__ uint32_t byte_at(const Words16& n, int base, uint32_t
   return (w >> ((b & 3u) * 8u)) & 0xffu;
 }
 
+template <int MODE>
 __global__ __launch_bounds__(kWideBlock) void trace_wide(TraceParams p, uint32_t cap) {
-  __shared__ uint64_t stk[kWideBlock / kWideLanes][kWideCap];
+  constexpr uint32_t NS = MODE == 1 ? 2 : 1;
+  __shared__ uint64_t stk[kWideBlock / kWideLanes][NS * kWideCap];
   const uint32_t tid = threadIdx.x;
   const uint32_t lane = tid & 63u;
   const uint32_t j = tid & (kWideLanes - 1);
   const uint32_t gbase = lane & ~(kWideLanes - 1);  // the group's first lane in the wave
-  lds_u64* st = (lds_u64*)&stk[tid / kWideLanes][0];
+  lds_u64* sta = (lds_u64*)&stk[tid / kWideLanes][0];
+  lds_u64* stb = (lds_u64*)&stk[tid / kWideLanes][(NS - 1) * kWideCap];
   const uint8_t* __restrict__ A = p.arena;
   const uint64_t bytes = p.arena_bytes;
   const uint64_t tlas = p.tlas;
+  const uint32_t* owner = p.owner;
+  const uint32_t first_treelet = p.first_treelet;
 
   bool active = false, exhausted = false, pending = false;
-  uint32_t depth = 0, err = 0, local_id = 0;
+  bool phase2 = false;
+  uint32_t da = 0, db = 0, err = 0, local_id = 0;
   F3 ro{}, rd{}, ridir{};
   float rtmin = 0.f, rtmax = 0.f;
   uint32_t rflags = 0;
@@ -1023,19 +1029,54 @@ __global__ __launch_bounds__(kWideBlock) void trace_wide(TraceParams p, uint32_t
     n_rec++;
   };
   // lane 0 of the group writes; the group's later pops read it back in order
-  auto push = [&](uint64_t v) {
-    if (depth >= cap) {
+  // (the list depths are updated through selects, not a reference to one of
+  // them, which would put both in scratch memory)
+  auto push = [&](uint64_t v, bool to_a) {
+    const bool a = MODE == 0 || to_a;
+    const uint32_t d = a ? da : db;
+    if (d >= cap) {
       err |= kErrStack;
       return;
     }
-    if (j == 0) st[depth] = v;
-    depth++;
+    if (j == 0) *(a ? sta + d : stb + d) = v;
+    da = a ? d + 1 : da;
+    db = a ? db : d + 1;
   };
   auto pop_next = [&]() -> bool {
-    if (depth == 0) return false;
-    e = st[--depth];
+    if (da == 0 && db == 0) return false;
+    const bool a = MODE == 0 || da > 0;
+    // moving to the other list: current_treelet_root := host address of the moved node (:1748-1754)
+    if (!a) phase2 = true;
+    const uint32_t d = (a ? da : db) - 1;
+    e = *(a ? sta + d : stb + d);
+    da = a ? d : da;
+    db = a ? db : d;
     if (n_rec > kMaxStepsPerRay) err |= kErrSteps;
     return true;
+  };
+  auto goes_current = [&](uint32_t slot) -> bool {
+    if (MODE == 0) return true;
+    if (phase2) return false;
+    return owner[slot] == first_treelet;
+  };
+  // first-treelet child bits of a node slot (one byte per slot after the owner words)
+  auto first_bits = [&](uint32_t slot) -> uint32_t {
+    return reinterpret_cast<const uint8_t*>(owner + (bytes >> 6))[slot];
+  };
+  auto load_instance = [&](uint32_t inst) {
+    const uint64_t ioff = (uint64_t)inst << 6;
+    if (ioff + 128 > bytes) {
+      err |= kErrRange;
+      return;
+    }
+    const uint4* q = reinterpret_cast<const uint4*>(A + ioff + 16);
+    const uint4 a = q[0], b = q[1], c = q[2];
+    const float m[12] = {__uint_as_float(a.x), __uint_as_float(a.y), __uint_as_float(a.z), __uint_as_float(a.w),
+                         __uint_as_float(b.x), __uint_as_float(b.y), __uint_as_float(b.z), __uint_as_float(b.w),
+                         __uint_as_float(c.x), __uint_as_float(c.y), __uint_as_float(c.z), __uint_as_float(c.w)};
+    cur = transform_ray(m, ro, rd, rtmin, rtmax);
+    cur_ok = slab_inputs_ok(cur.o, cur.idir, cur.tmin, cur.tmax);
+    cur_inst = inst;
   };
   // the hit record (as trace_kernel writes it): lanes 0-3 store one 16-byte word each
   auto write_hit = [&]() {
@@ -1119,7 +1160,8 @@ __global__ __launch_bounds__(kWideBlock) void trace_wide(TraceParams p, uint32_t
           cur_inst = kNoSlot;
           n_rec = n_nodes = fetch_bytes = n_any = 0;
           max_level = 1;
-          depth = 0;
+          da = db = 0;
+          phase2 = false;
           err = 0;
           has_e = false;
           if (tlas + 64 > bytes) {
@@ -1131,10 +1173,12 @@ __global__ __launch_bounds__(kWideBlock) void trace_wide(TraceParams p, uint32_t
             const F3 hi = mk3(__uint_as_float(tl_h1.y), __uint_as_float(tl_h1.z), __uint_as_float(tl_h1.w));
             float th;
             if (ray_box(lo, hi, ridir, ro, rtmin, rtmax, th)) {
-              if ((root & 63) || root + 64 > bytes)
+              if ((root & 63) || root + 64 > bytes) {
                 err |= kErrRange;
-              else
-                push(make_entry((uint32_t)(root >> 6), kFlTop, 1, 0));
+              } else {
+                const uint32_t rs = (uint32_t)(root >> 6);
+                push(make_entry(rs, kFlTop, 1, 0), goes_current(rs));
+              }
             }
           }
         }
@@ -1163,7 +1207,10 @@ __global__ __launch_bounds__(kWideBlock) void trace_wide(TraceParams p, uint32_t
     bool inner = false;
     if (step) {
       has_e = false;
-      if (!(flags & kFlLeaf)) {
+      if (MODE == 1 && !(flags & kFlTop) && e_inst(e) != cur_inst) load_instance(e_inst(e));
+      if (err != 0) {
+        // an instance reload out of the arena completes the ray next iteration
+      } else if (!(flags & kFlLeaf)) {
         inner = true;
       } else if (flags & kFlTop) {
         // ---- instance leaf: the BLAS root is the group's next entry ----------
@@ -1193,11 +1240,15 @@ __global__ __launch_bounds__(kWideBlock) void trace_wide(TraceParams p, uint32_t
               err |= kErrRange;
             } else {
               if (level > max_level) max_level = level;
-              if (err == 0) {
+              const bool to_a = MODE == 0 || (!phase2 && (first_bits(slot) & 1u));
+              // the root is the next entry when it goes on top of the list the
+              // next pop takes (see trace_kernel); otherwise it is pushed
+              if (err == 0 && (MODE == 0 || to_a || da == 0)) {
+                if (MODE == 1 && !to_a) phase2 = true;
                 e = make_entry((uint32_t)(root >> 6), 0u, level, slot);
                 has_e = true;
               } else {
-                push(make_entry((uint32_t)(root >> 6), 0u, level, slot));  // as trace_kernel does
+                push(make_entry((uint32_t)(root >> 6), 0u, level, slot), to_a);
               }
             }
           }
@@ -1217,16 +1268,22 @@ __global__ __launch_bounds__(kWideBlock) void trace_wide(TraceParams p, uint32_t
             float th = 0.f;
             const bool h = mt_test(v0, v1, v2, cur.o, cur.d, th);
             const float wt = th / cur.tmult;
-            const bool accept = h && rtmin <= wt && wt <= rtmax;
-            if (accept && (rflags & RTX_RAY_FLAG_OPAQUE) && wt < min_thit) min_thit = th / cur.tmult;
+            bool accept;
+            if (MODE == 0) {
+              accept = h && rtmin <= wt && wt <= rtmax;
+              if (accept && (rflags & RTX_RAY_FLAG_OPAQUE) && wt < min_thit) min_thit = th / cur.tmult;
+            } else {
+              accept = h && rtmin <= wt && wt <= rtmax && wt < min_thit;
+              if (accept) min_thit = th / cur.tmult;
+            }
             if (accept) {
               min_thit_obj = th;
               closest_leaf = slot;
               closest_inst = cur_inst;
               emit(64, RTX_TX_BVH_QUAD_LEAF_HIT);
               n_nodes++;
-              if (!(rflags & RTX_RAY_FLAG_OPAQUE)) n_any++;
-              if (rflags & RTX_RAY_FLAG_TERMINATE_ON_FIRST_HIT) depth = 0;
+              if (MODE == 0 && !(rflags & RTX_RAY_FLAG_OPAQUE)) n_any++;
+              if (rflags & RTX_RAY_FLAG_TERMINATE_ON_FIRST_HIT) da = db = 0;
             } else {
               emit(64, RTX_TX_BVH_QUAD_LEAF);
               n_nodes++;
@@ -1244,6 +1301,8 @@ __global__ __launch_bounds__(kWideBlock) void trace_wide(TraceParams p, uint32_t
         err |= kErrRange;
       } else {
         const Words16 nd = load64B(A + off);
+        // treelet order before the first switch: which children the first treelet owns
+        const uint32_t cur_bits = (MODE == 1 && !phase2) ? first_bits(slot) : 0u;
         emit(64, RTX_TX_BVH_INTERNAL_NODE);
         n_nodes++;
         const bool top = (flags & kFlTop) != 0;
@@ -1294,14 +1353,14 @@ __global__ __launch_bounds__(kWideBlock) void trace_wide(TraceParams p, uint32_t
               continue;
             }
             const uint32_t cs = (uint32_t)(c >> 6);
-            if (!leaf && deferred < 0) {
+            if (MODE == 0 && !leaf && deferred < 0) {
               deferred = i;
               deferred_cs = cs;
               continue;
             }
-            push(make_entry(cs, base_flags | (leaf ? kFlLeaf : 0u), lvl, inst));
+            push(make_entry(cs, base_flags | (leaf ? kFlLeaf : 0u), lvl, inst), MODE == 0 || ((cur_bits >> i) & 1u));
           }
-          if (deferred >= 0) push(make_entry(deferred_cs, base_flags, lvl, inst));
+          if (MODE == 0 && deferred >= 0) push(make_entry(deferred_cs, base_flags, lvl, inst), true);
         }
       }
     }
@@ -1385,17 +1444,20 @@ hipError_t launch_fill_zero(uint32_t* dst, uint32_t words, hipStream_t s) {
 
 uint32_t trace_block_size(uint32_t stage_bytes) { return stage_bytes ? 64u : 256u; }
 
-hipError_t launch_trace_wide(const TraceParams& p, uint32_t cap, uint32_t grid, hipStream_t s) {
+hipError_t launch_trace_wide(const TraceParams& p, uint32_t mode, uint32_t cap, uint32_t grid, hipStream_t s) {
   if (cap > kWideCap) return hipErrorInvalidValue;
-  hipLaunchKernelGGL(trace_wide, dim3(grid), dim3(kWideBlock), 0, s, p, cap);
+  if (mode == RTX_MODE_PLAIN)
+    hipLaunchKernelGGL(trace_wide<0>, dim3(grid), dim3(kWideBlock), 0, s, p, cap);
+  else
+    hipLaunchKernelGGL(trace_wide<1>, dim3(grid), dim3(kWideBlock), 0, s, p, cap);
   return hipGetLastError();
 }
 
-uint32_t trace_wide_blocks_per_cu() {
+uint32_t trace_wide_blocks_per_cu(uint32_t mode) {
   int nb = 0;
-  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, reinterpret_cast<const void*>(trace_wide), (int)kWideBlock,
-                                                   0) != hipSuccess)
-    return 0;
+  const void* f = mode == RTX_MODE_PLAIN ? reinterpret_cast<const void*>(trace_wide<0>)
+                                         : reinterpret_cast<const void*>(trace_wide<1>);
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, f, (int)kWideBlock, 0) != hipSuccess) return 0;
   return nb > 0 ? (uint32_t)nb : 0u;
 }
 
