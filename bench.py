@@ -351,7 +351,10 @@ def main():
             "unit": "GB/s",
             "frac": round(head["node_fetch_gbs"] / HBM_PEAK_GBS, 4),
             "traffic": load_traffic(args.workload, args.mode),
-            "kernel": "trace_kernel",
+            # rtx_trace's default kernel choice: eight lanes per ray for batches of at
+            # most 128 rays per CU without staging (each segment is one launch)
+            "kernel": ("trace_wide" if not MODES[args.mode]["stage"] and
+                       max(len(x) for x in segments) <= 128 * info["n_cus"] else "trace_kernel"),
             "kernel_ms": round(head["kernel_ms"], 4),
             "bytes_per_launch": head["fetch_bytes"],
         },
@@ -367,7 +370,8 @@ def main():
     if not args.no_cpu_baseline:
         log("cpu baseline ...")
         if batch is not None:
-            smp, what = batch, "the whole primary+bounce batch"
+            smp, what = batch, ("the whole primary+bounce batch" if args.workload == "diffuse"
+                                else "the whole primary batch")
         else:
             smp = np.concatenate([x[:: max(1, len(x) // 250000)] for x in segments])
             what = "about 250,000 evenly strided rays of each segment"
