@@ -1336,31 +1336,43 @@ __global__ __launch_bounds__(kWideBlock) void trace_wide(TraceParams p, uint32_t
           if (lvl > max_level) max_level = lvl;
           const uint32_t inst = top ? 0u : e_inst(e);
           const uint32_t base_flags = top ? kFlTop : 0u;
-          int deferred = -1;
-          uint32_t deferred_cs = 0;
-          uint64_t child = off + (uint64_t)((int64_t)(int32_t)nd.w[3] * 64);
+          // Lane j pushes hit child j itself, at the position the per-lane
+          // kernel's push sequence gives it: hit children in slot order (in
+          // plain order the first internal one last), a list's pushes failing
+          // from the one that would exceed its capacity on.
+          uint64_t c = off + (uint64_t)((int64_t)(int32_t)nd.w[3] * 64);
 #pragma unroll
-          for (int i = 0; i < 6; ++i) {
-            const uint32_t info = byte_of(nd, 22 + i) & 0x3fu;
-            const uint64_t c = child;
-            child += (uint64_t)(info & 3u) * 64;
-            if (!(hitmask & (1u << i))) continue;
-            const uint32_t ctype = info >> 2;
-            const bool leaf = ctype != 0;
-            if (top && leaf && ctype != 1u) err |= kErrType;
-            if ((c & 63) || c + 64 > bytes) {
-              err |= kErrRange;
-              continue;
-            }
-            const uint32_t cs = (uint32_t)(c >> 6);
-            if (MODE == 0 && !leaf && deferred < 0) {
-              deferred = i;
-              deferred_cs = cs;
-              continue;
-            }
-            push(make_entry(cs, base_flags | (leaf ? kFlLeaf : 0u), lvl, inst), MODE == 0 || ((cur_bits >> i) & 1u));
+          for (int i = 0; i < 5; ++i)
+            if ((uint32_t)i < j) c += (uint64_t)(byte_of(nd, 22 + i) & 3u) * 64;
+          const uint32_t ctype = j < 6u ? (byte_at(nd, 22, j) & 0x3fu) >> 2 : 0u;
+          const bool leaf = ctype != 0;
+          const bool mine = j < 6u && ((hitmask >> j) & 1u);
+          const bool pushed = mine && !(c & 63) && c + 64 <= bytes;
+          const bool to_a = MODE == 0 || ((cur_bits >> j) & 1u);
+          const uint32_t m_push = (uint32_t)(__ballot(pushed) >> gbase) & 0x3fu;
+          if ((__ballot(mine && top && leaf && ctype != 1u) >> gbase) & 0x3fu) err |= kErrType;
+          if (hitmask & ~m_push) err |= kErrRange;  // hit children outside the arena
+          const uint32_t below = (1u << j) - 1u;
+          uint32_t pos;
+          if (MODE == 0) {
+            const uint32_t m_int = (uint32_t)(__ballot(pushed && !leaf) >> gbase) & 0x3fu;
+            const uint32_t n = (uint32_t)__popc(m_push);
+            const uint32_t d = m_int ? (uint32_t)__builtin_ctz(m_int) : 8u;  // deferred child, 8 = none
+            const uint32_t rest = m_push & ~(d < 8u ? 1u << d : 0u);
+            pos = j == d ? da + n - 1u : da + (uint32_t)__popc(rest & below);
+            da = min(cap, da + n);
+          } else {
+            const uint32_t m_a = (uint32_t)(__ballot(pushed && to_a) >> gbase) & 0x3fu;
+            const uint32_t m_b = m_push & ~m_a;
+            pos = to_a ? da + (uint32_t)__popc(m_a & below) : db + (uint32_t)__popc(m_b & below);
+            da = min(cap, da + (uint32_t)__popc(m_a));
+            db = min(cap, db + (uint32_t)__popc(m_b));
           }
-          if (MODE == 0 && deferred >= 0) push(make_entry(deferred_cs, base_flags, lvl, inst), true);
+          const bool fits = pos < cap;
+          if (pushed && fits)
+            *(to_a ? sta + pos : stb + pos) =
+                make_entry((uint32_t)(c >> 6), base_flags | (leaf ? kFlLeaf : 0u), lvl, inst);
+          if ((__ballot(pushed && !fits) >> gbase) & 0x3fu) err |= kErrStack;
         }
       }
     }
