@@ -351,10 +351,11 @@ def main():
             "unit": "GB/s",
             "frac": round(head["node_fetch_gbs"] / HBM_PEAK_GBS, 4),
             "traffic": load_traffic(args.workload, args.mode),
-            # rtx_trace's default kernel choice: eight lanes per ray for batches of at
-            # most 128 rays per CU without staging (each segment is one launch)
-            "kernel": ("trace_wide" if not MODES[args.mode]["stage"] and
-                       max(len(x) for x in segments) <= 128 * info["n_cus"] else "trace_kernel"),
+            # rtx_trace's default kernel choice: eight lanes per ray without staging for
+            # batches of at most 128 (plain) or 256 (treelet) rays per CU; each segment
+            # is one launch
+            "kernel": ("trace_wide" if not MODES[args.mode]["stage"] and max(len(x) for x in segments) <=
+                       (128 if MODES[args.mode]["order"] == 0 else 256) * info["n_cus"] else "trace_kernel"),
             "kernel_ms": round(head["kernel_ms"], 4),
             "bytes_per_launch": head["fetch_bytes"],
         },

@@ -148,7 +148,8 @@ typedef struct rtx_trace_options {
   uint32_t lanes_per_ray; /* 1: one lane per ray; 8: eight lanes per ray, testing a node's six
                              children at once (either order, no records or staging), for batches
                              too small to fill the GPU; 0 = 8 when those hold and the batch has
-                             at most 128 rays per CU, else 1.  Results are identical.
+                             at most 128 rays per CU (PLAIN) or 256 (TREELET), else 1.  Results
+                             are identical.
                              rtx_trace only; batches ignore it. */
 } rtx_trace_options;
 

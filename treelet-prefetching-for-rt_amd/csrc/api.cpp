@@ -483,16 +483,18 @@ rtx_status rtx_trace(rtx_ctx* c, const rtx_ray* rays, uint32_t n, const rtx_trac
     if (!(opts->pred_fraction >= 0.f && opts->pred_fraction <= 1.f)) return RTX_ERR_INVALID_ARGUMENT;
     lds_stack = 8;
   }
-  // Eight lanes per ray (no records or staging) for batches of at most 128
-  // rays per CU, which leave most of the GPU idle: their time is the step
-  // latency of the longest rays, and a group tests a node's six children at
-  // once.
+  // Eight lanes per ray (no records or staging) for batches that leave most of
+  // the GPU idle, at most 128 rays per CU in plain order and 256 in treelet
+  // order (where the two kernels meet on the 1M-triangle scene): their time is
+  // the step latency of the longest rays, and a group tests a node's six
+  // children at once.
   if (opts->lanes_per_ray != 0 && opts->lanes_per_ray != 1 && opts->lanes_per_ray != kWideRayLanes)
     return RTX_ERR_INVALID_ARGUMENT;
   const bool wide_ok = !opts->emit_records && stage == 0;
   if (opts->lanes_per_ray == kWideRayLanes && !wide_ok) return RTX_ERR_INVALID_ARGUMENT;
   const bool wide = wide_ok && (opts->lanes_per_ray == kWideRayLanes ||
-                                (opts->lanes_per_ray == 0 && (uint64_t)n <= 128ull * (uint64_t)c->n_cus));
+                                (opts->lanes_per_ray == 0 &&
+                                 (uint64_t)n <= (opts->mode == RTX_MODE_PLAIN ? 128ull : 256ull) * c->n_cus));
   const uint32_t bs = trace_block_size(stage);
   // persistent blocks: by default as many as fit on a CU at once
   uint32_t blocks_per_cu = opts->waves_per_cu ? std::max(1u, opts->waves_per_cu * 64 / bs)
