@@ -34,6 +34,10 @@
 // keep their state until the wave's next refill, which writes their hit
 // records together.
 //
+// Small batches (at most 128 rays per CU in plain order, 256 in treelet
+// order) run trace_wide instead: eight lanes per ray, the six child boxes of
+// a node tested at once (see the section before its definition).
+//
 // Treelet staging (STAGE > 0, one wave per workgroup): the wave keeps up to
 // stage/slot treelet images in LDS slots, copied from the treelet-contiguous
 // image buffer with coalesced 16-byte loads, and an LDS hash table from node
@@ -969,7 +973,7 @@ __global__ __launch_bounds__(kScanBlock) void scan_add(uint64_t* out, uint32_t n
     if (base + k < n) out[base + k] += add;
 }
 
-// ---- cooperative plain-order traversal for small batches -------------------
+// ---- cooperative traversal for small batches (both orders) -----------------
 //
 // One ray per group of kWideLanes lanes.  Every lane of a group holds the same
 // ray state and takes the same branches; an internal node's six child boxes
@@ -981,6 +985,7 @@ __global__ __launch_bounds__(kScanBlock) void scan_add(uint64_t* out, uint32_t n
 // 0>: each list (one in plain order, two in treelet order) is an LDS array of
 // `cap` = K + spill_cap entries per ray, and a push fails exactly when the
 // per-lane kernel's LDS ring and spill area for that list are both full.
+// Lane j pushes hit child j itself, at its place in that push sequence.
 constexpr uint32_t kWideLanes = 8;
 constexpr uint32_t kWideCap = 144;  // K + spill_cap for K <= 16
 constexpr uint32_t kWideBlock = 256;
