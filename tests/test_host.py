@@ -166,3 +166,61 @@ def test_secondary_rays(rtx, oracle, c1):
     shadow, src2 = scene.secondary_rays(rays, h, kind=1, seed=3, light=(0, 10, 0))
     assert np.all(shadow["flags"] == (1 | 4 | 8))
     assert np.all(shadow["tmax"] > 0)
+
+
+_CSIZE = {"float": 4, "uint32_t": 4, "int32_t": 4, "uint64_t": 8, "int64_t": 8, "uint16_t": 2, "uint8_t": 1,
+          "size_t": 8, "double": 8}
+
+
+def _c_struct(header, name):
+    """(field name, byte offset, size) of a header struct, laid out with natural
+    alignment as the C compiler does for these types."""
+    text = open(os.path.join(ROOT, "include", header)).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    body = re.search(r"typedef struct %s \{(.*?)\} %s;" % (name, name), text, flags=re.S).group(1)
+    fields, off, align = [], 0, 1
+    for decl in body.split(";"):
+        decl = decl.strip()
+        if not decl:
+            continue
+        m = re.match(r"(?:const\s+)?(\w+)\s*(\**)\s*(.*)", decl)
+        base, ptr, names = m.group(1), m.group(2), m.group(3)
+        for nm in names.split(","):
+            nm = nm.strip()
+            ptr_here = ptr or ("*" if nm.startswith("*") else "")
+            nm = nm.lstrip("*").strip()
+            count = 1
+            arr = re.match(r"(\w+)\[(\d+)\]", nm)
+            if arr:
+                nm, count = arr.group(1), int(arr.group(2))
+            size = 8 if ptr_here else _CSIZE[base]
+            off = (off + size - 1) // size * size
+            fields.append((nm, off, size * count))
+            off += size * count
+            align = max(align, size)
+    return fields, (off + align - 1) // align * align
+
+
+@pytest.mark.parametrize("header,name,mirror", [
+    ("rtx.h", "rtx_trace_options", "_TraceOptions"), ("rtx.h", "rtx_prefetch_stats", "_PrefetchStats"),
+    ("rtx.h", "rtx_treelet_summary", "_TreeletSummary"), ("rtx.h", "rtx_batch_job", "_BatchJob"),
+    ("rtx_scene.h", "rtx_workload_desc", "_WorkloadDesc")])
+def test_ctypes_mirrors_match_the_headers(rtx, header, name, mirror):
+    """The Python structures passed to the library by pointer have the header's
+    fields, order, offsets and size (a field added on one side only would make
+    the library read past the caller's structure)."""
+    fields, size = _c_struct(header, name)
+    cls = getattr(rtx, mirror)
+    assert [f[0] for f in cls._fields_] == [f[0] for f in fields]
+    assert [getattr(cls, f[0]).offset for f in fields] == [f[1] for f in fields]
+    assert ctypes.sizeof(cls) == size
+
+
+@pytest.mark.parametrize("name,dtype", [("rtx_ray", "RAY_DTYPE"), ("rtx_hit", "HIT_DTYPE"),
+                                        ("rtx_record", "RECORD_DTYPE"), ("rtx_mlp_sample", "MLP_SAMPLE_DTYPE")])
+def test_numpy_records_match_the_headers(rtx, name, dtype):
+    fields, size = _c_struct("rtx.h", name)
+    dt = getattr(rtx, dtype)
+    assert list(dt.names) == [f[0] for f in fields]
+    assert [dt.fields[f[0]][1] for f in fields] == [f[1] for f in fields]
+    assert dt.itemsize == size
