@@ -138,7 +138,8 @@ typedef struct rtx_trace_options {
   uint32_t lds_stack;    /* stack entries per lane held in LDS (4, 8, 16); 0 = 8 for PLAIN, 4 for TREELET */
   uint32_t stage_bytes;  /* LDS bytes per wave for a staged treelet (4-64 KiB, power of two);
                             0 = the treelet budget, capped at 16 KiB */
-  uint32_t refill;       /* a wave fetches new rays once this many lanes are idle (1-64); 0 = default */
+  uint32_t refill;       /* a wave fetches new rays once this many lanes are idle (1-64); 0 = 40
+                            (48 for batches) */
   uint32_t slot_bytes;   /* staging: LDS slot size; stage_bytes / slot_bytes treelets stay resident
                             at once, least recently used evicted first.  0 = one slot */
   float min_score;       /* LEARNED: summed lane probability a predicted treelet needs before it is

@@ -517,7 +517,10 @@ rtx_status rtx_trace(rtx_ctx* c, const rtx_ray* rays, uint32_t n, const rtx_trac
   p.rays = rays;
   p.n = n;
   if (opts->refill > 64) return RTX_ERR_INVALID_ARGUMENT;
-  p.refill = opts->refill ? opts->refill : 32;
+  // default refill: 40 idle lanes (on the 1M-triangle diffuse batch 2.6 %
+  // faster than 32 and 0.5 % slower than 48; on the 21M-triangle scene 0.2 %
+  // faster than 32 and 1.2 % faster than 48)
+  p.refill = opts->refill ? opts->refill : 40;
   p.hits = hits;
   p.records = nullptr;
   p.record_capacity = 0;
