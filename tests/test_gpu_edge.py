@@ -118,3 +118,59 @@ def test_many_calls_and_errors_reuse_control_state(gpu, rtx, oracle, c1):
         if k % 25 == 0 or k in (40, 41, 100):
             from conftest import assert_hits_equal
             assert_hits_equal(hb.download(rtx.HIT_DTYPE, len(sub)), ref, f"call {k}")
+
+
+def _invert_child_boxes(arena, tlas, limit):
+    """Copy of the arena where up to `limit` internal-node children reached from
+    the TLAS (breadth first, so the top levels every ray visits) have their
+    lower and upper x bound bytes swapped (lo > hi): the reference's slab test
+    still reads such a box as the span between the two."""
+    import struct
+    A = arena.copy()
+    root = tlas + struct.unpack_from("<Q", A, tlas)[0]
+    queue, done, seen = [root], 0, set()
+    while queue and done < limit:
+        n = queue.pop(0)
+        if n in seen:
+            continue
+        seen.add(n)
+        c = n + 64 * struct.unpack_from("<i", A, n + 12)[0]
+        for k in range(6):
+            info = int(A[n + 22 + k]) & 0x3F
+            size, typ = info & 3, info >> 2
+            if not size:
+                continue
+            lo, hi = int(A[n + 28 + k]), int(A[n + 34 + k])
+            if lo < hi and done < limit:
+                A[n + 28 + k], A[n + 34 + k] = hi, lo
+                done += 1
+            if typ == 0:
+                queue.append(c)
+            c += 64 * size
+    assert done > 0
+    return A
+
+
+@pytest.mark.parametrize("mode", [0, 1])
+def test_inverted_child_boxes_match_the_oracle(gpu, rtx, oracle, c1, mode):
+    """An arena whose child boxes are not all ordered keeps the compare-select
+    slab tests (the sign-ordered form would read an inverted box as empty)."""
+    scene, rays = c1
+    A = _invert_child_boxes(scene.arena, scene.tlas_offset, 40)
+    gpu.set_scene(A, scene.tlas_offset)
+    part = None
+    if mode == 1:
+        gpu.build_treelets(8192)
+        part = oracle.Partition(A, scene.tlas_offset, 8192)
+    h_ref, o_ref, r_ref = oracle.trace(A, scene.tlas_offset, rays, mode=mode, partition=part)
+    h, o, r = gpu.trace(rays, mode=mode, records=True)
+    from conftest import assert_hits_equal, assert_records_equal
+    assert_hits_equal(h, h_ref, f"inverted boxes, mode {mode}")
+    assert_records_equal(o, r, o_ref, r_ref, f"inverted boxes, mode {mode}")
+    assert_hits_equal(gpu.trace(rays, mode=mode, lanes_per_ray=8), h_ref, f"inverted boxes, mode {mode}, 8 lanes")
+    # the reference reads a swapped pair as the same span, so the rays visit
+    # what they visit in the original arena; a test that took swapped bounds
+    # as an empty box would miss those children
+    h_orig = oracle.trace(scene.arena, scene.tlas_offset, rays, mode=mode, partition=None if mode == 0 else
+                          oracle.Partition(scene.arena, scene.tlas_offset, 8192), records=False)
+    assert np.array_equal(h_orig["n_nodes"], h_ref["n_nodes"]) and h_ref["n_nodes"].sum() > 0

@@ -37,6 +37,7 @@ struct rtx_ctx {
   uint8_t* arena_owned = nullptr;
   uint64_t arena_bytes = 0;
   uint64_t tlas = 0;
+  bool ordered = false;  // every child box the TLAS reaches has lo <= hi (boxes_ordered)
   // partition
   Partition part;
   // scratch
@@ -68,6 +69,7 @@ struct rtx_ctx {
     uint8_t* arena = nullptr;
     uint64_t bytes = 0;
     uint64_t tlas = 0;
+    bool ordered = false;  // boxes_ordered() of the arena
     Partition part;
   };
   std::vector<SceneSlot> scenes;
@@ -120,6 +122,7 @@ void free_scene(rtx_ctx* c) {
   c->arena_owned = nullptr;
   c->arena = nullptr;
   c->arena_bytes = 0;
+  c->ordered = false;
 }
 
 rtx_status ensure_spill(rtx_ctx* c, size_t entries) {
@@ -389,6 +392,7 @@ rtx_status rtx_set_scene(rtx_ctx* c, const void* host, size_t bytes, uint64_t tl
   c->arena = d;
   c->arena_bytes = bytes;
   c->tlas = tlas;
+  c->ordered = boxes_ordered(d, bytes, tlas, c->stream);
   return RTX_OK;
 }
 
@@ -405,6 +409,7 @@ rtx_status rtx_set_scene_device(rtx_ctx* c, const void* dev, size_t bytes, uint6
   c->arena = (const uint8_t*)dev;
   c->arena_bytes = bytes;
   c->tlas = tlas;
+  c->ordered = boxes_ordered(c->arena, bytes, tlas, c->stream);
   return RTX_OK;
 }
 
@@ -521,6 +526,7 @@ rtx_status rtx_trace(rtx_ctx* c, const rtx_ray* rays, uint32_t n, const rtx_trac
   // faster than 32 and 0.5 % slower than 48; on the 21M-triangle scene 0.2 %
   // faster than 32 and 1.2 % faster than 48)
   p.refill = opts->refill ? opts->refill : 40;
+  p.ordered_boxes = c->ordered ? 1u : 0u;
   p.hits = hits;
   p.records = nullptr;
   p.record_capacity = 0;
@@ -727,6 +733,10 @@ TraceParams batch_params(rtx_ctx* c, const BatchJob* jobs_dev, uint32_t n_jobs, 
   p.stats = c->stats;
   p.jobs = jobs_dev;
   p.n_jobs = n_jobs;
+  // sign-ordered slab tests only when every scene of the context allows them
+  p.ordered_boxes = 1;
+  for (const auto& sc : c->scenes)
+    if (!sc.ordered) p.ordered_boxes = 0;
   return p;
 }
 
@@ -750,6 +760,7 @@ rtx_status rtx_add_scene(rtx_ctx* c, const void* host, size_t bytes, uint64_t tl
   }
   sc.bytes = bytes;
   sc.tlas = tlas;
+  sc.ordered = boxes_ordered(sc.arena, bytes, tlas, c->stream);
   c->scenes.push_back(sc);
   *id = (uint32_t)(c->scenes.size() - 1);
   return RTX_OK;

@@ -74,6 +74,9 @@ struct TraceParams {
   // multi-scene batch (batch kernels only): rays are numbered across jobs
   const BatchJob* jobs;
   uint32_t n_jobs;
+  // every child box of the scene(s) has lower bound bytes <= upper ones, so a
+  // finite ray may pick each axis's near and far bound by its direction sign
+  uint32_t ordered_boxes;
 };
 
 // Most treelets resident at once in a wave's staging area.

@@ -39,6 +39,7 @@ rtx_status partition_copy_out(const Partition& p, uint64_t arena_bytes, hipStrea
 rtx_status partition_install(const uint8_t* arena, uint64_t bytes, uint64_t tlas, uint32_t budget, uint32_t n,
                              const uint64_t* roots, const uint64_t* list_offsets, const uint64_t* list_nodes,
                              const uint32_t* owner, hipStream_t s, Partition& out);
+bool boxes_ordered(const uint8_t* arena, uint64_t bytes, uint64_t tlas, hipStream_t s);
 rtx_status partition_validate(const uint8_t* arena, uint64_t bytes, uint64_t tlas, hipStream_t s,
                               uint64_t* n_nodes);
 

@@ -397,6 +397,7 @@ struct ValParams {
   uint8_t* kind;
   uint32_t* shared;
   uint32_t* error;
+  uint32_t* unordered;  // set when a valid child box has a lower bound byte above its upper one
 };
 
 __device__ void val_ref(const ValParams& p, uint64_t off, uint8_t k, bool push, uint32_t qkind) {
@@ -425,6 +426,8 @@ __global__ void validate_level(ValParams p) {
       const uint32_t info = nd[22 + k] & 0x3fu;
       const uint32_t size = info & 3u, type = info >> 2;
       if (size == 0) continue;
+      // lower x/y/z at bytes 28/40/52 + k, upper at 34/46/58 + k
+      if (nd[28 + k] > nd[34 + k] || nd[40 + k] > nd[46 + k] || nd[52 + k] > nd[58 + k]) atomicOr(p.unordered, 1u);
       const uint64_t len = (uint64_t)size * 64;
       if ((c & 63) || c + len > p.bytes) {
         atomicOr(p.error, kErrRange);
@@ -515,11 +518,11 @@ rtx_status map_err(uint32_t bits) {
 
 // Walks the DAG; fills kind[] and reports whether any node is referenced twice.
 rtx_status run_validate(const uint8_t* A, uint64_t bytes, uint64_t tlas, hipStream_t s, Scratch& sc, uint8_t* kind,
-                        uint32_t* shared_out, uint64_t* n_nodes) {
+                        uint32_t* shared_out, uint64_t* n_nodes, uint32_t* unordered_out = nullptr) {
   const uint64_t slots = bytes / 64;
   uint32_t* refs;
   uint64_t *qa, *qb;
-  uint32_t* ctr;  // [0] n_out, [1] error, [2] shared
+  uint32_t* ctr;  // [0] n_out, [1] error, [2] shared, [3] unordered
   CS(sc.alloc(&refs, slots));
   CS(sc.alloc(&qa, slots));
   CS(sc.alloc(&qb, slots));
@@ -544,7 +547,7 @@ rtx_status run_validate(const uint8_t* A, uint64_t bytes, uint64_t tlas, hipStre
   uint64_t total = 2;
   while (n_in > 0) {
     CK(hipMemsetAsync(ctr, 0, 4, s));
-    ValParams vp{A, bytes, qa, n_in, qb, ctr, refs, kind, ctr + 2, ctr + 1};
+    ValParams vp{A, bytes, qa, n_in, qb, ctr, refs, kind, ctr + 2, ctr + 1, ctr + 3};
     uint32_t grid = std::min<uint32_t>(4096, (n_in + 255) / 256);
     hipLaunchKernelGGL(validate_level, dim3(grid), dim3(256), 0, s, vp);
     CK(hipGetLastError());
@@ -557,10 +560,11 @@ rtx_status run_validate(const uint8_t* A, uint64_t bytes, uint64_t tlas, hipStre
     std::swap(qa, qb);
   }
   // count every referenced slot (leaves and headers included)
-  uint32_t c[3];
-  CK(hipMemcpyAsync(c, ctr, 12, hipMemcpyDeviceToHost, s));
+  uint32_t c[4];
+  CK(hipMemcpyAsync(c, ctr, 16, hipMemcpyDeviceToHost, s));
   CK(hipStreamSynchronize(s));
   if (shared_out) *shared_out = c[2];
+  if (unordered_out) *unordered_out = c[3];
   if (n_nodes) {
     std::vector<uint8_t> k(slots);
     CK(hipMemcpyAsync(k.data(), kind, slots, hipMemcpyDeviceToHost, s));
@@ -696,6 +700,24 @@ void partition_free(Partition& p) {
   hipFree(p.stage_slot);
   hipFree(p.image_slots);
   p = Partition{};
+}
+
+// Whether every internal node reachable from the TLAS has, for each child it
+// uses, lower bound bytes no greater than the upper ones on all three axes (so
+// the child's decoded box has lo <= hi).  False as well when the walk finds a
+// malformed arena or cannot allocate its scratch: traversal then keeps the
+// compare-select slab tests, and a malformed arena still reports at trace time.
+bool boxes_ordered(const uint8_t* A, uint64_t bytes, uint64_t tlas, hipStream_t s) {
+  Scratch sc;
+  uint8_t* kind;
+  if (sc.alloc(&kind, bytes / 64) != RTX_OK) return false;
+  uint32_t unordered = 1;
+  if (run_validate(A, bytes, tlas, s, sc, kind, nullptr, nullptr, &unordered) != RTX_OK) {
+    (void)hipGetLastError();
+    (void)hipStreamSynchronize(s);
+    return false;
+  }
+  return unordered == 0;
 }
 
 rtx_status partition_validate(const uint8_t* A, uint64_t bytes, uint64_t tlas, hipStream_t s, uint64_t* n_nodes) {

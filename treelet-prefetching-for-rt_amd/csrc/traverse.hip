@@ -826,26 +826,70 @@ void trace_kernel(TraceParams p) {
             const uint64_t first_child = off + (uint64_t)((int64_t)(int32_t)nd.w[3] * 64);
             const F3 sc = child_scales(ex, ey, ez);
             uint32_t hitmask = 0;
-            // v_min/v_max slab tests when no lane can produce a NaN (finite
-            // ray, non-NaN node origin); compare-selects otherwise
+            // When no lane can produce a NaN (finite ray, non-NaN node origin)
+            // and the arena's child boxes all have lo <= hi, each axis's near
+            // and far bound is picked by the sign of the inverse direction and
+            // the slab test needs only the max of the near and the min of the
+            // far distances: for such boxes min(lo', hi') and max(lo', hi')
+            // are exactly the picked products (rounding is monotonic), and the
+            // v_min/v_max forms differ from the reference's compare-selects
+            // only in the sign of a zero, which only enters comparisons.
+            // Otherwise the reference's compare-selects run.
             const bool fast = (top ? ray_ok : cur_ok) && !__builtin_isnan(f_of(nd, 0) + f_of(nd, 1) + f_of(nd, 2));
             auto test_children = [&](auto fast_tag) {
               constexpr bool FAST = decltype(fast_tag)::value;
+              if constexpr (FAST) {
+                // near / far bound bytes per axis chosen by the ray's direction sign
+                uint32_t nw[3][2], fw[3][2];
 #pragma unroll
-              for (int i = 0; i < 6; ++i) {
-                const uint32_t info = byte_of(nd, 22 + i) & 0x3fu;
-                const uint32_t size = info & 3u;
-                if (size > 0) {
-                  F3 lo, hi;
-                  child_box(nd, i, sc, lo, hi);
-                  float th;
-                  bool h = FAST ? ray_box_fast(lo, hi, idir, o, tmin, tmax, th) : ray_box(lo, hi, idir, o, tmin, tmax, th);
-                  if (h && th >= cull) h = false;
-                  if (h) hitmask |= 1u << i;
+                for (int a = 0; a < 3; ++a) {
+                  const int lw = 7 + 3 * a;
+                  const float d = a == 0 ? idir.x : a == 1 ? idir.y : idir.z;
+                  const bool neg = d < 0.0f;
+                  const uint32_t l0 = nd.w[lw], l1 = nd.w[lw + 1];
+                  const uint32_t u0 = __builtin_amdgcn_alignbyte(nd.w[lw + 2], nd.w[lw + 1], 2), u1 = nd.w[lw + 2] >> 16;
+                  nw[a][0] = neg ? u0 : l0;
+                  nw[a][1] = neg ? u1 : l1;
+                  fw[a][0] = neg ? l0 : u0;
+                  fw[a][1] = neg ? l1 : u1;
+                }
+                const float ox = f_of(nd, 0), oy = f_of(nd, 1), oz = f_of(nd, 2);
+#pragma unroll
+                for (int i = 0; i < 6; ++i) {
+                  const uint32_t info = byte_of(nd, 22 + i) & 0x3fu;
+                  if ((info & 3u) > 0) {
+                    const int wi = i >> 2, sh = 8 * (i & 3);
+                    const float nx = __builtin_fmaf((float)((nw[0][wi] >> sh) & 0xffu), sc.x, ox);
+                    const float ny = __builtin_fmaf((float)((nw[1][wi] >> sh) & 0xffu), sc.y, oy);
+                    const float nz = __builtin_fmaf((float)((nw[2][wi] >> sh) & 0xffu), sc.z, oz);
+                    const float fx = __builtin_fmaf((float)((fw[0][wi] >> sh) & 0xffu), sc.x, ox);
+                    const float fy = __builtin_fmaf((float)((fw[1][wi] >> sh) & 0xffu), sc.y, oy);
+                    const float fz = __builtin_fmaf((float)((fw[2][wi] >> sh) & 0xffu), sc.z, oz);
+                    const float tn = __builtin_fmaxf(__builtin_fmaxf(__builtin_fmaxf((nx - o.x) * idir.x, (ny - o.y) * idir.y),
+                                                                     (nz - o.z) * idir.z), tmin);
+                    const float tf = __builtin_fminf(__builtin_fminf(__builtin_fminf((fx - o.x) * idir.x, (fy - o.y) * idir.y),
+                                                                     (fz - o.z) * idir.z), tmax);
+                    const bool h = tn <= tf && !(tn >= cull);
+                    if (h) hitmask |= 1u << i;
+                  }
+                }
+              } else {
+#pragma unroll
+                for (int i = 0; i < 6; ++i) {
+                  const uint32_t info = byte_of(nd, 22 + i) & 0x3fu;
+                  const uint32_t size = info & 3u;
+                  if (size > 0) {
+                    F3 lo, hi;
+                    child_box(nd, i, sc, lo, hi);
+                    float th;
+                    bool h = ray_box(lo, hi, idir, o, tmin, tmax, th);
+                    if (h && th >= cull) h = false;
+                    if (h) hitmask |= 1u << i;
+                  }
                 }
               }
             };
-            if (__ballot(!fast) == 0)
+            if (__ballot(!(fast && p.ordered_boxes)) == 0)
               test_children(std::true_type{});
             else
               test_children(std::false_type{});
