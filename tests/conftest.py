@@ -69,12 +69,19 @@ def gpu(rtx):
 
 
 def assert_hits_equal(a, b, what=""):
-    """Bit-exact equality of two hit arrays (floats compared as bit patterns)."""
+    """Bit-exact equality of two hit arrays: floats compare as bit patterns,
+    except that a NaN equals any NaN.  IEEE 754 leaves the sign and payload of
+    a NaN made by an invalid operation (inf - inf, 0 * inf) open: x86 makes it
+    negative, the GPU positive."""
     for f in a.dtype.names:
         x, y = a[f], b[f]
         if x.dtype.kind == "f":
+            both_nan = np.isnan(x) & np.isnan(y)
             x, y = x.view(np.uint32), y.view(np.uint32)
-        bad = np.nonzero((x != y).reshape(len(a), -1).any(axis=1))[0]
+            diff = (x != y) & ~both_nan
+        else:
+            diff = x != y
+        bad = np.nonzero(diff.reshape(len(a), -1).any(axis=1))[0]
         assert len(bad) == 0, f"{what}: field {f} differs at {len(bad)} rays, first {bad[:5]}: {a[bad[:3]]} vs {b[bad[:3]]}"
 
 

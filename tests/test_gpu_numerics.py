@@ -1,0 +1,123 @@
+"""Numerical edge cases of the traversal against the oracle, bit for bit, in
+both orders and with one and eight lanes per ray: degenerate triangles (the
+Möller-Trumbore test has no determinant epsilon, so 1/det can be infinite),
+sheared / mirrored / non-uniformly scaled instances, extreme coordinate
+scales, and empty or negative ray intervals."""
+import numpy as np
+import pytest
+
+from conftest import assert_hits_equal, assert_records_equal
+
+pytestmark = pytest.mark.gpu
+
+
+def _rays_towards(rtx, centers, n, seed, spread=0.3, dist=6.0):
+    """n rays from random points around the scene towards jittered targets
+    near the given centers."""
+    rng = np.random.default_rng(seed)
+    r = np.zeros(n, rtx.RAY_DTYPE)
+    tgt = centers[rng.integers(0, len(centers), n)] + rng.normal(0, spread, (n, 3)) * np.abs(centers).max() * 0.05
+    d = rng.normal(0, 1, (n, 3))
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    org = tgt - d * dist
+    r["org"] = org.astype(np.float32)
+    r["dir"] = d.astype(np.float32)
+    r["tmin"] = 0.0
+    r["tmax"] = np.float32(1e30)
+    r["flags"] = 1
+    return r
+
+
+def _check_all(gpu, oracle, scene, rays, what):
+    """Both orders; records from the one-lane kernel, hits from both kernels."""
+    gpu.set_scene(scene.arena, scene.tlas_offset)
+    gpu.build_treelets(1024)
+    part = oracle.Partition(scene.arena, scene.tlas_offset, 1024)
+    for mode in (0, 1):
+        # with a partition built, plain-order records carry the node's treelet too
+        h_ref, o_ref, r_ref = oracle.trace(scene.arena, scene.tlas_offset, rays, mode=mode, partition=part)
+        h, o, r = gpu.trace(rays, mode=mode, records=True)
+        assert_hits_equal(h, h_ref, f"{what}, mode {mode}")
+        assert_records_equal(o, r, o_ref, r_ref, f"{what}, mode {mode}")
+        for lanes in (1, 8):
+            assert_hits_equal(gpu.trace(rays, mode=mode, lanes_per_ray=lanes), h_ref,
+                              f"{what}, mode {mode}, {lanes} lanes per ray")
+    return h_ref
+
+
+def test_degenerate_triangles(gpu, rtx, oracle):
+    """Zero-area triangles (repeated and collinear vertices) mixed into a mesh:
+    det = 0 makes 1/det infinite and u, v, t NaN or infinite."""
+    tris = rtx.displaced_sphere(16, 8, seed=11).copy()
+    rng = np.random.default_rng(3)
+    pick = rng.choice(len(tris), 96, replace=False)
+    t = tris.reshape(-1, 3, 3)
+    t[pick[:32], 1] = t[pick[:32], 0]                                   # repeated vertex
+    t[pick[32:64], 2] = 0.5 * (t[pick[32:64], 0] + t[pick[32:64], 1])   # collinear
+    t[pick[64:], :] = t[pick[64:], :1]                                  # a point
+    b = rtx.SceneBuilder()
+    bid = b.add_blas(tris, geometry_index=2)
+    b.add_instance(bid, instance_id=5)
+    scene = b.finish()
+    cen = t[pick].mean(axis=1)
+    rays = _rays_towards(rtx, cen.astype(np.float64), 4096, seed=5, spread=0.02, dist=3.0)
+    h = _check_all(gpu, oracle, scene, rays, "degenerate triangles")
+    assert h["hit"].sum() > 100
+
+
+def test_affine_instances(gpu, rtx, oracle):
+    """Instances under shear, non-uniform scale and mirroring (negative
+    determinant): the object-space ray has a non-unit direction before
+    normalisation and tMult differs per instance."""
+    rng = np.random.default_rng(17)
+    b = rtx.SceneBuilder()
+    blas = [b.add_blas(rtx.displaced_sphere(12, 6, seed=s), geometry_index=s) for s in range(3)]
+    centers = []
+    for k in range(24):
+        m = np.eye(3) + rng.normal(0, 0.35, (3, 3))       # shear and scale
+        m[:, 0] *= rng.choice([-1.0, 1.0]) * rng.uniform(0.3, 3.0)
+        m[:, 2] *= rng.uniform(0.3, 3.0)
+        pos = rng.uniform(-12, 12, 3)
+        o2w = np.hstack([m, pos[:, None]]).astype(np.float32)
+        b.add_instance(blas[k % 3], o2w, instance_id=100 + k)
+        centers.append(pos)
+    scene = b.finish()
+    rays = _rays_towards(rtx, np.array(centers), 8192, seed=9, spread=1.0, dist=30.0)
+    h = _check_all(gpu, oracle, scene, rays, "affine instances")
+    assert h["hit"].sum() > 1000
+
+
+@pytest.mark.parametrize("scale", [1e-9, 1e12])
+def test_extreme_coordinate_scales(gpu, rtx, oracle, scale):
+    """The same instanced geometry at 1e-9 and 1e12: the Möller-Trumbore
+    products range down to about 1e-28 and up to about 1e35 (near the top of
+    the float range), and the child bound exponents sit far from 0."""
+    b = rtx.SceneBuilder()
+    tris = (rtx.displaced_sphere(12, 6, seed=4).astype(np.float64) * scale).astype(np.float32)
+    bid = b.add_blas(tris, geometry_index=1)
+    rng = np.random.default_rng(2)
+    centers = []
+    for k in range(8):
+        pos = rng.uniform(-5, 5, 3) * scale
+        o2w = np.hstack([np.eye(3), pos[:, None]]).astype(np.float32)
+        b.add_instance(bid, o2w, instance_id=k)
+        centers.append(pos)
+    scene = b.finish()
+    rays = _rays_towards(rtx, np.array(centers), 4096, seed=4, spread=1.0, dist=20.0 * scale)
+    rays["tmax"] = np.float32(np.finfo(np.float32).max)
+    h = _check_all(gpu, oracle, scene, rays, f"coordinates x {scale:g}")
+    assert h["hit"].sum() > 1000
+
+
+def test_ray_intervals(gpu, rtx, oracle, c1):
+    """Empty (tmin > tmax), point (tmin == tmax), negative-start and zero-length
+    intervals, and a tmax that ends inside the geometry."""
+    scene, rays = c1
+    r = rays[:4096].copy()
+    r["tmin"][0::5] = 50.0
+    r["tmax"][0::5] = 10.0
+    r["tmin"][1::5] = r["tmax"][1::5] = 4.0
+    r["tmin"][2::5] = -100.0
+    r["tmax"][3::5] = 0.0
+    r["tmax"][4::5] = 2.5
+    _check_all(gpu, oracle, scene, r, "ray intervals")
