@@ -45,9 +45,9 @@ def _check_all(gpu, oracle, scene, rays, what):
     return h_ref
 
 
-def test_degenerate_triangles(gpu, rtx, oracle):
-    """Zero-area triangles (repeated and collinear vertices) mixed into a mesh:
-    det = 0 makes 1/det infinite and u, v, t NaN or infinite."""
+def _degenerate_scene(rtx):
+    """A mesh with zero-area triangles (repeated and collinear vertices, points)
+    and rays aimed at them."""
     tris = rtx.displaced_sphere(16, 8, seed=11).copy()
     rng = np.random.default_rng(3)
     pick = rng.choice(len(tris), 96, replace=False)
@@ -60,15 +60,11 @@ def test_degenerate_triangles(gpu, rtx, oracle):
     b.add_instance(bid, instance_id=5)
     scene = b.finish()
     cen = t[pick].mean(axis=1)
-    rays = _rays_towards(rtx, cen.astype(np.float64), 4096, seed=5, spread=0.02, dist=3.0)
-    h = _check_all(gpu, oracle, scene, rays, "degenerate triangles")
-    assert h["hit"].sum() > 100
+    return scene, _rays_towards(rtx, cen.astype(np.float64), 4096, seed=5, spread=0.02, dist=3.0)
 
 
-def test_affine_instances(gpu, rtx, oracle):
-    """Instances under shear, non-uniform scale and mirroring (negative
-    determinant): the object-space ray has a non-unit direction before
-    normalisation and tMult differs per instance."""
+def _affine_scene(rtx):
+    """Instances under shear, non-uniform scale and mirroring."""
     rng = np.random.default_rng(17)
     b = rtx.SceneBuilder()
     blas = [b.add_blas(rtx.displaced_sphere(12, 6, seed=s), geometry_index=s) for s in range(3)]
@@ -82,16 +78,11 @@ def test_affine_instances(gpu, rtx, oracle):
         b.add_instance(blas[k % 3], o2w, instance_id=100 + k)
         centers.append(pos)
     scene = b.finish()
-    rays = _rays_towards(rtx, np.array(centers), 8192, seed=9, spread=1.0, dist=30.0)
-    h = _check_all(gpu, oracle, scene, rays, "affine instances")
-    assert h["hit"].sum() > 1000
+    return scene, _rays_towards(rtx, np.array(centers), 8192, seed=9, spread=1.0, dist=30.0)
 
 
-@pytest.mark.parametrize("scale", [1e-9, 1e12])
-def test_extreme_coordinate_scales(gpu, rtx, oracle, scale):
-    """The same instanced geometry at 1e-9 and 1e12: the Möller-Trumbore
-    products range down to about 1e-28 and up to about 1e35 (near the top of
-    the float range), and the child bound exponents sit far from 0."""
+def _scaled_scene(rtx, scale):
+    """Eight instances of one mesh with every coordinate multiplied by scale."""
     b = rtx.SceneBuilder()
     tris = (rtx.displaced_sphere(12, 6, seed=4).astype(np.float64) * scale).astype(np.float32)
     bid = b.add_blas(tris, geometry_index=1)
@@ -105,6 +96,32 @@ def test_extreme_coordinate_scales(gpu, rtx, oracle, scale):
     scene = b.finish()
     rays = _rays_towards(rtx, np.array(centers), 4096, seed=4, spread=1.0, dist=20.0 * scale)
     rays["tmax"] = np.float32(np.finfo(np.float32).max)
+    return scene, rays
+
+
+def test_degenerate_triangles(gpu, rtx, oracle):
+    """Zero-area triangles mixed into a mesh: det = 0 makes 1/det infinite and
+    u, v, t NaN or infinite."""
+    scene, rays = _degenerate_scene(rtx)
+    h = _check_all(gpu, oracle, scene, rays, "degenerate triangles")
+    assert h["hit"].sum() > 100
+
+
+def test_affine_instances(gpu, rtx, oracle):
+    """Instances under shear, non-uniform scale and mirroring (negative
+    determinant): the object-space ray has a non-unit direction before
+    normalisation and tMult differs per instance."""
+    scene, rays = _affine_scene(rtx)
+    h = _check_all(gpu, oracle, scene, rays, "affine instances")
+    assert h["hit"].sum() > 1000
+
+
+@pytest.mark.parametrize("scale", [1e-9, 1e12])
+def test_extreme_coordinate_scales(gpu, rtx, oracle, scale):
+    """The same instanced geometry at 1e-9 and 1e12: the Möller-Trumbore
+    products range down to about 1e-28 and up to about 1e35 (near the top of
+    the float range), and the child bound exponents sit far from 0."""
+    scene, rays = _scaled_scene(rtx, scale)
     h = _check_all(gpu, oracle, scene, rays, f"coordinates x {scale:g}")
     assert h["hit"].sum() > 1000
 
@@ -121,3 +138,42 @@ def test_ray_intervals(gpu, rtx, oracle, c1):
     r["tmax"][3::5] = 0.0
     r["tmax"][4::5] = 2.5
     _check_all(gpu, oracle, scene, r, "ray intervals")
+
+
+@pytest.mark.parametrize("mode", [0, 1])
+@pytest.mark.parametrize("prefetch,slot", [(1, 0), (2, 0), (3, 0), (1, 1024)])
+def test_staged_kernels_on_affine_instances(gpu, rtx, oracle, mode, prefetch, slot):
+    """The LDS staging kernels on mirrored and sheared instances: staging
+    changes where node bytes are read, never the hits or records."""
+    scene, rays = _affine_scene(rtx)
+    gpu.set_scene(scene.arena, scene.tlas_offset)
+    gpu.build_treelets(1024)
+    part = oracle.Partition(scene.arena, scene.tlas_offset, 1024)
+    h_ref, o_ref, r_ref = oracle.trace(scene.arena, scene.tlas_offset, rays, mode=mode, partition=part)
+    h, o, r = gpu.trace(rays, mode=mode, prefetch=prefetch, records=True, stage_bytes=8192, slot_bytes=slot)
+    assert_hits_equal(h, h_ref, f"staged {prefetch}/{slot}, mode {mode}")
+    assert_records_equal(o, r, o_ref, r_ref, f"staged {prefetch}/{slot}, mode {mode}")
+
+
+@pytest.mark.parametrize("mode", [0, 1])
+def test_numerics_scenes_in_one_batch(gpu, rtx, oracle, mode):
+    """The scenes above as jobs of one multi-scene batch (and a graph replay)."""
+    items = [_degenerate_scene(rtx), _affine_scene(rtx), _scaled_scene(rtx, 1e12), _scaled_scene(rtx, 1e-9)]
+    jobs, refs = [], []
+    for scene, rays in items:
+        sid = gpu.add_scene(scene.arena, scene.tlas_offset)
+        part = None
+        if mode == 1:
+            gpu.build_scene_treelets(sid, 1024)
+            part = oracle.Partition(scene.arena, scene.tlas_offset, 1024)
+        jobs.append((sid, gpu.to_device(rays), len(rays), gpu.buffer(len(rays) * rtx.HIT_DTYPE.itemsize)))
+        refs.append(oracle.trace(scene.arena, scene.tlas_offset, rays, mode=mode, partition=part, records=False))
+    assert gpu.trace_batch(jobs, mode=mode) == 0
+    for k, ((_, _, n, hb), ref) in enumerate(zip(jobs, refs)):
+        assert_hits_equal(hb.download(rtx.HIT_DTYPE, n), ref, f"batch job {k}, mode {mode}")
+    g = gpu.batch_graph(jobs, mode=mode)
+    st, _ = g.launch(3)
+    g.close()
+    assert st == 0
+    for k, ((_, _, n, hb), ref) in enumerate(zip(jobs, refs)):
+        assert_hits_equal(hb.download(rtx.HIT_DTYPE, n), ref, f"batch graph job {k}, mode {mode}")
