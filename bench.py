@@ -1,25 +1,31 @@
 #!/usr/bin/env python3
 """Ray-traversal benchmark on one MI355X.
 
-Workload (1,048,576 triangles, diffuse): 1,024 instances of a 1,024-triangle
-displaced sphere (32x16 quads, radial displacement U[-0.05, 0.05], a distinct
-seed and BLAS per instance) at uniform positions in a 100^3 box with uniform
-scale in [0.5, 2]; 1024x1024 pinhole primary rays plus one cosine-weighted
-diffuse bounce ray per primary hit, all FP32, all resident in HBM.
-
-A step traces the whole batch (primary + bounce rays) once.  `value` is the
-batch's rays per second of the headline mode over the timed steps (wall clock
-around synchronous rtx_trace calls); every mode is measured and reported.
+Workload (default, 1,048,576 triangles, diffuse): 1,024 instances of a
+1,024-triangle displaced sphere (32x16 quads, radial displacement U[-0.05,
+0.05], a distinct seed and BLAS per instance) at uniform positions in a 100^3
+box with uniform scale in [0.5, 2]; 1024x1024 pinhole primary rays plus one
+cosine-weighted diffuse bounce ray per primary hit, all FP32, all resident in
+HBM.  A step traces the whole batch (primary + bounce rays) once.
 
 --workload large: 20,480 instances of the same sphere (20,971,520 triangles,
 1.7 GiB arena) in a 271^3 box, 3840x2160 pixels x 4 samples (a stratified 2x2
 grid per pixel) and 3 cosine-weighted diffuse bounces per path without Russian
-roulette; a step traces the four segments (camera rays, then each bounce) once.
+roulette; a step traces the four segments (camera rays, then each bounce) once,
+as four launches or as one persistent batch launch.
 
---workload c1: the small scene the reference's CPU path runs, a displaced UV sphere
-of 16,384 triangles over a 2-triangle ground quad, 128x128 primary rays.
+--workload c3: 64 scenes of instanced displaced spheres with log-uniform
+triangle counts in [1k, 2M], each with a log-uniform batch of [4k, 1M] rays
+(50 % primary, 25 % shadow with TerminateOnFirstHit, 25 % diffuse); a step is
+one replay of the hipGraph that holds the persistent batch launch over all 64.
 
-  python bench.py [--steps K] [--warmup W] [--budget BYTES] [--mode NAME] [--workload diffuse|large|c1]
+--workload c1: the small scene the reference's CPU path runs, a displaced UV
+sphere of 16,384 triangles over a 2-triangle ground quad, 128x128 primary rays.
+
+`value` is the headline mode's rays per second over the timed steps (wall
+clock around synchronous calls); every mode is measured and reported.
+
+  python bench.py [--steps K] [--warmup W] [--workload diffuse|large|c3|c1] [--mode NAME] [--modes a,b,...]
 """
 import argparse
 import json
@@ -36,19 +42,44 @@ HBM_PEAK_GBS = 8000.0
 METRIC = "Mrays/s plain vs treelet vs learned, 1M-tri diffuse; node-fetch GB/s % of HBM peak"
 
 # name -> traversal order, prefetch policy (0 none, 1 always, 2 popularity, 3 partial, 4 learned),
-# treelet budget, LDS staging bytes per wave, LDS slot bytes (0: one treelet resident)
+# treelet budget, LDS staging bytes per wave, LDS slot bytes (0: one treelet resident),
+# launch ("trace": rtx_trace per segment, "batch": one graph-captured persistent launch over
+# every segment, longest segments first), records (hits + node-access records, as traceRay
+# returns them) and claim (segments of camera then secondary rays: claim the secondary
+# rays first)
 MODES = {
-    "plain": dict(order=0, prefetch=0, budget=8192, stage=0, slot=0),
-    "treelet": dict(order=1, prefetch=0, budget=8192, stage=0, slot=0),
-    "plain+stage": dict(order=0, prefetch=1, budget=8192, stage=8192, slot=0),
-    "treelet+stage": dict(order=1, prefetch=1, budget=8192, stage=8192, slot=0),
-    "treelet+stage_pop": dict(order=1, prefetch=2, budget=8192, stage=8192, slot=0),
-    "treelet+stage_partial": dict(order=1, prefetch=3, budget=8192, stage=8192, slot=0),
-    "plain+slots": dict(order=0, prefetch=1, budget=1024, stage=8192, slot=1024),
-    "plain+learned": dict(order=0, prefetch=4, budget=1024, stage=8192, slot=1024),
+    "plain": dict(order=0, prefetch=0, budget=8192, stage=0, slot=0, launch="trace", records=False, claim=True),
+    "plain+input_order": dict(order=0, prefetch=0, budget=8192, stage=0, slot=0, launch="trace", records=False,
+                              claim=False),
+    "treelet": dict(order=1, prefetch=0, budget=8192, stage=0, slot=0, launch="trace", records=False, claim=False),
+    "plain+records": dict(order=0, prefetch=0, budget=8192, stage=0, slot=0, launch="trace", records=True,
+                          claim=True),
+    "treelet+records": dict(order=1, prefetch=0, budget=8192, stage=0, slot=0, launch="trace", records=True,
+                            claim=False),
+    "plain+stage": dict(order=0, prefetch=1, budget=8192, stage=8192, slot=0, launch="trace", records=False,
+                        claim=False),
+    "treelet+stage": dict(order=1, prefetch=1, budget=8192, stage=8192, slot=0, launch="trace", records=False,
+                          claim=False),
+    "treelet+stage_pop": dict(order=1, prefetch=2, budget=8192, stage=8192, slot=0, launch="trace", records=False,
+                              claim=False),
+    "treelet+stage_partial": dict(order=1, prefetch=3, budget=8192, stage=8192, slot=0, launch="trace",
+                                  records=False, claim=False),
+    "plain+slots": dict(order=0, prefetch=1, budget=1024, stage=8192, slot=1024, launch="trace", records=False,
+                        claim=False),
+    "plain+learned": dict(order=0, prefetch=4, budget=1024, stage=8192, slot=1024, launch="trace", records=False,
+                          claim=False),
+    "plain+batch": dict(order=0, prefetch=0, budget=8192, stage=0, slot=0, launch="batch", records=False,
+                        claim=False),
+    "treelet+batch": dict(order=1, prefetch=0, budget=8192, stage=0, slot=0, launch="batch", records=False,
+                          claim=False),
 }
-DEFAULT_MODES = "plain,treelet,plain+stage,treelet+stage,plain+slots,plain+learned"
-HEADLINE = "plain"
+DEFAULTS = {
+    "diffuse": ("plain", "plain,plain+input_order,treelet,plain+records,treelet+records,plain+stage,treelet+stage,"
+                         "plain+slots,plain+learned"),
+    "large": ("plain+batch", "plain+batch,plain,treelet,treelet+batch,plain+records,plain+slots,plain+learned"),
+    "c3": ("plain+batch", "plain+batch,treelet+batch"),
+    "c1": ("plain", "plain,treelet,plain+records,treelet+records"),
+}
 
 
 def c5_desc(rtx, width=7680, height=4320):
@@ -73,35 +104,57 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def cpu_baseline(scene, sample, what, target_s=10.0):
-    """The oracle (C restatement of traceRay) on host cores, plain order: passes over
-    `sample` until target_s seconds of work have run."""
+def host_threads():
+    """Host threads for the CPU baseline: the cores this process may run on,
+    bounded by OMP_NUM_THREADS where the host sets it (the CPU share a GPU's
+    process gets on a shared host)."""
+    n = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+    cap = os.environ.get("OMP_NUM_THREADS")
+    if cap and cap.isdigit() and int(cap) > 0:
+        n = min(n, int(cap))
+    return max(1, n)
+
+
+def cpu_baseline(samples, what, target_s=10.0):
+    """The oracle (C restatement of traceRay, plain order, hits only) on host cores:
+    passes over `samples` (a list of (arena, tlas, rays)) until target_s seconds of work
+    have run on all threads, then target_s / 3 seconds on one thread."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle
 
-    threads = max(1, min(16, os.cpu_count() or 1))
-    rays = done = 0
-    t0 = time.perf_counter()
-    while True:
-        oracle.trace_parallel(scene.arena, scene.tlas_offset, sample, mode=0, threads=threads)
-        rays += len(sample)
-        done += 1
-        dt = time.perf_counter() - t0
-        if dt >= target_s:
-            break
-    return {"value": round(rays / dt / 1e6, 4), "unit": "Mrays/s", "cores": threads, "kind": "port",
-            "sample": f"{what}: {len(sample)} rays x {done} passes, plain order, {dt:.1f} s on {threads} host threads"}
+    def run(threads, budget):
+        rays = passes = 0
+        t0 = time.perf_counter()
+        while True:
+            for arena, tlas, r in samples:
+                oracle.trace_parallel(arena, tlas, r, mode=0, threads=threads)
+                rays += len(r)
+            passes += 1
+            dt = time.perf_counter() - t0
+            if dt >= budget:
+                return rays / dt / 1e6, passes, dt
+
+    threads = host_threads()
+    v, passes, dt = run(threads, target_s)
+    n = sum(len(r) for _, _, r in samples)
+    single = [(a, t, r[:: max(1, threads)]) for a, t, r in samples]
+    v1, p1, dt1 = run(1, target_s / 3)
+    return {"value": round(v, 4), "unit": "Mrays/s", "cores": threads, "kind": "port",
+            "sample": f"{what}: {n} rays x {passes} passes, plain order, {dt:.1f} s on {threads} host threads",
+            "single_thread": {"value": round(v1, 4), "cores": 1,
+                              "sample": f"every {max(1, threads)}th of those rays x {p1} passes, {dt1:.1f} s"}}
 
 
-def training_rays_diffuse(scene, primary, hits):
-    """Rays other than the timed batch: a fifth of the pixels held out, diffuse
-    bounces of other seeds."""
-    idx = np.arange(len(primary))
-    keep = idx % 5 != 0
-    tp, th = primary[keep], hits[keep]
-    train = [tp[::2]]
+def training_rays_diffuse(rtx, ctx, scene):
+    """Held-out training rays for the learned prefetcher: primary rays of the same
+    camera at 768x768 pixels (no pixel centre of that grid lies on the 1024x1024 grid
+    of the timed batch) and diffuse bounces of their hits with seeds other than the
+    timed batch's."""
+    prim = rtx.primary_rays(c2_desc(rtx, 768, 768))
+    hits = ctx.trace(prim, mode=0)
+    train = [prim[::2]]
     for seed in (1, 2, 3, 4):
-        b, _ = scene.secondary_rays(tp[::4], th[::4], kind=0, seed=seed)
+        b, _ = scene.secondary_rays(prim[::2], hits[::2], kind=0, seed=seed)
         train.append(b)
     return np.concatenate(train)
 
@@ -136,6 +189,98 @@ def load_traffic(workload, mode):
         return None
 
 
+class Segment:
+    """Rays of one scene, resident in device memory with their hit buffer."""
+
+    def __init__(self, rtx, ctx, scene, rays, claim_start=0, scene_id=None):
+        self.scene, self.n, self.claim_start, self.scene_id = scene, len(rays), claim_start, scene_id
+        self.rays_host = rays
+        self.rays = ctx.to_device(rays)
+        self.hits = ctx.buffer(len(rays) * rtx.HIT_DTYPE.itemsize)
+
+
+def setup_single(rtx, ctx, args, info):
+    """diffuse / large / c1: one scene, segments of rays, and held-out training rays."""
+    if args.workload == "diffuse":
+        w = args.width or 1024
+        desc = c2_desc(rtx, w, w)
+    elif args.workload == "c1":
+        w = args.width or 128
+        desc = c1_desc(rtx, w, w)
+    else:
+        w = args.width or 7680
+        desc = c5_desc(rtx, w, w * 9 // 16)
+    t0 = time.perf_counter()
+    builder = rtx.workload_builder(desc)
+    mesh_s = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    scene = builder.finish(ctx=ctx if args.build == "device" else None)
+    build_s = time.perf_counter() - t0
+    log(f"scene: {scene.n_triangles} tris, {scene.n_blas} BLAS, {scene.arena.nbytes / 2**20:.1f} MiB arena, "
+        f"meshes in {mesh_s:.2f} s, BVH built ({args.build}) in {build_s:.2f} s")
+    ctx.set_scene(scene.arena, scene.tlas_offset)
+    primary = rtx.primary_rays(desc)
+    hits = ctx.trace(primary, mode=0)
+    claim = 0
+    if args.workload == "diffuse":
+        bounce, _ = scene.secondary_rays(primary, hits, kind=0, seed=99)
+        rays = [np.concatenate([primary, bounce])]
+        claim = len(primary)  # the bounce rays
+        log(f"batch: {len(primary)} primary ({int(hits['hit'].sum())} hits) + {len(bounce)} diffuse = "
+            f"{len(rays[0])} rays")
+        train_rays = training_rays_diffuse(rtx, ctx, scene)
+        workload = (f"1,024 instances x 1,024-tri displaced spheres ({scene.n_triangles} tris), {w}x{w} primary + "
+                    f"1 cosine diffuse bounce per hit ({len(rays[0])} rays/step)")
+    elif args.workload == "c1":
+        rays = [primary]
+        train_rays = primary
+        log(f"batch: {len(primary)} primary rays ({int(hits['hit'].sum())} hits)")
+        workload = (f"displaced UV sphere of 16,384 tris over a 2-tri ground quad ({scene.n_triangles} tris), "
+                    f"{w}x{w} primary rays ({len(primary)} rays/step)")
+    else:
+        # camera rays, then three diffuse bounces from each segment's hits; the
+        # training traces come from an earlier frame (other bounce seeds)
+        rays = [primary]
+        train = [primary[::16]]
+        seg, sh = primary, hits
+        tseg, tsh = primary[::16], hits[::16]
+        for b in range(3):
+            nxt, _ = scene.secondary_rays(seg, sh, kind=0, seed=1000 + b)
+            rays.append(nxt)
+            tn, _ = scene.secondary_rays(tseg, tsh, kind=0, seed=2000 + b)
+            train.append(tn)
+            if b < 2:
+                sh = ctx.trace(nxt, mode=0)
+                tsh = ctx.trace(tn, mode=0)
+            seg, tseg = nxt, tn
+        train_rays = np.concatenate(train)
+        log("segments: " + ", ".join(str(len(x)) for x in rays) + " rays")
+        workload = (f"20,480 instances x 1,024-tri displaced spheres ({scene.n_triangles} tris), {w // 2}x"
+                    f"{w * 9 // 32} pixels x 4 spp (2x2 stratified) x (camera + 3 diffuse bounces, no Russian "
+                    f"roulette) ({sum(len(x) for x in rays)} rays/step)")
+    del hits
+    segs = [Segment(rtx, ctx, scene, r, claim_start=claim if k == 0 else 0) for k, r in enumerate(rays)]
+    meta = {"arena_mib": round(scene.arena.nbytes / 2**20, 2), "bvh_build": args.build,
+            "bvh_build_s": round(build_s, 3), "mesh_gen_s": round(mesh_s, 3)}
+    return workload, segs, train_rays, scene, meta
+
+
+def setup_c3(rtx, ctx, args):
+    from rtx import workloads
+
+    t0 = time.perf_counter()
+    items = workloads.multi_scene_batch(ctx, n_scenes=64)
+    segs = [Segment(rtx, ctx, scene, rays, scene_id=sid) for sid, scene, rays in items]
+    tris = sum(s.scene.n_triangles for s in segs)
+    n = sum(s.n for s in segs)
+    log(f"c3: 64 scenes, {tris} tris, {n} rays, set up in {time.perf_counter() - t0:.1f} s")
+    workload = (f"64 scenes of instanced displaced spheres, log-uniform 1k-2M tris ({tris} tris in all), "
+                f"log-uniform 4k-1M rays each (50% primary, 25% shadow, 25% diffuse; {n} rays/step), "
+                f"one hipGraph replay of the persistent batch launch")
+    meta = {"arena_mib": round(sum(s.scene.arena.nbytes for s in segs) / 2**20, 2), "scenes": len(segs)}
+    return workload, segs, meta
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -146,12 +291,12 @@ def main():
     ap.add_argument("--learn-epochs", type=int, default=2)
     ap.add_argument("--learn-samples", type=int, default=3_000_000)
     ap.add_argument("--threshold", type=float, default=0.5, help="popularity share for the popularity policy")
-    ap.add_argument("--mode", default=HEADLINE, choices=sorted(MODES))
-    ap.add_argument("--modes", default=DEFAULT_MODES, help="modes to measure")
+    ap.add_argument("--mode", default=None, choices=sorted(MODES), help="headline mode")
+    ap.add_argument("--modes", default=None, help="modes to measure")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--width", type=int, default=0, help="override the image width (height follows the aspect)")
-    ap.add_argument("--workload", default="diffuse", choices=["diffuse", "large", "c1"])
+    ap.add_argument("--workload", default="diffuse", choices=["diffuse", "large", "c3", "c1"])
     ap.add_argument("--build", default="host", choices=["host", "device"],
                     help="BLAS builder: host SAH (default) or the device radix-tree builder")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
@@ -163,29 +308,29 @@ def main():
               "device (rays are independent, so a multi-GPU run would split the batch with the arena "
               "replicated)", file=sys.stderr)
         return 2
+    head_default, modes_default = DEFAULTS[args.workload]
+    head_mode = args.mode or head_default
+    modes = [m for m in (args.modes or modes_default).split(",") if m in MODES]
+    if head_mode not in modes:
+        modes.insert(0, head_mode)
+    if args.workload == "c3":
+        modes = [m for m in modes if MODES[m]["launch"] == "batch"]
+        if MODES[head_mode]["launch"] != "batch":
+            raise SystemExit("--workload c3 runs batch modes only")
 
     import rtx
 
-    if args.workload == "diffuse":
-        w = args.width or 1024
-        desc = c2_desc(rtx, w, w)
-    elif args.workload == "c1":
-        w = args.width or 128
-        desc = c1_desc(rtx, w, w)
-    else:
-        w = args.width or 7680
-        desc = c5_desc(rtx, w, w * 9 // 16)
     ctx = rtx.Context(0)
     info = ctx.device_info()
-    t0 = time.perf_counter()
-    builder = rtx.workload_builder(desc)
-    mesh_s = time.perf_counter() - t0
-    t0 = time.perf_counter()
-    scene = builder.finish(ctx=ctx if args.build == "device" else None)
-    build_s = time.perf_counter() - t0
-    log(f"scene: {scene.n_triangles} tris, {scene.n_blas} BLAS, {scene.arena.nbytes / 2**20:.1f} MiB arena, "
-        f"meshes in {mesh_s:.2f} s, BVH built ({args.build}) in {build_s:.2f} s")
-    ctx.set_scene(scene.arena, scene.tlas_offset)
+    train_rays = scene = None
+    if args.workload == "c3":
+        workload, segs, meta = setup_c3(rtx, ctx, args)
+    else:
+        workload, segs, train_rays, scene, meta = setup_single(rtx, ctx, args, info)
+        if any(MODES[m]["launch"] == "batch" for m in modes):
+            sid = ctx.add_scene(scene.arena, scene.tlas_offset)
+            for s in segs:
+                s.scene_id = sid
     if args.copy_probe:
         a, b = ctx.buffer(args.copy_probe << 20), ctx.buffer(args.copy_probe << 20)
         b.copy_from(a, args.copy_probe << 20)
@@ -193,56 +338,11 @@ def main():
         log(f"copy probe: {args.copy_probe} MiB device to device")
         a.free()
         b.free()
+    n = sum(s.n for s in segs)
 
-    t0 = time.perf_counter()
-    primary = rtx.primary_rays(desc)
-    hits = ctx.trace(primary, mode=0)
-    if args.workload == "diffuse":
-        bounce, _ = scene.secondary_rays(primary, hits, kind=0, seed=99)
-        segments = [np.concatenate([primary, bounce])]
-        log(f"batch: {len(primary)} primary ({int(hits['hit'].sum())} hits) + {len(bounce)} diffuse = "
-            f"{len(segments[0])} rays")
-        train_rays = training_rays_diffuse(scene, primary, hits)
-        workload = (f"1,024 instances x 1,024-tri displaced spheres ({scene.n_triangles} tris), {w}x{w} primary + "
-                    f"1 cosine diffuse bounce per hit ({len(segments[0])} rays/step)")
-    elif args.workload == "c1":
-        segments = [primary]
-        train_rays = primary
-        log(f"batch: {len(primary)} primary rays ({int(hits['hit'].sum())} hits)")
-        workload = (f"displaced UV sphere of 16,384 tris over a 2-tri ground quad ({scene.n_triangles} tris), "
-                    f"{w}x{w} primary rays ({len(primary)} rays/step)")
-    else:
-        # camera rays, then three diffuse bounces from each segment's hits; the
-        # training traces come from an earlier frame (other bounce seeds)
-        segments = [primary]
-        train = [primary[::16]]
-        seg, sh = primary, hits
-        tseg, tsh = primary[::16], hits[::16]
-        for b in range(3):
-            nxt, _ = scene.secondary_rays(seg, sh, kind=0, seed=1000 + b)
-            segments.append(nxt)
-            tn, _ = scene.secondary_rays(tseg, tsh, kind=0, seed=2000 + b)
-            train.append(tn)
-            if b < 2:
-                sh = ctx.trace(nxt, mode=0)
-                tsh = ctx.trace(tn, mode=0)
-            seg, tseg = nxt, tn
-        train_rays = np.concatenate(train)
-        log("segments: " + ", ".join(str(len(x)) for x in segments) + f" rays; set up in "
-            f"{time.perf_counter() - t0:.1f} s")
-        workload = (f"20,480 instances x 1,024-tri displaced spheres ({scene.n_triangles} tris), {w // 2}x"
-                    f"{w * 9 // 32} pixels x 4 spp (2x2 stratified) x (camera + 3 diffuse bounces, no Russian "
-                    f"roulette) ({sum(len(x) for x in segments)} rays/step)")
-    del hits
-    n = sum(len(x) for x in segments)
-    dev = [(ctx.to_device(x), len(x), ctx.buffer(len(x) * rtx.HIT_DTYPE.itemsize)) for x in segments]
-    batch = np.concatenate(segments) if args.workload != "large" else None
-
-    modes = [m for m in args.modes.split(",") if m in MODES]
-    if args.mode not in modes:
-        modes.insert(0, args.mode)
     partitions = {}
     cur_budget = None
+    batch_budget = None
     learned_info = None
     results = {}
     for name in modes:
@@ -251,21 +351,58 @@ def main():
             cfg["budget"] = args.budget
         if args.stage and cfg["stage"]:
             cfg["stage"] = args.stage
-        if (cfg["order"] == 1 or cfg["prefetch"]) and cur_budget != cfg["budget"]:
+        if cfg["launch"] == "trace" and (cfg["order"] == 1 or cfg["prefetch"]) and cur_budget != cfg["budget"]:
             t0 = time.perf_counter()
             partitions[cfg["budget"]] = ctx.build_treelets(cfg["budget"])["n_treelets"]
             cur_budget = cfg["budget"]
             log(f"treelets: {partitions[cur_budget]} at {cur_budget} B budget, built in "
                 f"{time.perf_counter() - t0:.2f} s")
+        if cfg["launch"] == "batch" and cfg["order"] == 1 and batch_budget != cfg["budget"]:
+            t0 = time.perf_counter()
+            for sid in sorted({s.scene_id for s in segs}):
+                ctx.build_scene_treelets(sid, cfg["budget"])
+            batch_budget = cfg["budget"]
+            log(f"batch scenes: treelets at {batch_budget} B built in {time.perf_counter() - t0:.2f} s")
         if cfg["prefetch"] == 4:
             learned_info = train_learned(ctx, scene, train_rays, args)
         kw = dict(mode=cfg["order"], prefetch=cfg["prefetch"], threshold=args.threshold, stage_bytes=cfg["stage"],
                   slot_bytes=cfg["slot"])
+        graph = None
+        rec_bufs = {}
+        if cfg["launch"] == "batch":
+            # the persistent launch claims jobs in order: longest segments first
+            order = sorted(range(len(segs)), key=lambda k: -segs[k].n)
+            jobs = [(segs[k].scene_id, segs[k].rays, segs[k].n, segs[k].hits) for k in order]
+            graph = ctx.batch_graph(jobs, mode=cfg["order"])
+        elif cfg["records"]:
+            for k, s in enumerate(segs):
+                ob = ctx.buffer((s.n + 1) * 8)
+                cap = s.n * 48
+                rb = ctx.buffer(cap * rtx.RECORD_DTYPE.itemsize)
+                st = ctx.trace_device(s.rays, s.n, s.hits, records_buf=rb, record_capacity=cap, offsets_buf=ob,
+                                      claim_start=s.claim_start if cfg["claim"] else 0, **kw)
+                if st == 5:
+                    cap = int(ob.download(np.uint64, s.n + 1)[-1])
+                    rb.free()
+                    rb = ctx.buffer(max(cap, 1) * rtx.RECORD_DTYPE.itemsize)
+                elif st != 0:
+                    raise rtx.RtxError(st, f"rtx_trace {name}")
+                rec_bufs[k] = (rb, cap, ob)
 
         def step():
+            if graph is not None:
+                st, ms = graph.launch(1)
+                if st != 0:
+                    raise rtx.RtxError(st, f"batch graph {name}")
+                return ms
             ms = 0.0
-            for rays_d, k, hits_d in dev:
-                st = ctx.trace_device(rays_d, k, hits_d, **kw)
+            for k, s in enumerate(segs):
+                extra = {}
+                if cfg["records"]:
+                    rb, cap, ob = rec_bufs[k]
+                    extra = dict(records_buf=rb, record_capacity=cap, offsets_buf=ob)
+                st = ctx.trace_device(s.rays, s.n, s.hits, claim_start=s.claim_start if cfg["claim"] else 0,
+                                      **extra, **kw)
                 if st != 0:
                     raise rtx.RtxError(st, f"rtx_trace {name}")
                 ms += ctx.last_kernel_ms()
@@ -280,46 +417,56 @@ def main():
             kms.append(step())
         ctx.synchronize()
         wall = (time.perf_counter() - t0) / args.steps
-        fetch = 0.0
-        nodes = 0.0
+        fetch = nodes = recs = 0.0
         pst = {}
-        for rays_d, k, hits_d in dev:
-            ctx.trace_device(rays_d, k, hits_d, **kw)
-            for key, v in ctx.prefetch_stats().items():
-                pst[key] = pst.get(key, 0) + v
-            h = hits_d.download(rtx.HIT_DTYPE, k)
+        for s in segs:
+            if cfg["launch"] == "trace" and cfg["prefetch"]:
+                ctx.trace_device(s.rays, s.n, s.hits, **kw)
+                for key, v in ctx.prefetch_stats().items():
+                    pst[key] = pst.get(key, 0) + v
+            h = s.hits.download(rtx.HIT_DTYPE, s.n)
             fetch += float(h["fetch_bytes"].astype(np.float64).sum())
             nodes += float(h["n_nodes"].astype(np.float64).sum())
+            recs += float(h["n_records"].astype(np.float64).sum())
+        if graph is not None:
+            graph.close()
+        for rb, _, ob in rec_bufs.values():
+            rb.free()
+            ob.free()
         kernel_ms = float(np.mean(kms))
-        results[name] = {
-            "mrays_s": n / wall / 1e6,
-            "ms_per_step": wall * 1e3,
-            "kernel_ms": kernel_ms,
-            "node_fetch_gbs": fetch / (kernel_ms * 1e-3) / 1e9,
-            "fetch_bytes": fetch,
-            "nodes_per_ray": nodes / n,
-            "config": cfg,
-            "prefetch": pst,
-        }
+        r = {"mrays_s": n / wall / 1e6, "ms_per_step": wall * 1e3, "kernel_ms": kernel_ms,
+             "node_fetch_gbs": fetch / (kernel_ms * 1e-3) / 1e9, "fetch_bytes": fetch, "nodes_per_ray": nodes / n,
+             "config": cfg}
+        if cfg["records"]:
+            r["records_per_ray"] = recs / n
+            r["record_bytes_per_step"] = recs * rtx.RECORD_DTYPE.itemsize
         if cfg["prefetch"]:
             reads = max(pst["node_reads"], 1)
-            results[name]["lds_hit_rate"] = pst["lds_hits"] / reads
-            results[name]["wasted_bytes_per_read"] = pst["wasted_bytes"] / reads
-        log(f"{name:>22}: {results[name]['mrays_s']:8.1f} Mrays/s  kernel {kernel_ms:.3f} ms  "
-            f"node fetch {results[name]['node_fetch_gbs']:.0f} GB/s"
-            + (f"  LDS hit rate {results[name]['lds_hit_rate']:.3f}  wasted {results[name]['wasted_bytes_per_read']:.2f}"
-               f" B/read" if cfg["prefetch"] else ""))
+            r["prefetch"] = pst
+            r["lds_hit_rate"] = pst["lds_hits"] / reads
+            r["wasted_bytes_per_read"] = pst["wasted_bytes"] / reads
+        results[name] = r
+        log(f"{name:>22}: {r['mrays_s']:8.1f} Mrays/s  {r['ms_per_step']:.3f} ms/step  kernel {kernel_ms:.3f} ms  "
+            f"node fetch {r['node_fetch_gbs']:.0f} GB/s"
+            + (f"  {r['records_per_ray']:.1f} records/ray" if cfg["records"] else "")
+            + (f"  LDS hit rate {r['lds_hit_rate']:.3f}  wasted {r['wasted_bytes_per_read']:.2f} B/read"
+               if cfg["prefetch"] else ""))
 
-    head = results[args.mode]
-    # host-buffer rate of the headline mode: upload, trace, download (PCIe-inclusive; not `value`)
-    hcfg = MODES[args.mode]
-    t0 = time.perf_counter()
-    for seg in segments:
-        ctx.trace(seg, mode=hcfg["order"], prefetch=hcfg["prefetch"], stage_bytes=hcfg["stage"],
-                  slot_bytes=hcfg["slot"])
-    pcie_mrays = n / (time.perf_counter() - t0) / 1e6
-    log(f"host buffers in and out ({args.mode}): {pcie_mrays:.1f} Mrays/s")
+    head = results[head_mode]
+    pcie_mrays = None
+    if args.workload != "c3":
+        # host-buffer rate of plain order: upload, trace, download (PCIe-inclusive; not `value`)
+        t0 = time.perf_counter()
+        for s in segs:
+            ctx.trace(s.rays_host, mode=0)
+        pcie_mrays = n / (time.perf_counter() - t0) / 1e6
+        log(f"host buffers in and out (plain): {pcie_mrays:.1f} Mrays/s")
 
+    single_kernel = "trace_kernel"
+    if MODES[head_mode]["launch"] == "trace" and not MODES[head_mode]["stage"] and not MODES[head_mode]["records"]:
+        lim = (128 if MODES[head_mode]["order"] == 0 else 256) * info["n_cus"]
+        if max(s.n for s in segs) <= lim:
+            single_kernel = "trace_wide"
     out = {
         "metric": METRIC,
         "value": round(head["mrays_s"], 3),
@@ -333,50 +480,37 @@ def main():
         "vs_baseline": None,
         "dtype": "fp32",
         "data": "synthetic",
-        "config": {
-            "workload": workload,
-            "mode": args.mode,
-            "partitions": {str(k): v for k, v in partitions.items()},
-            "arena_mib": round(scene.arena.nbytes / 2**20, 2),
-            "bvh_build": args.build,
-            "bvh_build_s": round(build_s, 3),
-            "mesh_gen_s": round(mesh_s, 3),
-            "device": info["name"],
-            "n_cus": info["n_cus"],
-        },
+        "config": {"workload": workload, "mode": head_mode, "partitions": {str(k): v for k, v in partitions.items()},
+                   **meta, "device": info["name"], "n_cus": info["n_cus"]},
         "roofline": {
             "bound": "hbm",
             "achieved": round(head["node_fetch_gbs"], 2),
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": round(head["node_fetch_gbs"] / HBM_PEAK_GBS, 4),
-            "traffic": load_traffic(args.workload, args.mode),
-            # rtx_trace's default kernel choice: eight lanes per ray without staging for
-            # batches of at most 128 (plain) or 256 (treelet) rays per CU; each segment
-            # is one launch
-            "kernel": ("trace_wide" if not MODES[args.mode]["stage"] and max(len(x) for x in segments) <=
-                       (128 if MODES[args.mode]["order"] == 0 else 256) * info["n_cus"] else "trace_kernel"),
+            "traffic": load_traffic(args.workload, head_mode),
+            "kernel": single_kernel,
             "kernel_ms": round(head["kernel_ms"], 4),
-            "bytes_per_launch": head["fetch_bytes"],
+            "bytes_per_launch": head["fetch_bytes"] / (len(segs) if MODES[head_mode]["launch"] == "trace" else 1),
         },
-        "modes": {k: {"mrays_s": round(v["mrays_s"], 3), "kernel_ms": round(v["kernel_ms"], 4),
-                      "node_fetch_gbs": round(v["node_fetch_gbs"], 2), "nodes_per_ray": round(v["nodes_per_ray"], 3),
-                      "config": v["config"], "prefetch": v["prefetch"],
-                      **({"lds_hit_rate": round(v["lds_hit_rate"], 4),
-                          "wasted_bytes_per_read": round(v["wasted_bytes_per_read"], 3)} if "lds_hit_rate" in v else {})}
-                  for k, v in results.items()},
+        "modes": {k: {kk: (round(vv, 4) if isinstance(vv, float) else vv) for kk, vv in v.items()
+                      if kk not in ("fetch_bytes",)} for k, v in results.items()},
         "learned": learned_info,
-        "pcie_inclusive_mrays_s": round(pcie_mrays, 2),
+        "pcie_inclusive_mrays_s": round(pcie_mrays, 2) if pcie_mrays else None,
     }
     if not args.no_cpu_baseline:
         log("cpu baseline ...")
-        if batch is not None:
-            smp, what = batch, ("the whole primary+bounce batch" if args.workload == "diffuse"
-                                else "the whole primary batch")
-        else:
-            smp = np.concatenate([x[:: max(1, len(x) // 250000)] for x in segments])
+        if args.workload == "c3":
+            samples = [(s.scene.arena, s.scene.tlas_offset, s.rays_host[:: max(1, s.n // 4096)]) for s in segs]
+            what = "about 4,096 evenly strided rays of each of the 64 scenes"
+        elif args.workload == "large":
+            samples = [(scene.arena, scene.tlas_offset,
+                        np.concatenate([s.rays_host[:: max(1, s.n // 250000)] for s in segs]))]
             what = "about 250,000 evenly strided rays of each segment"
-        out["cpu_baseline"] = cpu_baseline(scene, smp, what, args.cpu_seconds)
+        else:
+            samples = [(scene.arena, scene.tlas_offset, np.concatenate([s.rays_host for s in segs]))]
+            what = "the whole batch"
+        out["cpu_baseline"] = cpu_baseline(samples, what, args.cpu_seconds)
     print(json.dumps(out), flush=True)
     ctx.close()
     return 0

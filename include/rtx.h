@@ -48,7 +48,9 @@ typedef enum rtx_status {
   RTX_ERR_NO_PARTITION = 10,
   RTX_ERR_UNSUPPORTED = 11,
   RTX_ERR_OUT_OF_MEMORY = 12,
-  RTX_ERR_TIMEOUT = 13
+  RTX_ERR_TIMEOUT = 13,
+  RTX_ERR_STALE = 14                   /* a batch graph whose scenes' partitions were rebuilt, or whose
+                                          context was destroyed, after it was captured */
 } rtx_status;
 
 /* Traversal order. PLAIN follows traceRay, TREELET follows traceRayWithTreelets. */
@@ -83,6 +85,7 @@ enum {
 };
 
 #define RTX_NO_TREELET 0xFFFFFFFFu
+#define RTX_NO_NODE 0xFFFFFFFFFFFFFFFFull
 
 /* One ray: the traceRay operands decoded by trace_ray_impl
  * (src/cuda-sim/instructions.cc:7158-7227).  cull_mask, sbt_* and miss_index
@@ -116,7 +119,13 @@ typedef struct rtx_hit {
   uint32_t fetch_bytes;     /* sum of record sizes, leaf-descriptor records excluded */
   uint32_t n_any_hits;      /* accepted hits of a non-opaque ray (any-hit shader calls) */
   uint32_t max_level;       /* deepest tree level assigned (tree_level_map) */
-} rtx_hit; /* 64 bytes */
+  uint64_t instance_leaf;   /* arena offset of closest_instanceLeaf (vulkan_ray_tracing.cc:2856), the
+                               instance of the last accepted hit; Hit_data's worldToObjectMatrix and
+                               objectToWorldMatrix (:3002-3003) are its rows at +16 and +80.
+                               RTX_NO_NODE when the ray missed */
+  uint32_t accessed_bytes;  /* sum of all record sizes, leaf descriptors included (accessedDataSize, :2260) */
+  uint32_t reserved;
+} rtx_hit; /* 80 bytes */
 
 /* MemoryTransactionRecord (src/abstract_hardware_model.h:315-321) with the
  * node's treelet attached (addrToTreeletID, vulkan_ray_tracing.cc:468-472). */
@@ -133,7 +142,7 @@ typedef struct rtx_trace_options {
   uint32_t mode;         /* RTX_MODE_* */
   uint32_t prefetch;     /* RTX_PREFETCH_* */
   float threshold;       /* POPULARITY share threshold (reference default 0.5, tested 0.75) */
-  uint32_t emit_records; /* 1: fill the record buffer (two passes: count, then write) */
+  uint32_t emit_records; /* 1: fill the record buffer (see record_slab) */
   uint32_t waves_per_cu; /* persistent waves per CU; 0 = as many as fit on a CU at once */
   uint32_t lds_stack;    /* stack entries per lane held in LDS (4, 8, 16); 0 = 8 for PLAIN, 4 for TREELET */
   uint32_t stage_bytes;  /* LDS bytes per wave for a staged treelet (4-64 KiB, power of two);
@@ -152,6 +161,16 @@ typedef struct rtx_trace_options {
                              at most 128 rays per CU (PLAIN) or 256 (TREELET), else 1.  Results
                              are identical.
                              rtx_trace only; batches ignore it. */
+  uint32_t claim_start;  /* persistent waves claim rays starting at this index and wrap around to 0
+                            (outputs stay at input indices).  A batch that holds camera rays followed
+                            by secondary rays traces faster when the longer, incoherent secondary
+                            rays are claimed first: pass the index of the first secondary ray.
+                            Must be < n (0 for an empty batch).  rtx_trace only. */
+  uint32_t record_slab;  /* records, one traversal: each ray's first record_slab records go to a
+                            per-ray slab in library memory and are compacted into the caller's CSR
+                            buffer; only rays with more records are traced a second time.
+                            0 = 64; rtx_trace falls back to a counting pass plus a writing pass
+                            when n * record_slab * 16 bytes exceed 8 GiB */
 } rtx_trace_options;
 
 /* Prefetch accounting of the last rtx_trace call, modelled on the
@@ -231,6 +250,7 @@ rtx_status rtx_trace(rtx_ctx* ctx, const rtx_ray* rays_dev, uint32_t n, const rt
                      uint64_t* record_offsets_dev);
 rtx_status rtx_get_prefetch_stats(rtx_ctx* ctx, rtx_prefetch_stats* out);
 
+
 /* ---- learned next-treelet prefetch -----------------------------------------
  * No counterpart in the reference (its RT unit has popularity heuristics
  * only, src/gpgpu-sim/shader.cc:3419-3559).  RTX_PREFETCH_LEARNED reads a
@@ -305,7 +325,12 @@ rtx_status rtx_trace_batch(rtx_ctx* ctx, const rtx_batch_job* jobs, uint32_t n_j
 /* Capture the batch launch (counter clear + persistent kernel) in a HIP graph. */
 rtx_status rtx_batch_graph_create(rtx_ctx* ctx, const rtx_batch_job* jobs, uint32_t n_jobs,
                                   const rtx_trace_options* opts, rtx_batch_graph** out);
-/* Replay the graph `replays` times on the context's stream, wait, report kernel errors. */
+/* Replay the graph `replays` times on the context's stream, wait, report kernel errors.
+ * A graph holds the device addresses of its scenes' arenas and, in treelet order, of
+ * their partitions.  Rebuilding a partition it uses (rtx_build_scene_treelets,
+ * rtx_autotune_batch_treelets) or destroying the context makes the graph stale:
+ * launching it then returns RTX_ERR_STALE without running anything.  Destroy graphs
+ * before their context; a graph left over after rtx_destroy can still be destroyed. */
 rtx_status rtx_batch_graph_launch(rtx_batch_graph* g, uint32_t replays, float* ms);
 rtx_status rtx_batch_graph_destroy(rtx_batch_graph* g);
 /* Time candidate launch shapes (waves per CU, LDS stack depth, refill threshold)

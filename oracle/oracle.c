@@ -603,11 +603,13 @@ typedef struct {
   uint64_t base; /* first record index of this ray */
   uint32_t count;
   uint32_t fetch_bytes;
+  uint32_t accessed_bytes; /* accessedDataSize += transaction.size (vulkan_ray_tracing.cc:2260) */
   int overflow;
 } sink_t;
 
 static void emit(sink_t* s, uint64_t off, uint32_t size, uint32_t type, uint32_t level) {
   if (type != TX_DESC) s->fetch_bytes += size;
+  s->accessed_bytes += size;
   if (s->out) {
     uint64_t idx = s->base + s->count;
     if (idx < s->cap) {
@@ -686,6 +688,8 @@ static void finish_hit(const uint8_t* A, orc_hit* h, const oray* ray, float min_
     h->bary[0] = b.x;
     h->bary[1] = b.y;
     h->bary[2] = b.z;
+    /* Hit_data.worldToObjectMatrix / objectToWorldMatrix come from this leaf (:3002-3003) */
+    h->instance_leaf = closest_inst;
   }
 }
 
@@ -694,6 +698,7 @@ static int trace_plain(sink_t* s, const uint64_t tlas, const orc_ray* in, orc_hi
   const uint8_t* A = s->A;
   sstack st = {0};
   memset(h, 0, sizeof(*h));
+  h->instance_leaf = ~0ull; /* no closest instance until a hit */
   oray ray;
   ray.o = mk3(in->org[0], in->org[1], in->org[2]);
   ray.d = mk3(in->dir[0], in->dir[1], in->dir[2]);
@@ -886,6 +891,7 @@ static int trace_plain(sink_t* s, const uint64_t tlas, const orc_ray* in, orc_hi
   h->n_records = s->count;
   h->n_nodes = n_nodes;
   h->fetch_bytes = s->fetch_bytes;
+  h->accessed_bytes = s->accessed_bytes;
   h->n_any_hits = n_any;
   h->max_level = ts->max_level;
   free(st.e);
@@ -949,6 +955,7 @@ static int trace_treelet(sink_t* s, const uint64_t tlas, const orc_ray* in, orc_
   const uint8_t* A = s->A;
   const orc_partition* P = s->part;
   memset(h, 0, sizeof(*h));
+  h->instance_leaf = ~0ull; /* no closest instance until a hit */
   cur->head = cur->tail = cur->cap / 2;
   other->head = other->tail = other->cap / 2;
   oray ray;
@@ -1136,6 +1143,7 @@ static int trace_treelet(sink_t* s, const uint64_t tlas, const orc_ray* in, orc_
   h->n_records = s->count;
   h->n_nodes = n_nodes;
   h->fetch_bytes = s->fetch_bytes;
+  h->accessed_bytes = s->accessed_bytes;
   h->n_any_hits = 0;
   h->max_level = ts->max_level;
   return ORC_OK;

@@ -43,6 +43,9 @@ typedef struct orc_hit {
   float p[3];
   float bary[3];
   uint32_t n_records, n_nodes, fetch_bytes, n_any_hits, max_level;
+  uint64_t instance_leaf;  /* closest_instanceLeaf (vulkan_ray_tracing.cc:2856), ~0 on a miss */
+  uint32_t accessed_bytes; /* accessedDataSize of the ray (:2260): every record size */
+  uint32_t reserved;
 } orc_hit;
 
 typedef struct orc_record {

@@ -86,7 +86,7 @@ def test_empty_batch_and_record_overflow(gpu, rtx, c1):
     h, off, rec = gpu.trace(rays[:0], records=True)
     assert len(h) == 0 and len(rec) == 0 and int(off[0]) == 0
     n = 256
-    rb, hb, ob = gpu.to_device(rays[:n]), gpu.buffer(n * 64), gpu.buffer((n + 1) * 8)
+    rb, hb, ob = gpu.to_device(rays[:n]), gpu.buffer(n * rtx.HIT_DTYPE.itemsize), gpu.buffer((n + 1) * 8)
     recb = gpu.buffer(16 * 8)
     st = gpu.trace_device(rb, n, hb, records_buf=recb, record_capacity=8, offsets_buf=ob)
     assert st == 5  # RTX_ERR_TRACE_OVERFLOW: the count pass found more than 8 records
@@ -106,7 +106,7 @@ def test_many_calls_and_errors_reuse_control_state(gpu, rtx, oracle, c1):
     bad = tiny_scene.tiny_arena().copy()
     bad[64 + 12:64 + 16] = np.frombuffer(np.int32(1000).tobytes(), np.uint8)
     ray = tiny_scene.tiny_ray(rtx.RAY_DTYPE)
-    rb, hb = gpu.to_device(sub), gpu.buffer(len(sub) * 64)
+    rb, hb = gpu.to_device(sub), gpu.buffer(len(sub) * rtx.HIT_DTYPE.itemsize)
     for k in range(150):
         if k in (40, 41, 100):
             gpu.set_scene(bad, 0)

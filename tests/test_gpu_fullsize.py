@@ -87,3 +87,113 @@ def test_large_workload_sample(gpu, rtx, oracle):
     hb = gpu.trace(b1)
     _props(b1, hb)
     _sample_check(gpu, oracle, scene, b1, hb, 0, n=3000, seed=2, records=False)
+
+
+def _exact_props(rays, hits):
+    """Every ray, bit for bit where the reference's arithmetic fixes it: a hit's
+    point is origin + direction * t in FP32 (separate multiply and add,
+    vulkan_ray_tracing.cc:2997), a miss leaves the hit fields zero and names no
+    instance leaf."""
+    _props(rays, hits)
+    ok = hits["hit"] == 1
+    t = hits["t"][ok].astype(np.float32)
+    p = (rays["org"][ok].astype(np.float32) + rays["dir"][ok].astype(np.float32) * t[:, None]).astype(np.float32)
+    assert np.array_equal(hits["p"][ok].view(np.uint32), p.view(np.uint32))
+    miss = ~ok
+    assert np.all(hits["t"][miss] == 0) and np.all(hits["instance_leaf"][miss] == np.uint64(0xFFFFFFFFFFFFFFFF))
+    assert np.all(hits["accessed_bytes"] >= hits["fetch_bytes"])
+
+
+def test_c3_multi_scene_batch_at_full_size(rtx, oracle):
+    """64 scenes with log-uniform 1k-2M triangles and log-uniform 4k-1M mixed
+    rays (primary / shadow / diffuse) in one hipGraph replay of the persistent
+    batch launch, both traversal orders: 2,000 seeded rays of every scene bit
+    for bit against the oracle, exact properties over every ray."""
+    from rtx import workloads
+
+    ctx = rtx.Context(0)
+    try:
+        items = workloads.multi_scene_batch(ctx)
+        jobs = []
+        for sid, scene, rays in items:
+            jobs.append((sid, ctx.to_device(rays), len(rays), ctx.buffer(len(rays) * rtx.HIT_DTYPE.itemsize)))
+        n_rays = sum(j[2] for j in jobs)
+        n_tris = sum(s.n_triangles for _, s, _ in items)
+        print(f"c3: {len(items)} scenes, {n_tris} triangles, {n_rays} rays", flush=True)
+        assert len(items) == 64 and n_tris > 5_000_000 and n_rays > 2_000_000
+        results = {}
+        for mode in (0, 1):
+            if mode == 1:
+                for k, (sid, scene, _) in enumerate(items):
+                    ctx.build_scene_treelets(sid, 8192)
+            order = sorted(range(len(jobs)), key=lambda k: -jobs[k][2])
+            g = ctx.batch_graph([jobs[k] for k in order], mode=mode)
+            st, ms = g.launch(1)
+            g.close()
+            assert st == 0
+            print(f"c3 mode {mode}: one replay in {ms:.3f} ms", flush=True)
+            rng = np.random.default_rng(100 + mode)
+            out = []
+            for k, ((sid, scene, rays), (_, _, n, hb)) in enumerate(zip(items, jobs)):
+                h = hb.download(rtx.HIT_DTYPE, n)
+                _exact_props(rays, h)
+                idx = np.sort(rng.choice(n, size=min(2000, n), replace=False))
+                part = oracle.Partition(scene.arena, scene.tlas_offset, 8192) if mode == 1 else None
+                ref = oracle.trace(scene.arena, scene.tlas_offset, rays[idx], mode=mode, partition=part,
+                                   records=False)
+                assert_hits_equal(h[idx], ref, f"c3 scene {k}, mode {mode}")
+                out.append(h)
+            results[mode] = out
+        # both orders find the same closest hit for opaque rays that do not stop at their first hit
+        for (sid, scene, rays), h0, h1 in zip(items, results[0], results[1]):
+            full = rays["flags"] == 1
+            assert np.array_equal(h0["t"][full].view(np.uint32), h1["t"][full].view(np.uint32))
+    finally:
+        for _, rb, _, hb in locals().get("jobs", []):
+            rb.free()
+            hb.free()
+        ctx.close()
+
+
+def test_c5_full_frame_in_one_batch_launch(rtx, oracle):
+    """The large workload at its full size: 21M triangles, 3840x2160 pixels x 4
+    samples, camera rays plus three diffuse bounces (53M rays), traced as one
+    persistent batch launch over the four segments and as four rtx_trace
+    launches.  Both give the same bits for every ray; 2,000 seeded rays of
+    every segment match the oracle bit for bit."""
+    import bench
+
+    ctx = rtx.Context(0)
+    try:
+        d = bench.c5_desc(rtx)
+        scene = rtx.workload_scene(d)
+        print(f"c5: {scene.n_triangles} triangles, arena {scene.arena.nbytes / 2**30:.2f} GiB", flush=True)
+        ctx.set_scene(scene.arena, scene.tlas_offset)
+        segs = [rtx.primary_rays(d)]
+        hits = [ctx.trace(segs[0])]
+        for b in range(3):
+            nxt, _ = scene.secondary_rays(segs[-1], hits[-1], kind=0, seed=1000 + b)
+            segs.append(nxt)
+            hits.append(ctx.trace(nxt))
+        sizes = [len(s) for s in segs]
+        print(f"c5 segments: {sizes} rays", flush=True)
+        assert sizes[0] == 3840 * 2160 * 4 and sum(sizes) > 45_000_000
+        sid = ctx.add_scene(scene.arena, scene.tlas_offset)
+        bufs = [(ctx.to_device(s), ctx.buffer(len(s) * rtx.HIT_DTYPE.itemsize)) for s in segs]
+        jobs = [(sid, rb, len(s), hb) for s, (rb, hb) in zip(segs, bufs)]
+        order = sorted(range(4), key=lambda k: -sizes[k])
+        assert ctx.trace_batch([jobs[k] for k in order], mode=0) == 0
+        ms = ctx.last_kernel_ms()
+        print(f"c5 batch launch: {ms:.2f} ms, {sum(sizes) / ms / 1e3:.0f} Mrays/s", flush=True)
+        for k, (s, (rb, hb)) in enumerate(zip(segs, bufs)):
+            hb_hits = hb.download(rtx.HIT_DTYPE, len(s))
+            assert hb_hits.tobytes() == hits[k].tobytes(), f"segment {k}: batch launch differs from rtx_trace"
+            _exact_props(s, hits[k])
+            rng = np.random.default_rng(k)
+            idx = np.sort(rng.choice(len(s), size=2000, replace=False))
+            ref = oracle.trace(scene.arena, scene.tlas_offset, s[idx], mode=0, records=False)
+            assert_hits_equal(hits[k][idx], ref, f"c5 segment {k}")
+            rb.free()
+            hb.free()
+    finally:
+        ctx.close()

@@ -3,6 +3,8 @@ both orders and with one and eight lanes per ray: degenerate triangles (the
 Möller-Trumbore test has no determinant epsilon, so 1/det can be infinite),
 sheared / mirrored / non-uniformly scaled instances, extreme coordinate
 scales, and empty or negative ray intervals."""
+import types
+
 import numpy as np
 import pytest
 
@@ -138,6 +140,46 @@ def test_ray_intervals(gpu, rtx, oracle, c1):
     r["tmax"][3::5] = 0.0
     r["tmax"][4::5] = 2.5
     _check_all(gpu, oracle, scene, r, "ray intervals")
+
+
+def _overflowing_tlas_arena():
+    """The hand-assembled tiny arena with the TLAS root's x origin at 3e38 and
+    x exponent 127: its child box decodes to x in [3e38, +inf] (q 128 * 2^119
+    + 3e38 overflows), and the TLAS header's x bounds become [3e38, +inf] too."""
+    import struct
+    from golden import tiny_scene
+
+    A = bytearray(tiny_scene.tiny_arena().tobytes())
+    struct.pack_into("<f", A, 8, 3e38)          # header BoundsMin.x
+    struct.pack_into("<f", A, 20, np.inf)       # header BoundsMax.x
+    struct.pack_into("<f", A, 64, 3e38)         # TLAS root Origin.x
+    struct.pack_into("<b", A, 64 + 18, 127)     # TLAS root exponent x
+    return np.frombuffer(bytes(A), np.uint8).copy()
+
+
+def test_infinite_direction_against_overflowing_bounds(gpu, rtx, oracle):
+    """Rays with infinite direction components (idir 0 on that axis) against a
+    TLAS node whose x bound decodes to +inf: (inf - o) * 0 is NaN, which the
+    reference's compare-selects drop and a sign-picked near bound would not.
+    The child boxes are ordered (lo <= hi), so only the ray decides which slab
+    test runs."""
+    A = _overflowing_tlas_arena()
+    rng = np.random.default_rng(21)
+    n = 2048
+    r = np.zeros(n, rtx.RAY_DTYPE)
+    comp = np.array([np.inf, -np.inf, 0.0, 1.0, -1.0, 0.5], np.float32)
+    d = comp[rng.integers(0, len(comp), (n, 3))]
+    d[:, 2] = np.where(rng.random(n) < 0.5, 1.0, d[:, 2])
+    r["dir"] = d
+    r["org"] = np.stack([rng.uniform(-0.5, 0.5, n), rng.uniform(-0.5, 2.5, n), rng.uniform(-1, 4, n)], 1)
+    r["tmin"] = rng.choice(np.array([-10.0, 0.0], np.float32), n)
+    r["tmax"] = rng.choice(np.array([-0.5, 100.0, 1e30], np.float32), n)
+    r["flags"] = 1
+    r[0] = tuple([(0.25, 0.25, 2.0), -10.0, (np.inf, 0.0, 1.0), -0.5, 1, 0xff, 0, 0, 0, (0, 0, 0)])
+    _check_all(gpu, oracle, types.SimpleNamespace(arena=A, tlas_offset=0), r, "infinite directions")
+    # the hand-derived ray enters the instance leaf (its TLAS child box is hit at thit = -1)
+    h0, o0, r0 = oracle.trace(A, 0, r[:1], mode=0)
+    assert 2 in set(r0["type"].tolist()), "hand-derived ray should visit the instance leaf"
 
 
 @pytest.mark.parametrize("mode", [0, 1])

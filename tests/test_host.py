@@ -17,7 +17,7 @@ def _declared(header):
     return sorted(set(re.findall(r"\b(rtx_[a-z0-9_]+)\s*\(", text)))
 
 
-@pytest.mark.parametrize("header", ["rtx.h", "rtx_scene.h", "rtx_formats.h"])
+@pytest.mark.parametrize("header", ["rtx.h", "rtx_scene.h", "rtx_formats.h", "rtx_adapter.h"])
 def test_library_exports_every_declared_symbol(rtx, header):
     lib = ctypes.CDLL(rtx.LIB_PATH)
     names = _declared(header)
@@ -28,7 +28,7 @@ def test_library_exports_every_declared_symbol(rtx, header):
 
 
 def test_status_strings(rtx):
-    for s in range(14):
+    for s in range(15):
         assert rtx._lib.rtx_status_string(s)
     assert rtx.version().startswith("rtx-mi355x")
 

@@ -91,6 +91,24 @@ def test_ray_box_axis_parallel_and_nan(oracle):
     assert hit and th == 2.0
 
 
+def test_ray_box_infinite_direction_against_overflowing_bound(oracle):
+    """An infinite direction component gives idir 0 on that axis; a bound that
+    decoded to +inf then makes an (inf - o) * 0 = NaN slab product.  Derived by
+    hand from magic_max7 / magic_min7 (VRT:183-197) and MIN/MAX (VRTH:53-56):
+      x: lo (3e38 - 0) * 0 = 0, hi (inf - 0) * 0 = NaN
+         min side MAX(MIN(0, NaN), -10): 0 < NaN is false -> NaN; NaN > -10 is false -> -10
+         max side MIN(MAX(0, NaN), -0.5): 0 > NaN is false -> NaN; NaN < -0.5 is false -> -0.5
+      y: idir 2^80: lo -2^79, hi 2^79 -> min side -10, max side -0.5
+      z: lo (1 - 2) = -1, hi (5 - 2) = 3 -> min side MAX(-1, -10) = -1, max side MIN(3, -0.5) = -0.5
+    so the box is hit at thit = -1.  IEEE max/min with the near bound picked by
+    the sign of idir (+0: near = lo) would keep the x product 0 and miss
+    (0 > -0.5)."""
+    idir = oracle.calculate_idir([np.inf, 0.0, 1.0])
+    assert idir[0] == 0.0 and idir[1] == 2.0 ** 80 and idir[2] == 1.0
+    hit, th = oracle.ray_box_test([3e38, 0.0, 1.0], [np.inf, 1.0, 5.0], idir, [0.0, 0.5, 2.0], -10.0, -0.5)
+    assert hit and th == -1.0
+
+
 def _mt64(p0, p1, p2, o, d):
     e1, e2 = p1 - p0, p2 - p0
     m = np.array([-d, e1, e2]).T
@@ -202,6 +220,10 @@ def _check_tiny(oracle, mode, case, partition=None):
         else:
             assert h[0][k] == v, k
     assert h[0]["n_records"] == len(exp["records"])
+    # accessedDataSize (VRT:2260) sums every hand-derived record, descriptors included
+    assert h[0]["accessed_bytes"] == sum(size for _, size, _ in exp["records"])
+    # the closest hit's instance: the arena's only instance leaf (offset 128); none on a miss
+    assert h[0]["instance_leaf"] == (128 if exp["hit"].get("hit", 0) else 0xFFFFFFFFFFFFFFFF)
     return rec
 
 
@@ -360,6 +382,34 @@ def test_brute_force_nearest_hit_instanced(oracle, instanced, instanced_shared):
         hit_ref = np.isfinite(t_ref)
         assert np.array_equal(h["hit"].astype(bool), hit_ref)
         assert np.allclose(h["t"][hit_ref], t_ref[hit_ref], rtol=1e-5, atol=1e-6)
+
+
+def test_hit_instance_leaf_maps_the_object_hit_to_the_world_hit(oracle, instanced):
+    """instance_leaf names the instance whose matrices Hit_data carries
+    (VRT:2856, 3002-3003): its object->world rows (+80) take the object-space
+    hit point, the barycentric combination of the closest leaf's vertices (the
+    last QUAD_LEAF_HIT record), to the world hit point o + d t.  In plain order
+    every accepted hit overwrites the closest leaf while t only shrinks
+    (VRT:2850-2859), so there the check takes rays with one accepted hit."""
+    scene, rays = instanced
+    A = scene.arena
+    for mode, part in ((0, None), (1, oracle.Partition(A, scene.tlas_offset, 1024))):
+        h, off, rec = oracle.trace(A, scene.tlas_offset, rays, mode=mode, partition=part)
+        n_acc = np.add.reduceat((rec["type"] == 5).astype(np.int64), off[:-1].astype(np.int64))
+        idx = np.nonzero(h["hit"] & ((n_acc == 1) if mode == 0 else True))[0]
+        assert len(idx) > 200
+        for i in idx[:: max(1, len(idx) // 400)]:
+            r = rec[off[i]:off[i + 1]]
+            leaf = int(r["offset"][np.nonzero(r["type"] == 5)[0][-1]])
+            v = np.frombuffer(A[leaf + 16: leaf + 52].tobytes(), np.float32).reshape(3, 3).astype(np.float64)
+            bv, bw, bu = h["bary"][i].astype(np.float64)
+            obj = bu * v[0] + bv * v[1] + bw * v[2]
+            il = int(h["instance_leaf"][i])
+            rows = np.frombuffer(A[il + 80: il + 128].tobytes(), np.float32).reshape(4, 3).astype(np.float64)
+            world = obj @ rows[:3] + rows[3]
+            assert np.allclose(world, h["p"][i], rtol=1e-4, atol=1e-4), (mode, i, world, h["p"][i])
+            assert np.frombuffer(A[il + 72: il + 76].tobytes(), np.uint32)[0] == h["instance_index"][i]
+        assert (h["instance_leaf"][h["hit"] == 0] == 0xFFFFFFFFFFFFFFFF).all() or mode == 1
 
 
 def test_plain_and_treelet_agree_on_t(oracle, c1):

@@ -60,6 +60,12 @@ struct rtx_ctx {
   std::unordered_map<uint64_t, uint32_t> occupancy;
   uint64_t* scan_tmp = nullptr;
   size_t scan_tmp_entries = 0;
+  // slab records (one traversal with records): per-ray slabs and the list of
+  // rays whose records did not fit
+  rtx_record* slabs = nullptr;
+  size_t slab_entries = 0;
+  uint32_t* overflow_list = nullptr;
+  size_t overflow_entries = 0;
   rtx_prefetch_stats last_stats{};
   // learned prefetch table of the current partition
   uint4* pred_table = nullptr;
@@ -71,18 +77,24 @@ struct rtx_ctx {
     uint64_t tlas = 0;
     bool ordered = false;  // boxes_ordered() of the arena
     Partition part;
+    uint64_t part_gen = 0;  // bumped whenever the scene's partition is rebuilt
   };
   std::vector<SceneSlot> scenes;
   BatchJob* jobs_dev = nullptr;
+  // batch graphs captured on this context (detached by rtx_destroy)
+  std::vector<rtx_batch_graph*> graphs;
 };
 
 struct rtx_batch_graph {
-  rtx_ctx* ctx = nullptr;
+  rtx_ctx* ctx = nullptr;  // null once the context is destroyed
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
   BatchJob* jobs = nullptr;
   uint64_t* spill = nullptr;
   uint32_t* counters = nullptr;
+  // treelet-order graphs read their scenes' partitions: (scene, partition
+  // generation) at capture time
+  std::vector<std::pair<uint32_t, uint64_t>> deps;
 };
 
 namespace {
@@ -139,14 +151,28 @@ rtx_status ensure_spill(rtx_ctx* c, size_t entries) {
 // traversal mode (no traversal mode reaches it)
 constexpr uint32_t kWideOccupancyKey = 0xff00u;
 
-uint32_t cached_blocks_per_cu(rtx_ctx* c, uint32_t mode, uint32_t lds_stack, uint32_t stage) {
-  const uint64_t key = ((uint64_t)mode << 48) | ((uint64_t)lds_stack << 32) | stage;
+uint32_t cached_blocks_per_cu(rtx_ctx* c, uint32_t mode, uint32_t lds_stack, uint32_t stage, int records = kRecNone) {
+  const uint64_t key = ((uint64_t)mode << 48) | ((uint64_t)records << 40) | ((uint64_t)lds_stack << 32) | stage;
   auto it = c->occupancy.find(key);
   if (it != c->occupancy.end()) return it->second;
   const uint32_t b = mode >= kWideOccupancyKey ? trace_wide_blocks_per_cu(mode - kWideOccupancyKey)
-                                               : trace_blocks_per_cu(mode, false, lds_stack, stage);
+                                               : trace_blocks_per_cu(mode, records, lds_stack, stage);
   c->occupancy[key] = b;
   return b;
+}
+
+template <typename T>
+rtx_status ensure_buffer(T*& buf, size_t& have, size_t entries) {
+  if (entries <= have) return RTX_OK;
+  if (buf) HIP_TRY(hipFree(buf));
+  buf = nullptr;
+  have = 0;
+  if (hipMalloc(&buf, entries * sizeof(T)) != hipSuccess) {
+    (void)hipGetLastError();
+    return RTX_ERR_OUT_OF_MEMORY;
+  }
+  have = entries;
+  return RTX_OK;
 }
 
 rtx_status ensure_scan_tmp(rtx_ctx* c, size_t entries) {
@@ -198,6 +224,7 @@ const char* rtx_status_string(rtx_status s) {
     case RTX_ERR_UNSUPPORTED: return "unsupported input";
     case RTX_ERR_OUT_OF_MEMORY: return "out of memory";
     case RTX_ERR_TIMEOUT: return "timeout";
+    case RTX_ERR_STALE: return "batch graph is stale (partition rebuilt or context destroyed)";
   }
   return "unknown status";
 }
@@ -266,10 +293,13 @@ rtx_status rtx_destroy(rtx_ctx* c) {
   if (c->slots) hipFree(c->slots);
   if (c->err_host) hipHostFree(c->err_host);
   if (c->scan_tmp) hipFree(c->scan_tmp);
+  if (c->slabs) hipFree(c->slabs);
+  if (c->overflow_list) hipFree(c->overflow_list);
   for (auto& sc : c->scenes) {
     if (sc.arena) hipFree(sc.arena);
     partition_free(sc.part);
   }
+  for (rtx_batch_graph* g : c->graphs) g->ctx = nullptr;  // launches now report RTX_ERR_STALE
   if (c->jobs_dev) hipFree(c->jobs_dev);
   if (c->ev0) hipEventDestroy(c->ev0);
   if (c->ev1) hipEventDestroy(c->ev1);
@@ -456,6 +486,8 @@ rtx_status rtx_trace(rtx_ctx* c, const rtx_ray* rays, uint32_t n, const rtx_trac
   if (opts->mode != RTX_MODE_PLAIN && opts->mode != RTX_MODE_TREELET) return RTX_ERR_INVALID_ARGUMENT;
   if (opts->prefetch > RTX_PREFETCH_LEARNED) return RTX_ERR_INVALID_ARGUMENT;
   if (opts->emit_records && !offsets) return RTX_ERR_INVALID_ARGUMENT;
+  if (n > 0 ? opts->claim_start >= n : opts->claim_start != 0) return RTX_ERR_INVALID_ARGUMENT;
+  if (opts->record_slab > 1024) return RTX_ERR_INVALID_ARGUMENT;
   if (!c->arena) return RTX_ERR_NO_SCENE;
   if ((opts->mode == RTX_MODE_TREELET || opts->prefetch != RTX_PREFETCH_NONE) && !c->part.valid)
     return RTX_ERR_NO_PARTITION;
@@ -496,6 +528,12 @@ rtx_status rtx_trace(rtx_ctx* c, const rtx_ray* rays, uint32_t n, const rtx_trac
   if (opts->lanes_per_ray != 0 && opts->lanes_per_ray != 1 && opts->lanes_per_ray != kWideRayLanes)
     return RTX_ERR_INVALID_ARGUMENT;
   const bool wide_ok = !opts->emit_records && stage == 0;
+  // records in one traversal: per-ray slabs of `slab` records, compacted into
+  // the caller's CSR buffer; only rays with more records are traced again
+  const uint32_t slab = opts->record_slab ? opts->record_slab : 64;
+  const bool use_slab = opts->emit_records && (uint64_t)n * slab * sizeof(rtx_record) <= (8ull << 30) &&
+                        (uint64_t)n * slab < (1ull << 31);
+  const int main_rec = !opts->emit_records ? kRecNone : use_slab ? kRecSlab : kRecNone;
   if (opts->lanes_per_ray == kWideRayLanes && !wide_ok) return RTX_ERR_INVALID_ARGUMENT;
   const bool wide = wide_ok && (opts->lanes_per_ray == kWideRayLanes ||
                                 (opts->lanes_per_ray == 0 &&
@@ -504,7 +542,7 @@ rtx_status rtx_trace(rtx_ctx* c, const rtx_ray* rays, uint32_t n, const rtx_trac
   // persistent blocks: by default as many as fit on a CU at once
   uint32_t blocks_per_cu = opts->waves_per_cu ? std::max(1u, opts->waves_per_cu * 64 / bs)
                            : wide             ? cached_blocks_per_cu(c, kWideOccupancyKey + opts->mode, 0, 0)
-                                              : cached_blocks_per_cu(c, opts->mode, lds_stack, stage);
+                                              : cached_blocks_per_cu(c, opts->mode, lds_stack, stage, main_rec);
   if (blocks_per_cu == 0) blocks_per_cu = 16 * 64 / bs;
   uint32_t grid = (uint32_t)c->n_cus * blocks_per_cu;
   const uint64_t lanes = (uint64_t)n * (wide ? kWideRayLanes : 1u);
@@ -531,6 +569,8 @@ rtx_status rtx_trace(rtx_ctx* c, const rtx_ray* rays, uint32_t n, const rtx_trac
   p.records = nullptr;
   p.record_capacity = 0;
   p.record_offsets = nullptr;
+  p.claim_start = opts->claim_start;
+  p.ray_list = nullptr;
   p.owner = c->part.valid ? c->part.owner : nullptr;
   p.first_treelet = c->part.valid ? c->part.first_treelet : 0;
   p.spill = c->spill;
@@ -580,15 +620,25 @@ rtx_status rtx_trace(rtx_ctx* c, const rtx_ray* rays, uint32_t n, const rtx_trac
     return kernel_error_status(err);
   };
   c->kernel_timed = false;
+  if (use_slab) {
+    st = ensure_buffer(c->slabs, c->slab_entries, (size_t)n * slab);
+    if (st != RTX_OK) return st;
+    p.records = c->slabs;
+    p.record_slab = slab;
+  }
   st = use_slot();
   if (st != RTX_OK) return st;
+  uint32_t* main_slot = p.ray_counter;
   HIP_TRY(hipEventRecord(c->evk0, c->stream));
   if (wide)
     HIP_TRY(launch_trace_wide(p, opts->mode, lds_stack + spill_cap, grid, c->stream));
   else
-    HIP_TRY(launch_trace(p, opts->mode, false, lds_stack, stage, grid, c->stream));
+    HIP_TRY(launch_trace(p, opts->mode, main_rec, lds_stack, stage, grid, c->stream));
   HIP_TRY(hipEventRecord(c->evk1, c->stream));
   c->kernel_timed = true;
+  // staging statistics of the main traversal (a second pass traces some rays again)
+  unsigned long long stats[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  if (stage) HIP_TRY(hipMemcpyAsync(stats, p.stats, sizeof(stats), hipMemcpyDeviceToHost, c->stream));
   if (opts->emit_records) {
     const size_t tiles = (n + 1023) / 1024;
     st = ensure_scan_tmp(c, tiles + 1);
@@ -597,19 +647,40 @@ rtx_status rtx_trace(rtx_ctx* c, const rtx_ray* rays, uint32_t n, const rtx_trac
     uint64_t total = 0;
     HIP_TRY(hipMemcpyAsync(&total, offsets + n, sizeof(uint64_t), hipMemcpyDeviceToHost, c->stream));
     HIP_TRY(hipStreamSynchronize(c->stream));
-    c->slots_clean = true;
     st = launch_error();
     if (st != RTX_OK) return st;
-    if (total > capacity || (total > 0 && !records)) return RTX_ERR_TRACE_OVERFLOW;
+    if (total > capacity || (total > 0 && !records)) {
+      c->slots_clean = true;
+      return RTX_ERR_TRACE_OVERFLOW;
+    }
     p.records = records;
     p.record_capacity = capacity;
     p.record_offsets = offsets;
-    st = use_slot();
-    if (st != RTX_OK) return st;
-    HIP_TRY(launch_trace(p, opts->mode, true, lds_stack, stage, grid, c->stream));
+    uint32_t second = n;  // rays the writing pass traces
+    if (use_slab) {
+      // slab records to their CSR places; word 2 of the main launch's control
+      // slot (cleared with it) counts the rays whose records overflowed
+      st = ensure_buffer(c->overflow_list, c->overflow_entries, (size_t)n);
+      if (st != RTX_OK) return st;
+      HIP_TRY(launch_compact_records(hits, n, c->slabs, slab, offsets, records, c->overflow_list, main_slot + 2,
+                                     c->stream));
+      HIP_TRY(hipMemcpyAsync(&second, main_slot + 2, sizeof(uint32_t), hipMemcpyDeviceToHost, c->stream));
+      HIP_TRY(hipStreamSynchronize(c->stream));
+      p.ray_list = c->overflow_list;
+      p.n = second;
+      p.claim_start = 0;
+      p.record_slab = 0;
+    }
+    c->slots_clean = true;
+    if (second > 0) {
+      // the rays again, every record to its CSR place (hits are rewritten
+      // with the same values)
+      const uint32_t g2 = std::max(1u, (uint32_t)std::min<uint64_t>(grid, ((uint64_t)second + bs - 1) / bs));
+      st = use_slot();
+      if (st != RTX_OK) return st;
+      HIP_TRY(launch_trace(p, opts->mode, kRecCsr, lds_stack, stage, g2, c->stream));
+    }
   }
-  unsigned long long stats[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-  if (stage) HIP_TRY(hipMemcpyAsync(stats, p.stats, sizeof(stats), hipMemcpyDeviceToHost, c->stream));
   HIP_TRY(hipStreamSynchronize(c->stream));
   c->slots_clean = true;
   c->last_stats.node_reads = stats[0];
@@ -771,6 +842,7 @@ rtx_status rtx_build_scene_treelets(rtx_ctx* c, uint32_t id, uint32_t budget, rt
   if (budget < 128 || budget % 64 != 0 || budget > (1u << 20)) return RTX_ERR_INVALID_ARGUMENT;
   HIP_TRY(hipSetDevice(c->device));
   auto& sc = c->scenes[id];
+  sc.part_gen++;  // graphs captured against the old partition are stale, also if the build fails
   rtx_status st = partition_build(sc.arena, sc.bytes, sc.tlas, budget, c->stream, sc.part);
   if (st != RTX_OK) return st;
   if (out) *out = sc.part.summary;
@@ -821,6 +893,9 @@ rtx_status rtx_batch_graph_create(rtx_ctx* c, const rtx_batch_job* jobs, uint32_
   rtx_batch_graph* g = new (std::nothrow) rtx_batch_graph();
   if (!g) return RTX_ERR_OUT_OF_MEMORY;
   g->ctx = c;
+  c->graphs.push_back(g);
+  if (o->mode == RTX_MODE_TREELET)
+    for (uint32_t j = 0; j < n_jobs; ++j) g->deps.emplace_back(jobs[j].scene, c->scenes[jobs[j].scene].part_gen);
   auto fail = [&](rtx_status e) {
     rtx_batch_graph_destroy(g);
     return e;
@@ -849,6 +924,9 @@ rtx_status rtx_batch_graph_create(rtx_ctx* c, const rtx_batch_job* jobs, uint32_
 rtx_status rtx_batch_graph_launch(rtx_batch_graph* g, uint32_t replays, float* ms) {
   if (!g || !g->exec || replays == 0) return RTX_ERR_INVALID_ARGUMENT;
   rtx_ctx* c = g->ctx;
+  if (!c) return RTX_ERR_STALE;
+  for (const auto& d : g->deps)
+    if (c->scenes[d.first].part_gen != d.second) return RTX_ERR_STALE;
   HIP_TRY(hipSetDevice(c->device));
   HIP_TRY(hipEventRecord(c->ev0, c->stream));
   for (uint32_t r = 0; r < replays; ++r) HIP_TRY(hipGraphLaunch(g->exec, c->stream));
@@ -863,8 +941,10 @@ rtx_status rtx_batch_graph_launch(rtx_batch_graph* g, uint32_t replays, float* m
 rtx_status rtx_batch_graph_destroy(rtx_batch_graph* g) {
   if (!g) return RTX_ERR_INVALID_ARGUMENT;
   if (g->ctx) {
-    hipSetDevice(g->ctx->device);
-    hipStreamSynchronize(g->ctx->stream);
+    rtx_ctx* c = g->ctx;
+    hipSetDevice(c->device);
+    hipStreamSynchronize(c->stream);
+    c->graphs.erase(std::remove(c->graphs.begin(), c->graphs.end(), g), c->graphs.end());
   }
   if (g->exec) hipGraphExecDestroy(g->exec);
   if (g->graph) hipGraphDestroy(g->graph);

@@ -74,9 +74,12 @@ __device__ __forceinline__ bool finite3(F3 v) {
   return __builtin_isfinite(v.x) && __builtin_isfinite(v.y) && __builtin_isfinite(v.z);
 }
 
-// A ray whose slab products can never be NaN (given a non-NaN node origin).
-__device__ __forceinline__ bool slab_inputs_ok(F3 o, F3 idir, float tmin, float tmax) {
-  return finite3(o) && finite3(idir) && !__builtin_isnan(tmin) && !__builtin_isnan(tmax);
+// A ray whose slab products can never be NaN (given a non-NaN node origin):
+// a finite origin, and a finite direction, so that no inverse-direction
+// component is zero (|idir| lies in [2^-128, 2^80]) and an infinite bound
+// times idir stays infinite instead of becoming 0 * inf.
+__device__ __forceinline__ bool slab_inputs_ok(F3 o, F3 d, F3 idir, float tmin, float tmax) {
+  return finite3(o) && finite3(d) && finite3(idir) && !__builtin_isnan(tmin) && !__builtin_isnan(tmax);
 }
 
 // mt_ray_triangle_test, vulkan_ray_tracing.cc:3089-3111

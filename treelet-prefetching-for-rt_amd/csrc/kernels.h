@@ -13,10 +13,17 @@ enum : uint32_t {
   kErrType = 2u,       // malformed node type
   kErrRange = 4u,      // node outside the arena
   kErrRecords = 8u,    // record buffer overflow
-  kErrInstSlot = 16u,  // instance leaf beyond the 256 MiB the stack entry can name
+  kErrInstSlot = 16u,  // treelet order: an instance leaf 128 MiB or more away from the TLAS header
   kErrSteps = 32u,     // per-ray step limit (cyclic or corrupt BVH)
   kErrBudget = 64u,    // treelet node larger than the budget
   kErrQueue = 128u     // partitioner work buffer overflow
+};
+
+// Record output of a trace kernel.
+enum : int {
+  kRecNone = 0,  // hits only
+  kRecCsr = 1,   // records at record_offsets[ray]
+  kRecSlab = 2   // a ray's first record_slab records at ray * record_slab
 };
 
 constexpr uint32_t kBlock = 256;        // threads per block (4 waves of 64)
@@ -49,6 +56,9 @@ struct TraceParams {
   rtx_record* records;
   uint64_t record_capacity;
   const uint64_t* record_offsets;
+  uint32_t record_slab;    // slab records: ray r's first record_slab records at records[r * record_slab]
+  uint32_t claim_start;    // rays are claimed from this index on, wrapping to 0
+  const uint32_t* ray_list;  // nullable: claim k traces ray ray_list[k] (n = list length)
   const uint32_t* owner;  // node slot -> dense treelet index (nullable in plain mode)
   uint32_t first_treelet; // dense index of the treelet rooted at the TLAS header
   uint64_t* spill;        // global stack spill, spill_cap entries per stack per lane
@@ -84,13 +94,13 @@ constexpr uint32_t kMaxStageSlots = 16;
 
 // Launchers (defined in the .hip files).
 uint32_t trace_block_size(uint32_t stage_bytes);
-hipError_t launch_trace(const TraceParams& p, uint32_t mode, bool records, uint32_t lds_stack, uint32_t stage_bytes,
+hipError_t launch_trace(const TraceParams& p, uint32_t mode, int records, uint32_t lds_stack, uint32_t stage_bytes,
                         uint32_t grid, hipStream_t s);
 hipError_t launch_trace_batch(const TraceParams& p, uint32_t mode, uint32_t lds_stack, uint32_t grid,
                               hipStream_t s);
 // Workgroups of a trace kernel that fit on one CU at once (registers and LDS);
 // 0 when the configuration has no kernel.
-uint32_t trace_blocks_per_cu(uint32_t mode, bool records, uint32_t lds_stack, uint32_t stage_bytes);
+uint32_t trace_blocks_per_cu(uint32_t mode, int records, uint32_t lds_stack, uint32_t stage_bytes);
 uint32_t trace_batch_blocks_per_cu(uint32_t mode, uint32_t lds_stack);
 // Cooperative kernel for small batches (eight lanes per ray, no records or
 // staging), plain or treelet order: `cap` entries per list and ray (at most
@@ -102,5 +112,11 @@ uint32_t trace_wide_blocks_per_cu(uint32_t mode);
 hipError_t launch_fill_zero(uint32_t* dst, uint32_t words, hipStream_t s);
 hipError_t launch_scan_records(const rtx_hit* hits, uint32_t n, uint64_t* offsets, uint64_t* block_tmp,
                                hipStream_t s);
+// Copy each ray's slab records (at most `slab`) to its CSR position; rays with
+// more than `slab` records are appended to overflow_list (count in
+// overflow_count, which must start at 0).
+hipError_t launch_compact_records(const rtx_hit* hits, uint32_t n, const rtx_record* slabs, uint32_t slab,
+                                  const uint64_t* offsets, rtx_record* records, uint32_t* overflow_list,
+                                  uint32_t* overflow_count, hipStream_t s);
 
 }  // namespace rtxk

@@ -683,6 +683,8 @@ rtx_status build_images(const uint8_t* A, uint64_t bytes, hipStream_t s, const u
   for (uint32_t v : ib) p.max_image_bytes = std::max(p.max_image_bytes, v);
   CS(dmalloc(&p.images, total));
   CS(dmalloc(&p.image_slots, total / 64));
+  // units no node claims stay "none", so staging never inserts them
+  if (total) CK(hipMemsetAsync(p.image_slots, 0xff, total / 64 * 4, s));
   hipLaunchKernelGGL(image_copy, dim3(g), dim3(256), 0, s, p.n, p.list_offsets, p.list_nodes, p.owner, kind,
                      p.stage_slot, p.image_offset, A, p.images, p.image_slots);
   CK(hipGetLastError());
@@ -885,6 +887,16 @@ rtx_status partition_install(const uint8_t* A, uint64_t bytes, uint64_t tlas, ui
   const uint64_t n_list = lo[n];
   for (uint64_t i = 0; i < n_list; ++i)
     if (ln[i] % 64 || ln[i] >= bytes) return RTX_ERR_INVALID_ARGUMENT;
+  // Each list must be de-duplicated (VRT:1312-1356 removes repeats): a node
+  // listed twice would get two image positions, and image_layout keeps only one.
+  {
+    std::vector<uint64_t> sorted;
+    for (uint32_t i = 0; i < n; ++i) {
+      sorted.assign(ln + lo[i], ln + lo[i + 1]);
+      std::sort(sorted.begin(), sorted.end());
+      if (std::adjacent_find(sorted.begin(), sorted.end()) != sorted.end()) return RTX_ERR_INVALID_ARGUMENT;
+    }
+  }
   uint32_t ft = RTX_NO_TREELET;
   for (uint32_t i = 0; i < n; ++i)
     if (roots[i] == tlas) ft = i;

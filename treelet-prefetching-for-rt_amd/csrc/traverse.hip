@@ -50,6 +50,7 @@
 // also stages the treelets a prediction table names as the lanes' likely next
 // ones.  Staging changes where node bytes are read from, never which nodes
 // are visited.
+#include <algorithm>
 #include <type_traits>
 
 #include "device_common.h"
@@ -64,14 +65,30 @@ constexpr uint32_t kFlTop = 1u, kFlLeaf = 2u;
 constexpr uint32_t kNoSlot = 0xffffffffu;
 constexpr uint32_t kNone = 0xffffffffu;
 
-__device__ __forceinline__ uint64_t make_entry(uint32_t slot, uint32_t flags, uint32_t level, uint32_t inst) {
+// A stack entry: node slot (32 bits), flags (2), tree level (8) and the
+// instance leaf that opened the entry (22 bits, treelet order only).  The
+// instance is stored as its slot's distance from the TLAS header slot, a
+// signed 22-bit value, so instance leaves may lie anywhere within 128 MiB of
+// the TLAS header whatever the arena's size.
+constexpr int32_t kInstRelMax = 1 << 21;
+__device__ __forceinline__ uint64_t make_entry(uint32_t slot, uint32_t flags, uint32_t level, uint32_t inst_rel) {
   uint32_t lv = level > 255u ? 255u : level;
-  return (uint64_t)slot | ((uint64_t)(flags | (lv << 2) | (inst << 10)) << 32);
+  return (uint64_t)slot | ((uint64_t)(flags | (lv << 2) | ((inst_rel & 0x3fffffu) << 10)) << 32);
 }
 __device__ __forceinline__ uint32_t e_slot(uint64_t e) { return (uint32_t)e; }
 __device__ __forceinline__ uint32_t e_flags(uint64_t e) { return (uint32_t)(e >> 32) & 3u; }
 __device__ __forceinline__ uint32_t e_level(uint64_t e) { return ((uint32_t)(e >> 32) >> 2) & 255u; }
-__device__ __forceinline__ uint32_t e_inst(uint64_t e) { return (uint32_t)(e >> 42); }
+__device__ __forceinline__ uint32_t e_inst_rel(uint64_t e) { return (uint32_t)(e >> 42); }
+// absolute instance-leaf slot of an entry, given the TLAS header slot
+__device__ __forceinline__ uint32_t e_inst(uint64_t e, uint32_t tlas_slot) {
+  return tlas_slot + (uint32_t)(((int32_t)(e_inst_rel(e) << 10)) >> 10);
+}
+// the relative field of instance-leaf slot `inst`, or false when it is out of range
+__device__ __forceinline__ bool inst_rel_of(uint32_t inst, uint32_t tlas_slot, uint32_t& rel) {
+  const int64_t d = (int64_t)inst - (int64_t)tlas_slot;
+  rel = (uint32_t)d & 0x3fffffu;
+  return d >= -kInstRelMax && d < kInstRelMax;
+}
 
 struct Stack {
   uint32_t bot, cnt, g;
@@ -111,6 +128,28 @@ __device__ __forceinline__ uint32_t lanes_below(uint64_t mask) {
 
 __device__ __forceinline__ uint32_t slot_hash(uint32_t s) { return s * 2654435761u; }
 
+// Ray traced by claim k of a single-scene launch: a listed ray, else rays in
+// input order starting at claim_start and wrapping to 0.
+__device__ __forceinline__ uint32_t claimed_ray(const TraceParams& p, uint32_t k) {
+  if (p.ray_list) return p.ray_list[k];
+  const uint32_t tail = p.n - p.claim_start;  // rays from claim_start to the end
+  return k < tail ? k + p.claim_start : k - tail;
+}
+
+// The reference's accessedDataSize of one ray (vulkan_ray_tracing.cc:2260):
+// every record size, the 8-byte leaf descriptors included.  It follows from
+// the counters a ray keeps.  With a internal nodes, b instance leaves and c
+// primitive leaves (the TLAS header record emitted):
+//   fetch = 64 + 64a + 192b + 64c,  nodes = a + b + c,  records = 1 + a + 2b + 2c
+// so b = (fetch/64 - 1 - nodes) / 2, c = records - 1 - nodes - b, and the
+// descriptors add 8c bytes.
+__device__ __forceinline__ uint32_t accessed_bytes_of(uint32_t n_rec, uint32_t n_nodes, uint32_t fetch) {
+  if (n_rec == 0) return 0u;
+  const uint32_t b = ((fetch >> 6) - 1u - n_nodes) >> 1;
+  const uint32_t c = n_rec - 1u - n_nodes - b;
+  return fetch + 8u * c;
+}
+
 __device__ __forceinline__ uint32_t wave_sum(uint32_t v) {
   for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
   return v;
@@ -118,8 +157,8 @@ __device__ __forceinline__ uint32_t wave_sum(uint32_t v) {
 
 // Non-staged kernels are compiled for five waves per SIMD (at most 96 VGPRs);
 // staged kernels are bounded by their LDS image instead.
-template <int MODE, bool REC, int K, int STAGE, bool MULTI = false>
-__global__ __launch_bounds__(STAGE ? 64 : 256) __attribute__((amdgpu_waves_per_eu(STAGE || REC ? 1 : 5)))
+template <int MODE, int REC, int K, int STAGE, bool MULTI = false>
+__global__ __launch_bounds__(STAGE ? 64 : 256) __attribute__((amdgpu_waves_per_eu(STAGE ? 1 : REC ? 4 : 5)))
 void trace_kernel(TraceParams p) {
   static_assert(!MULTI || (!REC && STAGE == 0), "batch kernels write hits only");
   constexpr uint32_t BS = STAGE ? 64u : 256u;
@@ -161,7 +200,6 @@ void trace_kernel(TraceParams p) {
   Stack sa{0, 0, 0}, sb{0, 0, 0};
   bool active = false, exhausted = false;
   bool pending = false;  // the lane's completed ray still has to write its hit record
-  uint32_t ray_id = 0;
   uint32_t err = 0;
 
   F3 ro{}, rd{}, ridir{};
@@ -209,7 +247,9 @@ void trace_kernel(TraceParams p) {
     if (type != RTX_TX_BVH_PRIMITIVE_LEAF_DESCRIPTOR) fetch_bytes += size;
     if (REC) {
       uint64_t idx = rec_base + n_rec;
-      if (idx < p.record_capacity) {
+      // a slab keeps the ray's first record_slab records; the rest are written
+      // by a second pass over the rays that had more
+      if (REC == kRecSlab ? n_rec < p.record_slab : idx < p.record_capacity) {
         uint32_t tl = owner ? owner[off >> 6] : RTX_NO_TREELET;
         uint4 v;
         v.x = (uint32_t)off;
@@ -217,7 +257,7 @@ void trace_kernel(TraceParams p) {
         v.z = tl;
         v.w = size | (type << 16) | ((lvl > 255u ? 255u : lvl) << 24);
         reinterpret_cast<uint4*>(p.records)[idx] = v;
-      } else {
+      } else if (REC == kRecCsr) {
         err |= kErrRecords;
       }
     }
@@ -233,6 +273,8 @@ void trace_kernel(TraceParams p) {
     if (!ok) err |= kErrStack;
   };
 
+  // TLAS header slot of the lane's scene (instance leaves are stored relative to it)
+  auto tlas_slot = [&]() -> uint32_t { return (uint32_t)(tlas >> 6); };
   // Treelet order: does `slot` belong to the current ("first") treelet?
   auto goes_current = [&](uint32_t slot) -> bool {
     if (MODE == 0) return true;
@@ -304,7 +346,7 @@ void trace_kernel(TraceParams p) {
                    __uint_as_float(b.x), __uint_as_float(b.y), __uint_as_float(b.z), __uint_as_float(b.w),
                    __uint_as_float(c.x), __uint_as_float(c.y), __uint_as_float(c.z), __uint_as_float(c.w)};
     cur = transform_ray(m, ro, rd, rtmin, rtmax);
-    cur_ok = slab_inputs_ok(cur.o, cur.idir, cur.tmin, cur.tmax);
+    cur_ok = slab_inputs_ok(cur.o, cur.d, cur.idir, cur.tmin, cur.tmax);
     cur_inst = inst;
   };
 
@@ -437,6 +479,7 @@ void trace_kernel(TraceParams p) {
   // rays instead of in nearly every iteration.
   auto write_hit = [&]() {
     uint4 o0 = make_uint4(0, 0, 0, 0), o1 = make_uint4(0, 0, 0, 0), o2 = make_uint4(0, 0, 0, 0);
+    uint4 o4 = make_uint4(0xffffffffu, 0xffffffffu, 0, 0);
     if (err == 0 && min_thit < rtmax && closest_leaf != kNoSlot) {
       const Words16 q = load64B(A + ((uint64_t)closest_leaf << 6));
       const uint32_t inst_id = *reinterpret_cast<const uint32_t*>(A + ((uint64_t)closest_inst << 6) + 72);
@@ -458,14 +501,18 @@ void trace_kernel(TraceParams p) {
       o1 = make_uint4(__float_as_uint(min_thit), __float_as_uint(ip.x), __float_as_uint(ip.y),
                       __float_as_uint(ip.z));
       o2 = make_uint4(__float_as_uint(bc.x), __float_as_uint(bc.y), __float_as_uint(bc.z), n_rec);
+      o4.x = closest_inst << 6;
+      o4.y = closest_inst >> 26;
     } else {
       o2.w = n_rec;
     }
+    o4.z = accessed_bytes_of(n_rec, n_nodes, fetch_bytes);
     uint4* hq = reinterpret_cast<uint4*>(hits_out + local_id);
     hq[0] = o0;
     hq[1] = o1;
     hq[2] = o2;
     hq[3] = make_uint4(n_nodes, fetch_bytes, n_any, max_level);
+    hq[4] = o4;
     if (err) {
       atomicOr(p.error, err);
       if (p.host_err) *reinterpret_cast<volatile uint32_t*>(p.host_err) = 1u;
@@ -494,9 +541,9 @@ void trace_kernel(TraceParams p) {
       base = __shfl(base, (int)leader, 64);
       if ((uint64_t)base + nidle >= p.n) exhausted = true;
       if (!active) {
-        const uint32_t r = base + lanes_below(idle);
-        if (r < p.n) {
-          ray_id = r;
+        const uint32_t k = base + lanes_below(idle);
+        if (k < p.n) {
+          const uint32_t r = MULTI ? k : claimed_ray(p, k);
           active = true;
           local_id = r;
           if constexpr (MULTI) {
@@ -527,7 +574,7 @@ void trace_kernel(TraceParams p) {
           rtmax = __uint_as_float(r1.w);
           rflags = r2.x;
           ridir = calc_idir(rd);
-          ray_ok = slab_inputs_ok(ro, ridir, rtmin, rtmax);
+          ray_ok = slab_inputs_ok(ro, rd, ridir, rtmin, rtmax);
           octant = (rd.x < 0.f ? 1u : 0u) | (rd.y < 0.f ? 2u : 0u) | (rd.z < 0.f ? 4u : 0u);
           min_thit = rtmax;
           min_thit_obj = 0.f;
@@ -536,7 +583,7 @@ void trace_kernel(TraceParams p) {
           cur_inst = kNoSlot;
           n_rec = n_nodes = fetch_bytes = n_any = 0;
           max_level = 1;  // tree_level_map[topRootAddr] = 1
-          rec_base = REC ? p.record_offsets[r] : 0;
+          rec_base = REC == kRecCsr ? p.record_offsets[r] : REC == kRecSlab ? (uint64_t)r * p.record_slab : 0;
           sa = Stack{0, 0, 0};
           sb = Stack{0, 0, 0};
           phase2 = false;
@@ -678,7 +725,7 @@ void trace_kernel(TraceParams p) {
     // ---- process the entry ----------------------------------------------------
     if (step) {
       has_e = false;
-      if (MODE == 1 && !(flags & kFlTop) && e_inst(e) != cur_inst) load_instance(e_inst(e));
+      if (MODE == 1 && !(flags & kFlTop) && e_inst(e, tlas_slot()) != cur_inst) load_instance(e_inst(e, tlas_slot()));
       // lanes on internal nodes (their own entry, or one reached through a
       // leaf in this iteration) share one internal-node pass at the end
       bool inner = err == 0 && !(flags & kFlLeaf);
@@ -695,6 +742,9 @@ void trace_kernel(TraceParams p) {
             if (pos != kNone && (pos & ((1u << cap_shift) - 1)) + 1 < sl_units[pos >> cap_shift]) ipos = pos;
           uint4 w4[4];
           read_words(off, ipos, 16, w4, 4);
+          // bytes 16..79 span both units of the leaf
+          if constexpr (STAGE > 0)
+            if (ipos != kNone) used[ipos + 1] = 1;
           const uint4 a = w4[0], b = w4[1], c = w4[2], d = w4[3];
           float m[12] = {__uint_as_float(a.x), __uint_as_float(a.y), __uint_as_float(a.z), __uint_as_float(a.w),
                          __uint_as_float(b.x), __uint_as_float(b.y), __uint_as_float(b.z), __uint_as_float(b.w),
@@ -709,9 +759,10 @@ void trace_kernel(TraceParams p) {
             const uint2 ro2 = *reinterpret_cast<const uint2*>(A + blas);
             emit(blas, 64, RTX_TX_BVH_STRUCTURE, level);
             cur = transform_ray(m, ro, rd, rtmin, rtmax);
-            cur_ok = slab_inputs_ok(cur.o, cur.idir, cur.tmin, cur.tmax);
+            cur_ok = slab_inputs_ok(cur.o, cur.d, cur.idir, cur.tmin, cur.tmax);
             cur_inst = slot;
-            if (slot >= (1u << 22)) err |= kErrInstSlot;
+            uint32_t irel = 0;
+            if (MODE == 1 && !inst_rel_of(slot, tlas_slot(), irel)) err |= kErrInstSlot;
             const uint64_t root = blas + ((uint64_t)ro2.x | ((uint64_t)ro2.y << 32));
             if ((root & 63) || root + 64 > bytes) {
               err |= kErrRange;
@@ -726,10 +777,10 @@ void trace_kernel(TraceParams p) {
               // tested in this iteration's internal-node pass instead.
               if (err == 0 && (MODE == 0 || to_a || st_empty(sa))) {
                 if (MODE == 1 && !to_a) phase2 = true;  // current_treelet_root := host address (:1748-1754)
-                e = make_entry(rs, 0u, level, slot);
+                e = make_entry(rs, 0u, level, irel);
                 inner = true;
               } else {
-                push_entry(make_entry(rs, 0u, level, slot), to_a);
+                push_entry(make_entry(rs, 0u, level, irel), to_a);
               }
             }
           }
@@ -792,7 +843,7 @@ void trace_kernel(TraceParams p) {
         const uint32_t flags = e_flags(e);
         const uint32_t level = e_level(e);
         const uint64_t off = (uint64_t)slot << 6;
-        if (MODE == 1 && !(flags & kFlTop) && e_inst(e) != cur_inst) load_instance(e_inst(e));
+        if (MODE == 1 && !(flags & kFlTop) && e_inst(e, tlas_slot()) != cur_inst) load_instance(e_inst(e, tlas_slot()));
         if (err == 0) {
           // ---- internal node (top :2499-2599 / bottom :2686-2786) ----------
           if (off + 64 > bytes) {
@@ -896,7 +947,7 @@ void trace_kernel(TraceParams p) {
             if (hitmask) {
               const uint32_t lvl = level + 1;
               if (lvl > max_level) max_level = lvl;
-              const uint32_t inst = top ? 0u : e_inst(e);
+              const uint32_t inst = top ? 0u : e_inst_rel(e);
               const uint32_t base_flags = top ? kFlTop : 0u;
               int deferred = -1;
               uint32_t deferred_cs = 0;
@@ -1103,6 +1154,7 @@ __global__ __launch_bounds__(kWideBlock) void trace_wide(TraceParams p, uint32_t
     if (n_rec > kMaxStepsPerRay) err |= kErrSteps;
     return true;
   };
+  auto tlas_slot = [&]() -> uint32_t { return (uint32_t)(tlas >> 6); };
   auto goes_current = [&](uint32_t slot) -> bool {
     if (MODE == 0) return true;
     if (phase2) return false;
@@ -1124,12 +1176,13 @@ __global__ __launch_bounds__(kWideBlock) void trace_wide(TraceParams p, uint32_t
                          __uint_as_float(b.x), __uint_as_float(b.y), __uint_as_float(b.z), __uint_as_float(b.w),
                          __uint_as_float(c.x), __uint_as_float(c.y), __uint_as_float(c.z), __uint_as_float(c.w)};
     cur = transform_ray(m, ro, rd, rtmin, rtmax);
-    cur_ok = slab_inputs_ok(cur.o, cur.idir, cur.tmin, cur.tmax);
+    cur_ok = slab_inputs_ok(cur.o, cur.d, cur.idir, cur.tmin, cur.tmax);
     cur_inst = inst;
   };
   // the hit record (as trace_kernel writes it): lanes 0-3 store one 16-byte word each
   auto write_hit = [&]() {
     uint4 o0 = make_uint4(0, 0, 0, 0), o1 = make_uint4(0, 0, 0, 0), o2 = make_uint4(0, 0, 0, 0);
+    uint4 o4 = make_uint4(0xffffffffu, 0xffffffffu, 0, 0);
     if (err == 0 && min_thit < rtmax && closest_leaf != kNoSlot) {
       const Words16 q = load64B(A + ((uint64_t)closest_leaf << 6));
       const uint32_t inst_id = *reinterpret_cast<const uint32_t*>(A + ((uint64_t)closest_inst << 6) + 72);
@@ -1149,9 +1202,12 @@ __global__ __launch_bounds__(kWideBlock) void trace_wide(TraceParams p, uint32_t
       o1 = make_uint4(__float_as_uint(min_thit), __float_as_uint(ip.x), __float_as_uint(ip.y),
                       __float_as_uint(ip.z));
       o2 = make_uint4(__float_as_uint(bc.x), __float_as_uint(bc.y), __float_as_uint(bc.z), n_rec);
+      o4.x = closest_inst << 6;
+      o4.y = closest_inst >> 26;
     } else {
       o2.w = n_rec;
     }
+    o4.z = accessed_bytes_of(n_rec, n_nodes, fetch_bytes);
     uint4* hq = reinterpret_cast<uint4*>(p.hits + local_id);
     if (j == 0)
       hq[0] = o0;
@@ -1161,6 +1217,8 @@ __global__ __launch_bounds__(kWideBlock) void trace_wide(TraceParams p, uint32_t
       hq[2] = o2;
     else if (j == 3)
       hq[3] = make_uint4(n_nodes, fetch_bytes, n_any, max_level);
+    else if (j == 4)
+      hq[4] = o4;
     if (err && j == 0) {
       atomicOr(p.error, err);
       if (p.host_err) *reinterpret_cast<volatile uint32_t*>(p.host_err) = 1u;
@@ -1189,8 +1247,9 @@ __global__ __launch_bounds__(kWideBlock) void trace_wide(TraceParams p, uint32_t
       base = __shfl(base, (int)leader, 64);
       if ((uint64_t)base + nidle >= p.n) exhausted = true;
       if (!active) {
-        const uint32_t r = base + (uint32_t)__popcll(idle & ((1ull << gbase) - 1ull));
-        if (r < p.n) {
+        const uint32_t k = base + (uint32_t)__popcll(idle & ((1ull << gbase) - 1ull));
+        if (k < p.n) {
+          const uint32_t r = claimed_ray(p, k);
           active = true;
           local_id = r;
           const uint4* rq = reinterpret_cast<const uint4*>(p.rays + r);
@@ -1201,7 +1260,7 @@ __global__ __launch_bounds__(kWideBlock) void trace_wide(TraceParams p, uint32_t
           rtmax = __uint_as_float(r1.w);
           rflags = r2.x;
           ridir = calc_idir(rd);
-          ray_ok = slab_inputs_ok(ro, ridir, rtmin, rtmax);
+          ray_ok = slab_inputs_ok(ro, rd, ridir, rtmin, rtmax);
           min_thit = rtmax;
           min_thit_obj = 0.f;
           closest_leaf = kNoSlot;
@@ -1256,7 +1315,7 @@ __global__ __launch_bounds__(kWideBlock) void trace_wide(TraceParams p, uint32_t
     bool inner = false;
     if (step) {
       has_e = false;
-      if (MODE == 1 && !(flags & kFlTop) && e_inst(e) != cur_inst) load_instance(e_inst(e));
+      if (MODE == 1 && !(flags & kFlTop) && e_inst(e, tlas_slot()) != cur_inst) load_instance(e_inst(e, tlas_slot()));
       if (err != 0) {
         // an instance reload out of the arena completes the ray next iteration
       } else if (!(flags & kFlLeaf)) {
@@ -1281,9 +1340,10 @@ __global__ __launch_bounds__(kWideBlock) void trace_wide(TraceParams p, uint32_t
             const uint2 ro2 = *reinterpret_cast<const uint2*>(A + blas);
             emit(64, RTX_TX_BVH_STRUCTURE);
             cur = transform_ray(m, ro, rd, rtmin, rtmax);
-            cur_ok = slab_inputs_ok(cur.o, cur.idir, cur.tmin, cur.tmax);
+            cur_ok = slab_inputs_ok(cur.o, cur.d, cur.idir, cur.tmin, cur.tmax);
             cur_inst = slot;
-            if (slot >= (1u << 22)) err |= kErrInstSlot;
+            uint32_t irel = 0;
+            if (MODE == 1 && !inst_rel_of(slot, tlas_slot(), irel)) err |= kErrInstSlot;
             const uint64_t root = blas + ((uint64_t)ro2.x | ((uint64_t)ro2.y << 32));
             if ((root & 63) || root + 64 > bytes) {
               err |= kErrRange;
@@ -1294,10 +1354,10 @@ __global__ __launch_bounds__(kWideBlock) void trace_wide(TraceParams p, uint32_t
               // next pop takes (see trace_kernel); otherwise it is pushed
               if (err == 0 && (MODE == 0 || to_a || da == 0)) {
                 if (MODE == 1 && !to_a) phase2 = true;
-                e = make_entry((uint32_t)(root >> 6), 0u, level, slot);
+                e = make_entry((uint32_t)(root >> 6), 0u, level, irel);
                 has_e = true;
               } else {
-                push(make_entry((uint32_t)(root >> 6), 0u, level, slot), to_a);
+                push(make_entry((uint32_t)(root >> 6), 0u, level, irel), to_a);
               }
             }
           }
@@ -1383,7 +1443,7 @@ __global__ __launch_bounds__(kWideBlock) void trace_wide(TraceParams p, uint32_t
         if (hitmask) {
           const uint32_t lvl = level + 1;
           if (lvl > max_level) max_level = lvl;
-          const uint32_t inst = top ? 0u : e_inst(e);
+          const uint32_t inst = top ? 0u : e_inst_rel(e);
           const uint32_t base_flags = top ? kFlTop : 0u;
           // Lane j pushes hit child j itself, at the position the per-lane
           // kernel's push sequence gives it: hit children in slot order (in
@@ -1431,7 +1491,7 @@ __global__ __launch_bounds__(kWideBlock) void trace_wide(TraceParams p, uint32_t
 
 using TraceFn = void (*)(TraceParams);
 
-template <int MODE, bool REC>
+template <int MODE, int REC>
 TraceFn pick_k(uint32_t lds_stack, uint32_t stage) {
   if (stage == 0) {
     switch (lds_stack) {
@@ -1451,10 +1511,35 @@ TraceFn pick_k(uint32_t lds_stack, uint32_t stage) {
   }
 }
 
-TraceFn pick_trace(uint32_t mode, bool records, uint32_t lds_stack, uint32_t stage) {
-  if (mode == RTX_MODE_PLAIN)
-    return records ? pick_k<0, true>(lds_stack, stage) : pick_k<0, false>(lds_stack, stage);
-  return records ? pick_k<1, true>(lds_stack, stage) : pick_k<1, false>(lds_stack, stage);
+template <int MODE>
+TraceFn pick_rec(int records, uint32_t lds_stack, uint32_t stage) {
+  switch (records) {
+    case kRecNone: return pick_k<MODE, kRecNone>(lds_stack, stage);
+    case kRecCsr: return pick_k<MODE, kRecCsr>(lds_stack, stage);
+    case kRecSlab: return pick_k<MODE, kRecSlab>(lds_stack, stage);
+    default: return nullptr;
+  }
+}
+
+TraceFn pick_trace(uint32_t mode, int records, uint32_t lds_stack, uint32_t stage) {
+  return mode == RTX_MODE_PLAIN ? pick_rec<0>(records, lds_stack, stage) : pick_rec<1>(records, lds_stack, stage);
+}
+
+// ---- slab records -> CSR ----------------------------------------------------
+// Thread t copies record t % slab of ray t / slab when the ray has that many;
+// the thread of a ray's record `slab` (one past the slab) lists the ray for
+// the second pass instead.
+__global__ __launch_bounds__(256) void compact_records(const rtx_hit* hits, uint32_t n, const uint4* slabs,
+                                                       uint32_t slab, const uint64_t* offsets, uint4* records,
+                                                       uint32_t* overflow_list, uint32_t* overflow_count) {
+  // the host keeps n * slab below 2^31
+  const uint32_t total = n * slab;
+  for (uint32_t t = blockIdx.x * blockDim.x + threadIdx.x; t < total; t += gridDim.x * blockDim.x) {
+    const uint32_t r = t / slab, k = t - r * slab;
+    const uint32_t cnt = hits[r].n_records;
+    if (k < cnt) records[offsets[r] + k] = slabs[t];
+    if (k == 0 && cnt > slab) overflow_list[atomicAdd(overflow_count, 1u)] = r;
+  }
 }
 
 TraceFn pick_batch(uint32_t mode, uint32_t lds_stack) {
@@ -1505,6 +1590,17 @@ hipError_t launch_fill_zero(uint32_t* dst, uint32_t words, hipStream_t s) {
 
 uint32_t trace_block_size(uint32_t stage_bytes) { return stage_bytes ? 64u : 256u; }
 
+hipError_t launch_compact_records(const rtx_hit* hits, uint32_t n, const rtx_record* slabs, uint32_t slab,
+                                  const uint64_t* offsets, rtx_record* records, uint32_t* overflow_list,
+                                  uint32_t* overflow_count, hipStream_t s) {
+  if (n == 0 || slab == 0) return hipSuccess;
+  const uint64_t total = (uint64_t)n * slab;
+  const uint32_t grid = (uint32_t)std::min<uint64_t>(65536, (total + 255) / 256);
+  hipLaunchKernelGGL(compact_records, dim3(grid), dim3(256), 0, s, hits, n, reinterpret_cast<const uint4*>(slabs),
+                     slab, offsets, reinterpret_cast<uint4*>(records), overflow_list, overflow_count);
+  return hipGetLastError();
+}
+
 hipError_t launch_trace_wide(const TraceParams& p, uint32_t mode, uint32_t cap, uint32_t grid, hipStream_t s) {
   if (cap > kWideCap) return hipErrorInvalidValue;
   if (mode == RTX_MODE_PLAIN)
@@ -1522,12 +1618,12 @@ uint32_t trace_wide_blocks_per_cu(uint32_t mode) {
   return nb > 0 ? (uint32_t)nb : 0u;
 }
 
-hipError_t launch_trace(const TraceParams& p, uint32_t mode, bool records, uint32_t lds_stack, uint32_t stage_bytes,
+hipError_t launch_trace(const TraceParams& p, uint32_t mode, int records, uint32_t lds_stack, uint32_t stage_bytes,
                         uint32_t grid, hipStream_t s) {
   return launch_fn(pick_trace(mode, records, lds_stack, stage_bytes), p, grid, trace_block_size(stage_bytes), s);
 }
 
-uint32_t trace_blocks_per_cu(uint32_t mode, bool records, uint32_t lds_stack, uint32_t stage_bytes) {
+uint32_t trace_blocks_per_cu(uint32_t mode, int records, uint32_t lds_stack, uint32_t stage_bytes) {
   return blocks_per_cu(pick_trace(mode, records, lds_stack, stage_bytes), trace_block_size(stage_bytes));
 }
 

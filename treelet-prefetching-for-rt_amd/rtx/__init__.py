@@ -28,7 +28,7 @@ STATUS_NAMES = {
     0: "RTX_OK", 1: "RTX_ERR_INVALID_ARGUMENT", 2: "RTX_ERR_BAD_NODE_TYPE", 3: "RTX_ERR_OFFSET_OUT_OF_ARENA",
     4: "RTX_ERR_NODE_LARGER_THAN_BUDGET", 5: "RTX_ERR_TRACE_OVERFLOW", 6: "RTX_ERR_STACK_OVERFLOW",
     7: "RTX_ERR_HIP", 8: "RTX_ERR_NO_DEVICE", 9: "RTX_ERR_NO_SCENE", 10: "RTX_ERR_NO_PARTITION",
-    11: "RTX_ERR_UNSUPPORTED", 12: "RTX_ERR_OUT_OF_MEMORY", 13: "RTX_ERR_TIMEOUT",
+    11: "RTX_ERR_UNSUPPORTED", 12: "RTX_ERR_OUT_OF_MEMORY", 13: "RTX_ERR_TIMEOUT", 14: "RTX_ERR_STALE",
 }
 MODE_PLAIN, MODE_TREELET = 0, 1
 PREFETCH_NONE, PREFETCH_ALWAYS, PREFETCH_POPULARITY, PREFETCH_PARTIAL, PREFETCH_LEARNED = 0, 1, 2, 3, 4
@@ -44,10 +44,12 @@ RAY_DTYPE = np.dtype([("org", "<f4", 3), ("tmin", "<f4"), ("dir", "<f4", 3), ("t
 HIT_DTYPE = np.dtype([("hit", "<u4"), ("primitive_index", "<u4"), ("geometry_index", "<u4"),
                       ("instance_index", "<u4"), ("t", "<f4"), ("p", "<f4", 3), ("bary", "<f4", 3),
                       ("n_records", "<u4"), ("n_nodes", "<u4"), ("fetch_bytes", "<u4"), ("n_any_hits", "<u4"),
-                      ("max_level", "<u4")])
+                      ("max_level", "<u4"), ("instance_leaf", "<u8"), ("accessed_bytes", "<u4"),
+                      ("reserved", "<u4")])
 RECORD_DTYPE = np.dtype([("offset", "<u8"), ("treelet", "<u4"), ("size", "<u2"), ("type", "u1"),
                          ("level", "u1")])
-assert RAY_DTYPE.itemsize == 64 and HIT_DTYPE.itemsize == 64 and RECORD_DTYPE.itemsize == 16
+assert RAY_DTYPE.itemsize == 64 and HIT_DTYPE.itemsize == 80 and RECORD_DTYPE.itemsize == 16
+NO_NODE = 0xFFFFFFFFFFFFFFFF
 
 
 class RtxError(RuntimeError):
@@ -61,7 +63,8 @@ class _TraceOptions(ctypes.Structure):
                 ("emit_records", ctypes.c_uint32), ("waves_per_cu", ctypes.c_uint32),
                 ("lds_stack", ctypes.c_uint32), ("stage_bytes", ctypes.c_uint32), ("refill", ctypes.c_uint32),
                 ("slot_bytes", ctypes.c_uint32), ("min_score", ctypes.c_float), ("pred_fraction", ctypes.c_float),
-                ("lanes_per_ray", ctypes.c_uint32)]
+                ("lanes_per_ray", ctypes.c_uint32), ("claim_start", ctypes.c_uint32),
+                ("record_slab", ctypes.c_uint32)]
 
 
 class _PrefetchStats(ctypes.Structure):
@@ -122,6 +125,11 @@ _sig("rtx_build_treelets", _i32, _vp, _u32, _P(_TreeletSummary))
 _sig("rtx_get_treelets", _i32, _vp, _vp, _vp, _vp, _vp)
 _sig("rtx_set_treelets", _i32, _vp, _u32, _u32, _vp, _vp, _vp, _vp)
 _sig("rtx_trace", _i32, _vp, _vp, _u32, _P(_TraceOptions), _vp, _vp, _u64, _vp)
+_sig("rtx_adapter_create", _i32, _i32, _vp, _u64, _u64, _u32, _P(_vp))
+_sig("rtx_adapter_destroy", _i32, _vp)
+_sig("rtx_adapter_trace_ray", _i32, _vp, _i32, _u32, _u32, _u32, _u32, _u32, _P(_f32), _f32, _P(_f32), _f32, _vp, _vp,
+     _u64, _P(_u64))
+_sig("rtx_adapter_trace_lanes", _i32, _vp, _i32, _vp, _u32, _vp, _vp, _u64, _vp)
 _sig("rtx_get_prefetch_stats", _i32, _vp, _P(_PrefetchStats))
 _sig("rtx_last_kernel_ms", _i32, _vp, _P(_f32))
 _sig("rtx_add_scene", _i32, _vp, _vp, ctypes.c_size_t, _u64, _P(_u32))
@@ -178,6 +186,7 @@ EXPORTED_SYMBOLS = [
     "rtx_builder_finish", "rtx_builder_finish_device", "rtx_workload_scene_device", "rtx_workload_builder", "rtx_builder_copy_arena", "rtx_builder_arena_info", "rtx_builder_counts", "rtx_builder_blas_offset",
     "rtx_mesh_displaced_sphere", "rtx_workload_scene", "rtx_workload_primary_rays", "rtx_workload_secondary_rays",
     "rtx_format_treelet_trace", "rtx_as_dump_write", "rtx_as_dump_info", "rtx_as_dump_read",
+    "rtx_adapter_create", "rtx_adapter_destroy", "rtx_adapter_trace_ray", "rtx_adapter_trace_lanes",
 ]
 
 
@@ -582,32 +591,44 @@ class Context:
 
     def trace_device(self, rays_buf, n, hits_buf, mode=MODE_PLAIN, prefetch=PREFETCH_NONE, threshold=0.5,
                      records_buf=None, record_capacity=0, offsets_buf=None, waves_per_cu=0, lds_stack=0,
-                     stage_bytes=0, refill=0, slot_bytes=0, min_score=0.0, pred_fraction=0.0, lanes_per_ray=0):
+                     stage_bytes=0, refill=0, slot_bytes=0, min_score=0.0, pred_fraction=0.0, lanes_per_ray=0,
+                     claim_start=0, record_slab=0):
         """rtx_trace on device buffers; returns the rtx_status (0 = ok)."""
         o = _TraceOptions(mode, prefetch, threshold, 1 if offsets_buf is not None else 0, waves_per_cu, lds_stack,
-                          stage_bytes, refill, slot_bytes, min_score, pred_fraction, lanes_per_ray)
+                          stage_bytes, refill, slot_bytes, min_score, pred_fraction, lanes_per_ray, claim_start,
+                          record_slab)
         return _lib.rtx_trace(self._h, rays_buf.ptr, n, ctypes.byref(o), hits_buf.ptr,
                               records_buf.ptr if records_buf is not None else None, record_capacity,
                               offsets_buf.ptr if offsets_buf is not None else None)
 
-    def trace(self, rays, mode=MODE_PLAIN, prefetch=PREFETCH_NONE, threshold=0.5, records=False, **kw):
-        """Host arrays in and out.  Returns hits, or (hits, offsets, records) when records=True."""
+    def trace(self, rays, mode=MODE_PLAIN, prefetch=PREFETCH_NONE, threshold=0.5, records=False,
+              records_per_ray=48, **kw):
+        """Host arrays in and out.  Returns hits, or (hits, offsets, records) when records=True.
+
+        With records, one rtx_trace call fills a buffer of records_per_ray * n
+        records; when the rays produce more, the call reports the total and is
+        repeated with a buffer of that size."""
         rays = np.ascontiguousarray(rays, dtype=RAY_DTYPE)
         n = len(rays)
         rb = self.to_device(rays)
-        hb = self.buffer(n * HIT_DTYPE.itemsize)
+        hb = self.buffer(max(n, 1) * HIT_DTYPE.itemsize)
         if not records:
             _check(self.trace_device(rb, n, hb, mode, prefetch, threshold, **kw), "rtx_trace")
             return hb.download(HIT_DTYPE, n)
         ob = self.buffer((n + 1) * 8)
-        st = self.trace_device(rb, n, hb, mode, prefetch, threshold, None, 0, ob, **kw)
-        if st not in (OK, 5):
-            raise RtxError(st, "rtx_trace")
+        cap = max(int(records_per_ray) * n, 1)
+        recb = self.buffer(cap * RECORD_DTYPE.itemsize)
+        st = self.trace_device(rb, n, hb, mode, prefetch, threshold, recb, cap, ob, **kw)
+        if st == 5:
+            total = int(ob.download(np.uint64, n + 1)[-1])
+            recb.free()
+            cap = max(total, 1)
+            recb = self.buffer(cap * RECORD_DTYPE.itemsize)
+            st = self.trace_device(rb, n, hb, mode, prefetch, threshold, recb, cap, ob, **kw)
+        _check(st, "rtx_trace")
         offsets = ob.download(np.uint64, n + 1)
         total = int(offsets[-1])
-        recb = self.buffer(max(total, 1) * RECORD_DTYPE.itemsize)
-        _check(self.trace_device(rb, n, hb, mode, prefetch, threshold, recb, total, ob, **kw), "rtx_trace")
-        return hb.download(HIT_DTYPE, n), ob.download(np.uint64, n + 1), recb.download(RECORD_DTYPE, total)
+        return hb.download(HIT_DTYPE, n), offsets, recb.download(RECORD_DTYPE, total)
 
     def set_prefetch_table(self, table):
         """Install the learned-prefetch table of the current partition: uint32 [n_treelets * 8, 4]."""
@@ -656,7 +677,7 @@ class Context:
 
     @staticmethod
     def _opts(mode=MODE_PLAIN, waves_per_cu=0, lds_stack=0, refill=0):
-        return _TraceOptions(mode, PREFETCH_NONE, 0.0, 0, waves_per_cu, lds_stack, 0, refill, 0, 0.0, 0.0)
+        return _TraceOptions(mode, PREFETCH_NONE, 0.0, 0, waves_per_cu, lds_stack, 0, refill, 0, 0.0, 0.0, 0, 0, 0)
 
     def trace_batch(self, jobs, **opts):
         arr = self._jobs(jobs)
