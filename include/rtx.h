@@ -250,6 +250,20 @@ rtx_status rtx_trace(rtx_ctx* ctx, const rtx_ray* rays_dev, uint32_t n, const rt
                      uint64_t* record_offsets_dev);
 rtx_status rtx_get_prefetch_stats(rtx_ctx* ctx, rtx_prefetch_stats* out);
 
+/* rtx_trace that allocates nothing.  No counterpart in the reference: it is for
+ * callers that cannot allocate device memory while they trace (tight serving
+ * loops, harnesses that forbid hipMalloc).  Hits only, no staging: the options
+ * are those of rtx_trace with prefetch RTX_PREFETCH_NONE and emit_records 0
+ * (else RTX_ERR_UNSUPPORTED).  The kernels work in a static device array of
+ * 128 MiB per GPU, held until the process ends: 256 bytes of control words and
+ * the per-lane global stack spill.  The launch keeps at least 24 spill entries
+ * per lane (per list in treelet order), shrinking its grid if needed, and at
+ * most 128; a ray whose stack outgrows LDS plus that spill returns
+ * RTX_ERR_STACK_OVERFLOW.  Calls on one GPU share the array, so calls that
+ * overlap on one GPU (other threads, other contexts on the device) race;
+ * each call clears the control words it uses and waits for its kernel. */
+rtx_status rtx_trace_static(rtx_ctx* ctx, const rtx_ray* rays_dev, uint32_t n, const rtx_trace_options* opts,
+                            rtx_hit* hits_dev);
 
 /* ---- learned next-treelet prefetch -----------------------------------------
  * No counterpart in the reference (its RT unit has popularity heuristics

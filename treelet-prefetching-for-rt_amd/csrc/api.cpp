@@ -83,6 +83,12 @@ struct rtx_ctx {
   BatchJob* jobs_dev = nullptr;
   // batch graphs captured on this context (detached by rtx_destroy)
   std::vector<rtx_batch_graph*> graphs;
+  // rtx_trace_static: this device's copy of the static scratch array (from
+  // hipGetSymbolAddress) and the occupancy of the kernels it may launch,
+  // [mode][lds_stack 4/8/16] and the eight-lanes-per-ray kernels [mode]
+  uint8_t* static_scratch = nullptr;
+  uint32_t static_occ[2][3] = {{0, 0, 0}, {0, 0, 0}};
+  uint32_t static_wide_occ[2] = {0, 0};
 };
 
 struct rtx_batch_graph {
@@ -277,6 +283,15 @@ rtx_status rtx_create(int device, rtx_ctx** out) {
     return fail(RTX_ERR_OUT_OF_MEMORY);
   if (hipHostGetDevicePointer(reinterpret_cast<void**>(&c->err_host_dev), c->err_host, 0) != hipSuccess)
     return fail(RTX_ERR_HIP);
+  // set-up of rtx_trace_static, so that it allocates and queries nothing
+  void* sp = nullptr;
+  if (static_scratch_init(device, &sp) != hipSuccess || !sp) return fail(RTX_ERR_HIP);
+  c->static_scratch = static_cast<uint8_t*>(sp);
+  const uint32_t ks[3] = {4, 8, 16};
+  for (uint32_t m = 0; m < 2; ++m) {
+    for (int k = 0; k < 3; ++k) c->static_occ[m][k] = trace_blocks_per_cu(m, kRecNone, ks[k], 0);
+    c->static_wide_occ[m] = trace_wide_blocks_per_cu(m);
+  }
   *out = c;
   return RTX_OK;
 }
@@ -692,6 +707,87 @@ rtx_status rtx_trace(rtx_ctx* c, const rtx_ray* rays, uint32_t n, const rtx_trac
   c->last_stats.prediction_hits = stats[6];
   c->last_stats.fallbacks = stats[7];
   return launch_error();
+}
+
+// rtx_trace without allocation: hits only, the library's static device
+// scratch for control words and stack spill (see rtx.h).
+rtx_status rtx_trace_static(rtx_ctx* c, const rtx_ray* rays, uint32_t n, const rtx_trace_options* opts,
+                            rtx_hit* hits) {
+  if (!c || !opts || (n > 0 && (!rays || !hits))) return RTX_ERR_INVALID_ARGUMENT;
+  if (opts->mode != RTX_MODE_PLAIN && opts->mode != RTX_MODE_TREELET) return RTX_ERR_INVALID_ARGUMENT;
+  if (opts->prefetch != RTX_PREFETCH_NONE || opts->emit_records) return RTX_ERR_UNSUPPORTED;
+  if (n > 0 ? opts->claim_start >= n : opts->claim_start != 0) return RTX_ERR_INVALID_ARGUMENT;
+  if (opts->refill > 64) return RTX_ERR_INVALID_ARGUMENT;
+  if (!c->arena) return RTX_ERR_NO_SCENE;
+  if (opts->mode == RTX_MODE_TREELET && !c->part.valid) return RTX_ERR_NO_PARTITION;
+  if (((uintptr_t)rays & 15) || ((uintptr_t)hits & 15)) return RTX_ERR_INVALID_ARGUMENT;
+  if (!c->static_scratch) return RTX_ERR_HIP;
+  const uint32_t lds_stack = opts->lds_stack ? opts->lds_stack : (opts->mode == RTX_MODE_PLAIN ? 8 : 4);
+  const int ki = lds_stack == 4 ? 0 : lds_stack == 8 ? 1 : lds_stack == 16 ? 2 : -1;
+  if (ki < 0) return RTX_ERR_INVALID_ARGUMENT;
+  if (opts->lanes_per_ray != 0 && opts->lanes_per_ray != 1 && opts->lanes_per_ray != kWideRayLanes)
+    return RTX_ERR_INVALID_ARGUMENT;
+  HIP_TRY(hipSetDevice(c->device));
+  c->kernel_timed = false;
+  memset(&c->last_stats, 0, sizeof(c->last_stats));
+  if (n == 0) return RTX_OK;
+  const bool wide = opts->lanes_per_ray == kWideRayLanes ||
+                    (opts->lanes_per_ray == 0 &&
+                     (uint64_t)n <= (opts->mode == RTX_MODE_PLAIN ? 128ull : 256ull) * c->n_cus);
+  const uint32_t bs = 256;
+  uint32_t per_cu = opts->waves_per_cu ? std::max(1u, opts->waves_per_cu * 64 / bs)
+                    : wide             ? c->static_wide_occ[opts->mode]
+                                       : c->static_occ[opts->mode][ki];
+  if (per_cu == 0) per_cu = 4;
+  const uint64_t lanes = (uint64_t)n * (wide ? kWideRayLanes : 1u);
+  uint32_t grid = (uint32_t)std::min<uint64_t>((uint64_t)c->n_cus * per_cu, (lanes + bs - 1) / bs);
+  grid = std::max(grid, 1u);
+  // The spill area is what the scratch holds beyond its control words, split
+  // over every lane of the launch (one region per list in treelet order);
+  // the grid shrinks when fewer than kStaticMinSpill entries per lane would fit.
+  const uint32_t ns = opts->mode == RTX_MODE_TREELET ? 2 : 1;
+  const uint64_t spill_bytes = kStaticScratchBytes - kStaticControlBytes;
+  uint32_t spill_cap = 0;
+  if (!wide) {
+    const uint64_t max_grid = spill_bytes / ((uint64_t)bs * ns * 8 * kStaticMinSpill);
+    grid = (uint32_t)std::min<uint64_t>(grid, max_grid);
+    spill_cap = (uint32_t)std::min<uint64_t>(128, spill_bytes / ((uint64_t)grid * bs * ns * 8));
+  }
+  uint32_t* control = reinterpret_cast<uint32_t*>(c->static_scratch);  // [0] ray counter, [1] error word
+
+  TraceParams p{};
+  p.arena = c->arena;
+  p.arena_bytes = c->arena_bytes;
+  p.tlas = c->tlas;
+  p.rays = rays;
+  p.n = n;
+  p.refill = opts->refill ? opts->refill : 40;
+  p.ordered_boxes = c->ordered ? 1u : 0u;
+  p.hits = hits;
+  p.owner = c->part.valid ? c->part.owner : nullptr;
+  p.first_treelet = c->part.valid ? c->part.first_treelet : 0;
+  p.spill = reinterpret_cast<uint64_t*>(c->static_scratch + kStaticControlBytes);
+  p.spill_cap = spill_cap;
+  p.claim_start = opts->claim_start;
+  p.ray_counter = control;
+  p.error = control + 1;
+  p.host_err = c->err_host_dev;
+  volatile uint32_t* herr = c->err_host;
+  *herr = 0;
+  // every call starts from cleared control words (a fill kernel, no memset)
+  HIP_TRY(launch_fill_zero(control, 2, c->stream));
+  HIP_TRY(hipEventRecord(c->evk0, c->stream));
+  if (wide)
+    HIP_TRY(launch_trace_wide(p, opts->mode, lds_stack + 128, grid, c->stream));
+  else
+    HIP_TRY(launch_trace(p, opts->mode, kRecNone, lds_stack, 0, grid, c->stream));
+  HIP_TRY(hipEventRecord(c->evk1, c->stream));
+  c->kernel_timed = true;
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  if (*herr == 0) return RTX_OK;
+  uint32_t err = 0;
+  HIP_TRY(hipMemcpy(&err, control + 1, sizeof(uint32_t), hipMemcpyDeviceToHost));
+  return kernel_error_status(err);
 }
 
 rtx_status rtx_set_prefetch_table(rtx_ctx* c, const uint32_t* table, uint64_t n_rows) {

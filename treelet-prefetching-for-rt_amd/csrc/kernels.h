@@ -92,6 +92,15 @@ struct TraceParams {
 // Most treelets resident at once in a wave's staging area.
 constexpr uint32_t kMaxStageSlots = 16;
 
+// Device scratch of rtx_trace_static (static_scratch.hip): 256 bytes of
+// control words, then the per-lane global stack spill.
+constexpr size_t kStaticScratchBytes = 128ull << 20;
+constexpr size_t kStaticControlBytes = 256;
+constexpr uint32_t kStaticMinSpill = 24;  // spill entries per lane and list it keeps at least
+// Address of the calling thread's current device's copy of the scratch array
+// (looked up once per device; `device` must be current).
+hipError_t static_scratch_init(int device, void** out);
+
 // Launchers (defined in the .hip files).
 uint32_t trace_block_size(uint32_t stage_bytes);
 hipError_t launch_trace(const TraceParams& p, uint32_t mode, int records, uint32_t lds_stack, uint32_t stage_bytes,

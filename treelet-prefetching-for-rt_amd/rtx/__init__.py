@@ -125,6 +125,7 @@ _sig("rtx_build_treelets", _i32, _vp, _u32, _P(_TreeletSummary))
 _sig("rtx_get_treelets", _i32, _vp, _vp, _vp, _vp, _vp)
 _sig("rtx_set_treelets", _i32, _vp, _u32, _u32, _vp, _vp, _vp, _vp)
 _sig("rtx_trace", _i32, _vp, _vp, _u32, _P(_TraceOptions), _vp, _vp, _u64, _vp)
+_sig("rtx_trace_static", _i32, _vp, _vp, _u32, _P(_TraceOptions), _vp)
 _sig("rtx_adapter_create", _i32, _i32, _vp, _u64, _u64, _u32, _P(_vp))
 _sig("rtx_adapter_destroy", _i32, _vp)
 _sig("rtx_adapter_trace_ray", _i32, _vp, _i32, _u32, _u32, _u32, _u32, _u32, _P(_f32), _f32, _P(_f32), _f32, _vp, _vp,
@@ -177,7 +178,7 @@ EXPORTED_SYMBOLS = [
     "rtx_status_string", "rtx_version", "rtx_device_count", "rtx_create", "rtx_destroy", "rtx_synchronize",
     "rtx_malloc", "rtx_free", "rtx_memcpy_h2d", "rtx_memcpy_d2h", "rtx_memcpy_d2d", "rtx_memset", "rtx_timer_start",
     "rtx_timer_stop", "rtx_device_info", "rtx_set_scene", "rtx_set_scene_device", "rtx_validate_scene",
-    "rtx_build_treelets", "rtx_get_treelets", "rtx_set_treelets", "rtx_trace", "rtx_get_prefetch_stats",
+    "rtx_build_treelets", "rtx_get_treelets", "rtx_set_treelets", "rtx_trace", "rtx_trace_static", "rtx_get_prefetch_stats",
     "rtx_last_kernel_ms", "rtx_add_scene", "rtx_build_scene_treelets", "rtx_trace_batch", "rtx_batch_graph_create",
     "rtx_batch_graph_launch", "rtx_batch_graph_destroy", "rtx_autotune_batch", "rtx_autotune_batch_treelets",
     "rtx_set_prefetch_table", "rtx_mlp_create", "rtx_mlp_destroy", "rtx_mlp_param_count", "rtx_mlp_get_params",
@@ -600,6 +601,13 @@ class Context:
         return _lib.rtx_trace(self._h, rays_buf.ptr, n, ctypes.byref(o), hits_buf.ptr,
                               records_buf.ptr if records_buf is not None else None, record_capacity,
                               offsets_buf.ptr if offsets_buf is not None else None)
+
+    def trace_static_device(self, rays_buf, n, hits_buf, mode=MODE_PLAIN, waves_per_cu=0, lds_stack=0, refill=0,
+                            lanes_per_ray=0, claim_start=0):
+        """rtx_trace_static (hits only, no allocation) on device buffers; returns the rtx_status."""
+        o = _TraceOptions(mode, PREFETCH_NONE, 0.0, 0, waves_per_cu, lds_stack, 0, refill, 0, 0.0, 0.0, lanes_per_ray,
+                          claim_start, 0)
+        return _lib.rtx_trace_static(self._h, rays_buf.ptr, n, ctypes.byref(o), hits_buf.ptr)
 
     def trace(self, rays, mode=MODE_PLAIN, prefetch=PREFETCH_NONE, threshold=0.5, records=False,
               records_per_ray=48, **kw):
