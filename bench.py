@@ -41,7 +41,7 @@ import numpy as np  # noqa: E402
 HBM_PEAK_GBS = 8000.0
 METRIC = "Mrays/s plain vs treelet vs learned, 1M-tri diffuse; node-fetch GB/s % of HBM peak"
 
-# name -> traversal order, prefetch policy (0 none, 1 always, 2 popularity, 3 partial, 4 learned),
+# name -> traversal order, prefetch policy (0 none, 1 always, 2 popularity, 3 partial, 4 learned, 5 latter part),
 # treelet budget, LDS staging bytes per wave, LDS slot bytes (0: one treelet resident),
 # launch ("trace": rtx_trace per segment, "batch": one graph-captured persistent launch over
 # every segment, longest segments first), records (hits + node-access records, as traceRay
@@ -64,6 +64,8 @@ MODES = {
                               claim=False),
     "treelet+stage_partial": dict(order=1, prefetch=3, budget=8192, stage=8192, slot=0, launch="trace",
                                   records=False, claim=False),
+    "treelet+stage_latter": dict(order=1, prefetch=5, budget=8192, stage=8192, slot=0, launch="trace",
+                                 records=False, claim=False),
     "plain+slots": dict(order=0, prefetch=1, budget=1024, stage=8192, slot=1024, launch="trace", records=False,
                         claim=False),
     "plain+learned": dict(order=0, prefetch=4, budget=1024, stage=8192, slot=1024, launch="trace", records=False,
@@ -122,11 +124,11 @@ def cpu_baseline(samples, what, target_s=10.0):
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle
 
-    def run(threads, budget):
+    def run(threads, budget, items):
         rays = passes = 0
         t0 = time.perf_counter()
         while True:
-            for arena, tlas, r in samples:
+            for arena, tlas, r in items:
                 oracle.trace_parallel(arena, tlas, r, mode=0, threads=threads)
                 rays += len(r)
             passes += 1
@@ -135,10 +137,10 @@ def cpu_baseline(samples, what, target_s=10.0):
                 return rays / dt / 1e6, passes, dt
 
     threads = host_threads()
-    v, passes, dt = run(threads, target_s)
+    v, passes, dt = run(threads, target_s, samples)
     n = sum(len(r) for _, _, r in samples)
     single = [(a, t, r[:: max(1, threads)]) for a, t, r in samples]
-    v1, p1, dt1 = run(1, target_s / 3)
+    v1, p1, dt1 = run(1, target_s / 3, single)
     return {"value": round(v, 4), "unit": "Mrays/s", "cores": threads, "kind": "port",
             "sample": f"{what}: {n} rays x {passes} passes, plain order, {dt:.1f} s on {threads} host threads",
             "single_thread": {"value": round(v1, 4), "cores": 1,

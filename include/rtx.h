@@ -62,9 +62,14 @@ enum {
   RTX_PREFETCH_ALWAYS = 1,     /* heuristic 0: stage the wave's most popular treelet */
   RTX_PREFETCH_POPULARITY = 2, /* heuristic 1: stage it only if its share >= threshold */
   RTX_PREFETCH_PARTIAL = 3,    /* heuristic 2: stage the first round(n*share) nodes */
-  RTX_PREFETCH_LEARNED = 4     /* MLP predicts the next treelet; heuristic 0 is the fallback */
+  RTX_PREFETCH_LEARNED = 4,    /* MLP predicts the next treelet; heuristic 0 is the fallback */
+  RTX_PREFETCH_LATTER = 5      /* heuristic 3: stage the last round(n*share) units */
 };
-/* Heuristic numbering follows src/gpgpu-sim/shader.cc:3437-3559. */
+/* Heuristics 0-3 follow src/gpgpu-sim/shader.cc:3437-3590: the wave counts the
+ * treelet of every active lane's next node and takes the most popular one,
+ * ties to the lowest root offset (:3424-3446); share = its count / the lanes
+ * counted.  A treelet already resident in the wave's staging area is not
+ * staged again (:3564). */
 
 /* SPIR-V ray flags used by the traversal (SpvRayFlags*KHRMask). */
 enum {
@@ -166,6 +171,11 @@ typedef struct rtx_trace_options {
                             by secondary rays traces faster when the longer, incoherent secondary
                             rays are claimed first: pass the index of the first secondary ray.
                             Must be < n (0 for an empty batch).  rtx_trace only. */
+  uint32_t record_layout; /* 1: each record carries its node's offset in the treelet layout of
+                             rtx_build_treelet_layout instead of its arena offset, as
+                             -remap_to_treelet_layout makes the reference record remapped addresses
+                             (vulkan_ray_tracing.cc:2446-2450, 2509-2512); RTX_NO_NODE for a node
+                             placed nowhere.  Needs the layout (else RTX_ERR_NO_PARTITION) */
   uint32_t record_slab;  /* records, one traversal: each ray's first record_slab records go to a
                             per-ray slab in library memory and are compacted into the caller's CSR
                             buffer; only rays with more records are traced a second time.
@@ -173,17 +183,34 @@ typedef struct rtx_trace_options {
                             when n * record_slab * 16 bytes exceed 8 GiB */
 } rtx_trace_options;
 
-/* Prefetch accounting of the last rtx_trace call, modelled on the
- * TIMELY / NEVER_USED classes of src/gpgpu-sim/gpu-sim.cc:1750-1868. */
+/* Prefetch accounting of the last rtx_trace call.  The effectiveness classes
+ * of the reference's prefetch tracker (prefetch_request_effectiveness,
+ * src/gpgpu-sim/gpu-cache.h:99-131; classified at gpu-cache.cc:1960-2117,
+ * printed at gpu-sim.cc:1750-1868), per 64-byte unit staged into a wave's LDS:
+ *   TIMELY      read from LDS at least once before its eviction (a demand hit
+ *               on a prefetched line);
+ *   NEVER_USED  evicted unread (wasted_bytes; includes TOO_EARLY);
+ *   TOO_EARLY   evicted unread, and its node then read from global memory by
+ *               the same wave (a demand miss after the prefetch's eviction);
+ *   TOO_LATE    staged after the same wave had read its node from global
+ *               memory (a prefetch landing on data a demand load fetched);
+ *   LATE        a demand read while the prefetch is in flight: always 0 here,
+ *               because a wave finishes staging before its next node read.
+ * TOO_EARLY and TOO_LATE remember node slots in 128-entry direct-mapped
+ * tables per wave, so they are lower bounds; a node counts 64 bytes. */
 typedef struct rtx_prefetch_stats {
   uint64_t node_reads;        /* node reads issued by the traversal */
-  uint64_t lds_hits;          /* node reads served from a staged treelet (timely) */
+  uint64_t lds_hits;          /* node reads served from a staged treelet */
   uint64_t staged_treelets;   /* treelet loads into LDS */
   uint64_t staged_bytes;      /* bytes copied from HBM into LDS */
-  uint64_t wasted_bytes;      /* staged bytes never read before eviction (never used) */
+  uint64_t wasted_bytes;      /* NEVER_USED: staged bytes never read before eviction */
   uint64_t predictions;       /* LEARNED: treelets staged because the table predicted them */
   uint64_t prediction_hits;   /* LEARNED: predicted stagings that served a node read before eviction */
   uint64_t fallbacks;         /* LEARNED: staging decisions where no lane had a table entry */
+  uint64_t timely_bytes;      /* TIMELY: staged bytes read at least once; timely + wasted = staged */
+  uint64_t too_early_bytes;   /* TOO_EARLY (part of wasted_bytes) */
+  uint64_t too_late_bytes;    /* TOO_LATE */
+  uint64_t late_bytes;        /* LATE: 0 */
 } rtx_prefetch_stats;
 
 typedef struct rtx_treelet_summary {
@@ -235,6 +262,31 @@ rtx_status rtx_build_treelets(rtx_ctx* ctx, uint32_t budget_bytes, rtx_treelet_s
  *  owner[arena_bytes/64]      dense treelet index owning the node at offset 64*i, or RTX_NO_TREELET */
 rtx_status rtx_get_treelets(rtx_ctx* ctx, uint64_t* roots, uint64_t* list_offsets,
                             uint64_t* list_nodes, uint32_t* owner);
+/* remapBVHToTreeletLayout (vulkan_ray_tracing.cc:1473-1509) of the current
+ * partition, with -treelet_remap_stride `stride` (gpu-sim.cc:461-464): treelet
+ * t occupies [t * (budget + stride), ...) of a treelet-contiguous address space,
+ * its root first, then the other nodes of its list in order (instance leaf 128
+ * bytes, every other node 64); a node several lists hold keeps the place of the
+ * first (lowest root offset).  Also the per-treelet metadata table of
+ * load_treelet_metadata (:1598-1605): budget/64 entries of 4 bytes per treelet,
+ * entry j the signed byte distance from the treelet root's place to the place
+ * of its list entry j, -1 past the list.  RTX_ERR_UNSUPPORTED where the
+ * reference asserts (:1483): a treelet root that an earlier treelet's list
+ * already placed.  Rebuilding the partition drops the layout. */
+typedef struct rtx_treelet_layout_summary {
+  uint32_t stride;           /* treelet_remap_stride */
+  uint32_t block_bytes;      /* budget + stride: the distance between treelet roots */
+  uint64_t layout_bytes;     /* n_treelets * block_bytes (the treelet_layout_bvh allocation, :1478) */
+  uint64_t mapped_nodes;     /* nodes given a place */
+  uint32_t metadata_entries; /* per treelet: budget / 64 */
+  uint32_t reserved;
+} rtx_treelet_layout_summary;
+rtx_status rtx_build_treelet_layout(rtx_ctx* ctx, uint32_t stride, rtx_treelet_layout_summary* out);
+/* Copy the layout to host memory (either pointer may be NULL):
+ *  remap[arena_bytes/64]                layout offset of the node at arena offset 64*i, or RTX_NO_NODE
+ *  metadata[n_treelets * budget/64]     the metadata table */
+rtx_status rtx_get_treelet_layout(rtx_ctx* ctx, uint64_t* remap, int32_t* metadata);
+
 /* Install a partition computed elsewhere (same arrays as rtx_get_treelets). */
 rtx_status rtx_set_treelets(rtx_ctx* ctx, uint32_t budget_bytes, uint32_t n_treelets,
                             const uint64_t* roots, const uint64_t* list_offsets,

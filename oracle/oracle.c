@@ -589,6 +589,97 @@ int orc_partition_copy(const orc_partition* p, uint64_t* roots, uint64_t* list_o
   return ORC_OK;
 }
 
+/* Slots holding instance leaves: the typed-1 children of TLAS internal nodes. */
+static int instance_slots(const uint8_t* A, uint64_t bytes, uint64_t tlas, uint8_t* is_inst) {
+  const uint64_t slots = bytes / 64;
+  uint8_t* seen = (uint8_t*)calloc(slots, 1);
+  uint64_t* q = (uint64_t*)malloc(slots * sizeof(uint64_t));
+  if (!seen || !q) {
+    free(seen);
+    free(q);
+    return ORC_ERR_OUT_OF_MEMORY;
+  }
+  size_t head = 0, tail = 0;
+  hdr_t h = unpack_hdr(A + tlas);
+  uint64_t root = tlas + h.root_offset;
+  int st = ORC_OK;
+  if (!in_arena(root, 64, bytes) || root % 64) {
+    st = ORC_ERR_OFFSET_OUT_OF_ARENA;
+    goto out;
+  }
+  q[tail++] = root;
+  seen[root / 64] = 1;
+  while (head < tail) {
+    const uint64_t a = q[head++];
+    inode_t n = unpack_inode(A + a);
+    uint64_t c = a + (int64_t)n.child_offset * 64;
+    for (int i = 0; i < 6; i++) {
+      if (n.size[i] > 0) {
+        if (c % 64 || !in_arena(c, 64, bytes)) {
+          st = ORC_ERR_OFFSET_OUT_OF_ARENA;
+          goto out;
+        }
+        if (n.type[i] == 1) {
+          is_inst[c / 64] = 1;
+        } else if (n.type[i] == 0 && !seen[c / 64]) {
+          seen[c / 64] = 1;
+          q[tail++] = c;
+        }
+      }
+      c += n.size[i] * 64;
+    }
+  }
+out:
+  free(seen);
+  free(q);
+  return st;
+}
+
+int orc_treelet_layout(const uint8_t* A, uint64_t bytes, uint64_t tlas, const orc_partition* p, uint32_t stride,
+                       uint64_t* remap, int32_t* metadata) {
+  if (!A || !p || !remap || bytes != p->arena_bytes) return ORC_ERR_INVALID_ARGUMENT;
+  const uint64_t slots = bytes / 64;
+  uint8_t* is_inst = (uint8_t*)calloc(slots, 1);
+  if (!is_inst) return ORC_ERR_OUT_OF_MEMORY;
+  int st = instance_slots(A, bytes, tlas, is_inst);
+  if (st != ORC_OK) {
+    free(is_inst);
+    return st;
+  }
+  for (uint64_t s = 0; s < slots; s++) remap[s] = ~0ull;
+  const uint64_t block = (uint64_t)p->budget + stride;
+  const uint32_t per = p->budget / 64;
+  /* for (auto root : treelet_roots): ascending root address (:1484) */
+  for (uint32_t t = 0; t < p->n && st == ORC_OK; t++) {
+    const uint64_t root = p->roots[t];
+    const uint64_t base = (uint64_t)t * block; /* treelet_layout_bvh + i * (max_treelet_size + stride) */
+    if (remap[root / 64] != ~0ull) { /* assert(mapping.count(root) == 0), :1483 */
+      st = ORC_ERR_INVALID_ARGUMENT;
+      break;
+    }
+    remap[root / 64] = base;
+    uint64_t next = base + (is_inst[root / 64] ? 128 : 64); /* + root.first.size */
+    for (uint64_t i = p->list_offsets[t]; i < p->list_offsets[t + 1]; i++) {
+      const uint64_t a = p->list_nodes[i];
+      if (a == root) continue;
+      if (remap[a / 64] == ~0ull) remap[a / 64] = next; /* the first mapping stays (:1496-1502) */
+      next += is_inst[a / 64] ? 128 : 64;
+    }
+  }
+  if (st == ORC_OK && metadata) {
+    for (uint32_t t = 0; t < p->n; t++) {
+      int32_t* m = metadata + (uint64_t)t * per;
+      const uint64_t r = remap[p->roots[t] / 64];
+      uint64_t j = 0;
+      for (uint64_t i = p->list_offsets[t]; i < p->list_offsets[t + 1] && j < per; i++, j++)
+        m[j] = (int32_t)((int64_t)remap[p->list_nodes[i] / 64] - (int64_t)r);
+      for (; j < per; j++) m[j] = -1;
+    }
+  }
+  free(is_inst);
+  return st;
+}
+
 /* ------------------------------------------------------------------------ */
 /* Traversal                                                                  */
 /* ------------------------------------------------------------------------ */

@@ -83,6 +83,20 @@ uint64_t orc_partition_total_bytes(const orc_partition* p);
 int orc_partition_copy(const orc_partition* p, uint64_t* roots, uint64_t* list_offsets, uint64_t* list_nodes,
                        uint32_t* owner);
 
+/* remapBVHToTreeletLayout (vulkan_ray_tracing.cc:1473-1509): treelet t (roots
+ * in ascending address order) occupies bytes [t*(budget+stride), ...) of the
+ * treelet layout, its root first, then the other nodes of its list in order,
+ * each advancing by its size (128 for an instance leaf, 64 otherwise); a node
+ * already placed by an earlier treelet keeps that place.  remap[arena/64]
+ * receives each node's layout offset (~0 for nodes in no list).  metadata
+ * [n * budget/64] (the per-treelet table of :1598-1605) receives, for list
+ * entry j of treelet t, the signed byte offset of the entry's layout place
+ * from its root's, -1 past the end of the list.  Returns
+ * ORC_ERR_INVALID_ARGUMENT where the reference asserts: a root already placed
+ * by an earlier treelet (:1483). */
+int orc_treelet_layout(const uint8_t* arena, uint64_t arena_bytes, uint64_t tlas_offset, const orc_partition* p,
+                       uint32_t stride, uint64_t* remap, int32_t* metadata);
+
 /* mode 0 = traceRay, 1 = traceRayWithTreelets (needs a partition).  The
  * partition, when given in mode 0, only labels records with treelets.
  * records may be NULL (then only record_offsets / counts are produced).

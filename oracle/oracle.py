@@ -51,6 +51,7 @@ _sig("orc_partition_count", _u32, _vp)
 _sig("orc_partition_list_entries", _u64, _vp)
 _sig("orc_partition_total_bytes", _u64, _vp)
 _sig("orc_partition_copy", _i32, _vp, _vp, _vp, _vp, _vp)
+_sig("orc_treelet_layout", _i32, _vp, _u64, _u64, _vp, _u32, _vp, _vp)
 _sig("orc_trace", _i32, _vp, _u64, _u64, _vp, _i32, _vp, _u32, _vp, _vp, _u64, _vp)
 _sig("orc_trace_parallel", _i32, _vp, _u64, _u64, _vp, _i32, _vp, _u32, _vp, _i32)
 _sig("orc_ray_box_test", _i32, _vp, _vp, _vp, _vp, _f32, _f32, _P(_f32))
@@ -102,6 +103,18 @@ class Partition:
     @property
     def n_treelets(self):
         return len(self.roots)
+
+    def layout(self, stride=0):
+        """remapBVHToTreeletLayout (vulkan_ray_tracing.cc:1473-1509) with -treelet_remap_stride
+        `stride`: (remap[arena/64] of layout offsets, ~0 for nodes in no list; metadata
+        [n_treelets, budget/64] of byte offsets from each treelet root's place, -1 past a list)."""
+        remap = np.empty(self.arena.nbytes // 64, np.uint64)
+        meta = np.empty((self.n_treelets, self.budget // 64), np.int32)
+        st = _lib.orc_treelet_layout(_p(self.arena), self.arena.nbytes, self.tlas, self._h, stride, _p(remap),
+                                     _p(meta))
+        if st != 0:
+            raise OracleError(st, "orc_treelet_layout")
+        return remap, meta
 
     def nodes_of(self, t):
         return self.list_nodes[self.list_offsets[t]:self.list_offsets[t + 1]]

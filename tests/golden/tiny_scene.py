@@ -88,6 +88,26 @@ def tiny_arena():
     return np.frombuffer(bytes(A), dtype=np.uint8).copy()
 
 
+def tiny_shared_arena():
+    """Two instance leaves of one BLAS (960 bytes):
+        0   TLAS header     RootNodeOffset 64
+       64   TLAS root       children at 128: I1 (instance, ChildSize 2), I2 (instance, ChildSize 2)
+      128   I1              identity, BVHAddress 256 (-> 384), InstanceID 7
+      256   I2              identity, BVHAddress 128 (-> 384), InstanceID 8
+      384.. the tiny arena's BLAS (bytes 256..832 of tiny_arena), moved up by 128
+    """
+    src = tiny_arena()
+    A = bytearray(960)
+    A[0:64] = _header(64, (-1, -1, 0), (3, 3, 6))
+    A[64:128] = _internal((-1, -1, 1), (3, 3, 3), 1, [(2, 1, (0, 128, 0, 128, 0, 128)), (2, 1, (0, 128, 0, 128, 0, 128))])
+    for at, bvh, iid in ((128, 256, 7), (256, 128, 8)):
+        leaf = bytearray(src[128:256].tobytes())
+        struct.pack_into("<QII", leaf, 64, bvh, iid, 0)
+        A[at:at + 128] = leaf
+    A[384:960] = src[256:832].tobytes()
+    return np.frombuffer(bytes(A), dtype=np.uint8).copy()
+
+
 def tiny_ray(ray_dtype, flags=1):
     r = np.zeros(1, dtype=ray_dtype)
     r["org"] = (0.25, 0.25, 0.0)

@@ -183,7 +183,7 @@ def test_infinite_direction_against_overflowing_bounds(gpu, rtx, oracle):
 
 
 @pytest.mark.parametrize("mode", [0, 1])
-@pytest.mark.parametrize("prefetch,slot", [(1, 0), (2, 0), (3, 0), (1, 1024)])
+@pytest.mark.parametrize("prefetch,slot", [(1, 0), (2, 0), (3, 0), (5, 0), (1, 1024), (5, 1024)])
 def test_staged_kernels_on_affine_instances(gpu, rtx, oracle, mode, prefetch, slot):
     """The LDS staging kernels on mirrored and sheared instances: staging
     changes where node bytes are read, never the hits or records."""

@@ -1,4 +1,6 @@
 """GPU traversal and partition vs the CPU oracle, bit-exact (integer, index and FP32 bit patterns)."""
+import types
+
 import numpy as np
 import pytest
 
@@ -80,7 +82,7 @@ def test_plain_records_carry_treelets(gpu, oracle, c1):
 
 
 @pytest.mark.parametrize("mode", [0, 1])
-@pytest.mark.parametrize("prefetch,stage", [(1, 4096), (1, 8192), (2, 8192), (3, 16384), (1, 65536)])
+@pytest.mark.parametrize("prefetch,stage", [(1, 4096), (1, 8192), (2, 8192), (3, 16384), (1, 65536), (5, 8192), (5, 16384)])
 def test_staging_never_changes_results(gpu, oracle, c1, mode, prefetch, stage):
     """Treelet staging into LDS only moves where node bytes are read from."""
     scene, rays = c1
@@ -97,6 +99,12 @@ def test_staging_never_changes_results(gpu, oracle, c1, mode, prefetch, stage):
     assert s["lds_hits"] <= s["node_reads"]
     assert s["wasted_bytes"] <= s["staged_bytes"]
     assert s["staged_treelets"] > 0 and s["lds_hits"] > 0
+    # effectiveness classes: every staged unit ends TIMELY or NEVER_USED (all
+    # slots are evicted when the waves end); TOO_EARLY is part of NEVER_USED;
+    # a TIMELY unit served at least one LDS read; LATE cannot happen
+    assert s["timely_bytes"] + s["wasted_bytes"] == s["staged_bytes"]
+    assert s["too_early_bytes"] <= s["wasted_bytes"] and s["timely_bytes"] // 128 <= s["lds_hits"]
+    assert s["late_bytes"] == 0 and s["timely_bytes"] > 0
 
 
 @pytest.mark.parametrize("mode", [0, 1])
@@ -141,3 +149,68 @@ def test_nonfinite_rays_bit_exact(gpu, oracle, c1):
     h, o, rec = gpu.trace(r, mode=1, records=True)
     assert_hits_equal(h, h_ref, "treelet, non-finite rays")
     assert_records_equal(o, rec, o_ref, r_ref, "treelet, non-finite rays")
+
+
+def _layout_scenes(rtx, c1, instanced):
+    from golden import tiny_scene
+    tiny = types.SimpleNamespace(arena=tiny_scene.tiny_arena(), tlas_offset=0)
+    shared = types.SimpleNamespace(arena=tiny_scene.tiny_shared_arena(), tlas_offset=0)
+    return [("tiny", tiny, (256, 192, 8192)), ("tiny shared", shared, (256, 8192)),
+            ("c1", c1[0], (1024, 8192)), ("instanced", instanced[0], (1024, 4096))]
+
+
+@pytest.mark.parametrize("stride", [0, 256])
+def test_treelet_layout_matches_the_oracle(gpu, rtx, oracle, c1, instanced, stride):
+    """Device remapBVHToTreeletLayout: the same place for every node and the same
+    metadata table as the oracle (itself pinned by hand-derived layouts)."""
+    for name, scene, budgets in _layout_scenes(rtx, c1, instanced):
+        gpu.set_scene(scene.arena, scene.tlas_offset)
+        for budget in budgets:
+            gpu.build_treelets(budget)
+            s = gpu.build_treelet_layout(stride)
+            remap, meta = gpu.get_treelet_layout()
+            part = oracle.Partition(scene.arena, scene.tlas_offset, budget)
+            r_ref, m_ref = part.layout(stride)
+            assert np.array_equal(remap, r_ref), f"{name} @ {budget}: layout places differ"
+            assert np.array_equal(meta, m_ref), f"{name} @ {budget}: metadata differs"
+            assert s["block_bytes"] == budget + stride and s["layout_bytes"] == part.n_treelets * (budget + stride)
+            assert s["mapped_nodes"] == int((r_ref != np.uint64(0xFFFFFFFFFFFFFFFF)).sum())
+
+
+def test_treelet_layout_refuses_where_the_reference_asserts(gpu, rtx, oracle, instanced_shared):
+    """A shared BLAS whose treelet roots are also listed by earlier treelets: the
+    reference asserts (vulkan_ray_tracing.cc:1483); both oracle and device refuse."""
+    scene, _ = instanced_shared
+    gpu.set_scene(scene.arena, scene.tlas_offset)
+    gpu.build_treelets(1024)
+    with pytest.raises(oracle.OracleError):
+        oracle.Partition(scene.arena, scene.tlas_offset, 1024).layout(0)
+    with pytest.raises(rtx.RtxError) as e:
+        gpu.build_treelet_layout(0)
+    assert e.value.status == 11
+
+
+@pytest.mark.parametrize("mode", [0, 1])
+def test_records_in_treelet_layout_addresses(gpu, rtx, oracle, c1, instanced, mode):
+    """-remap_to_treelet_layout: every record names its node's layout place
+    (vulkan_ray_tracing.cc:2446-2450, 2509-2512); everything else is unchanged."""
+    for name, scene, budgets in _layout_scenes(rtx, c1, instanced):
+        rays = c1[1] if name == "c1" else instanced[1] if name == "instanced" else \
+            np.concatenate([tiny_scene_ray(rtx, f) for f in (1, 0, 5)])
+        gpu.set_scene(scene.arena, scene.tlas_offset)
+        budget = budgets[0]
+        gpu.build_treelets(budget)
+        gpu.build_treelet_layout(128)
+        part = oracle.Partition(scene.arena, scene.tlas_offset, budget)
+        remap, _ = part.layout(128)
+        h_ref, o_ref, r_ref = oracle.trace(scene.arena, scene.tlas_offset, rays, mode=mode, partition=part)
+        r_ref = r_ref.copy()
+        r_ref["offset"] = remap[(r_ref["offset"] // 64).astype(np.int64)]
+        h, o, r = gpu.trace(rays, mode=mode, records=True, record_layout=1)
+        assert_hits_equal(h, h_ref, f"{name} layout records, mode {mode}")
+        assert_records_equal(o, r, o_ref, r_ref, f"{name} layout records, mode {mode}")
+
+
+def tiny_scene_ray(rtx, flags):
+    from golden import tiny_scene
+    return tiny_scene.tiny_ray(rtx.RAY_DTYPE, flags=flags)

@@ -279,6 +279,75 @@ def test_malformed_arenas(oracle):
 
 
 # ---- whole scenes: brute force and invariants ----------------------------------
+@pytest.mark.parametrize("budget", [256, 192])
+def test_tiny_treelet_layout_hand_derived(oracle, budget):
+    p = oracle.Partition(tiny_scene.tiny_arena(), 0, budget)
+    remap, meta = p.layout(256)
+    exp = tiny_scene.expected()["layouts"][str(budget)]
+    got = {str(i * 64): int(v) for i, v in enumerate(remap) if v != np.uint64(0xFFFFFFFFFFFFFFFF)}
+    assert got == exp
+    # metadata: each list entry's place relative to its root's place, -1 past the list
+    for t in range(p.n_treelets):
+        nodes = p.nodes_of(t)
+        assert list(meta[t, :len(nodes)]) == [exp[str(int(a))] - exp[str(int(p.roots[t]))] for a in nodes]
+        assert np.all(meta[t, len(nodes):] == -1)
+
+
+def test_tiny_shared_blas_partition_and_layout_hand_derived(oracle):
+    """Two instance leaves of one BLAS: the BLAS header and root are listed by
+    both instance treelets; the later root owns them (VRT:479-488) while the
+    layout keeps the first treelet's place (VRT:1496-1502)."""
+    exp = tiny_scene.expected()["shared"]
+    A = tiny_scene.tiny_shared_arena()
+    p = oracle.Partition(A, 0, 256)
+    assert p.roots.tolist() == exp["partition_256"]["roots"]
+    assert [p.nodes_of(t).tolist() for t in range(p.n_treelets)] == exp["partition_256"]["lists"]
+    assert {str(i * 64): int(o) for i, o in enumerate(p.owner) if o != 0xFFFFFFFF} == exp["partition_256"]["owner"]
+    remap, meta = p.layout(256)
+    assert {str(i * 64): int(v) for i, v in enumerate(remap) if v != np.uint64(0xFFFFFFFFFFFFFFFF)} == exp["layout_256"]
+    assert meta[2].tolist() == exp["metadata_256_treelet_2"]
+
+
+def _layout_invariants(p, remap, stride, sizes):
+    block = p.budget + stride
+    none = np.uint64(0xFFFFFFFFFFFFFFFF)
+    placed = remap != none
+    # every listed node is placed; roots at their block starts
+    for t in range(p.n_treelets):
+        assert remap[int(p.roots[t]) // 64] == t * block
+        for a in p.nodes_of(t):
+            r = int(remap[int(a) // 64])
+            # a node lies in the block of the first treelet listing it: never a later one
+            assert r < (t + 1) * block
+    # placed nodes never overlap and stay inside their block
+    idx = np.nonzero(placed)[0]
+    order = np.argsort(remap[idx])
+    pos = remap[idx][order].astype(np.int64)
+    sz = sizes[idx][order]
+    assert np.all(pos[1:] >= pos[:-1] + sz[:-1])
+    assert np.all(pos % block + sz <= block)
+
+
+def test_treelet_layout_invariants(oracle, c1, instanced, instanced_shared):
+    """Shared BLAS nodes keep the place of the first treelet that lists them
+    (vulkan_ray_tracing.cc:1496-1502); places never overlap."""
+    for scene, _ in (c1, instanced, instanced_shared):
+        A = scene.arena
+        kinds = _node_kinds(A, scene.tlas_offset)
+        sizes = np.full(A.nbytes // 64, 64, np.int64)
+        for off, (kind, _) in kinds.items():
+            if kind == "inst":
+                sizes[off // 64] = 128
+        for budget, stride in ((1024, 0), (2048, 256), (8192, 64)):
+            p = oracle.Partition(A, scene.tlas_offset, budget)
+            try:
+                remap, meta = p.layout(stride)
+            except oracle.OracleError:
+                continue  # a root listed by an earlier treelet: the reference asserts (:1483)
+            _layout_invariants(p, remap, stride, sizes)
+            assert meta.shape == (p.n_treelets, budget // 64)
+
+
 def _brute_nearest(scene, rays):
     """float64 nearest hit over every triangle of every instance (world t)."""
     import rtx

@@ -485,6 +485,39 @@ rtx_status rtx_get_treelets(rtx_ctx* c, uint64_t* roots, uint64_t* list_offsets,
   return partition_copy_out(c->part, c->arena_bytes, c->stream, roots, list_offsets, list_nodes, owner);
 }
 
+rtx_status rtx_build_treelet_layout(rtx_ctx* c, uint32_t stride, rtx_treelet_layout_summary* out) {
+  if (!c) return RTX_ERR_INVALID_ARGUMENT;
+  if (!c->part.valid) return RTX_ERR_NO_PARTITION;
+  if (stride > (1u << 24)) return RTX_ERR_INVALID_ARGUMENT;
+  HIP_TRY(hipSetDevice(c->device));
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  rtx_status st = partition_layout(c->arena, c->arena_bytes, c->tlas, stride, c->stream, c->part);
+  if (st != RTX_OK) return st;
+  if (out) {
+    memset(out, 0, sizeof(*out));
+    out->stride = stride;
+    out->block_bytes = c->part.budget + stride;
+    out->layout_bytes = (uint64_t)c->part.n * out->block_bytes;
+    out->mapped_nodes = c->part.layout_mapped;
+    out->metadata_entries = c->part.budget / 64;
+  }
+  return RTX_OK;
+}
+
+rtx_status rtx_get_treelet_layout(rtx_ctx* c, uint64_t* remap, int32_t* metadata) {
+  if (!c) return RTX_ERR_INVALID_ARGUMENT;
+  if (!c->part.valid || !c->part.layout_remap) return RTX_ERR_NO_PARTITION;
+  HIP_TRY(hipSetDevice(c->device));
+  if (remap)
+    HIP_TRY(hipMemcpyAsync(remap, c->part.layout_remap, (c->arena_bytes / 64) * sizeof(uint64_t),
+                           hipMemcpyDeviceToHost, c->stream));
+  if (metadata)
+    HIP_TRY(hipMemcpyAsync(metadata, c->part.layout_metadata, (size_t)c->part.n * (c->part.budget / 64) * 4,
+                           hipMemcpyDeviceToHost, c->stream));
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  return RTX_OK;
+}
+
 rtx_status rtx_set_treelets(rtx_ctx* c, uint32_t budget, uint32_t n, const uint64_t* roots,
                             const uint64_t* list_offsets, const uint64_t* list_nodes, const uint32_t* owner) {
   if (!c || !roots || !list_offsets || !list_nodes || !owner || n == 0) return RTX_ERR_INVALID_ARGUMENT;
@@ -499,11 +532,12 @@ rtx_status rtx_trace(rtx_ctx* c, const rtx_ray* rays, uint32_t n, const rtx_trac
                      rtx_record* records, uint64_t capacity, uint64_t* offsets) {
   if (!c || !opts || (n > 0 && (!rays || !hits))) return RTX_ERR_INVALID_ARGUMENT;
   if (opts->mode != RTX_MODE_PLAIN && opts->mode != RTX_MODE_TREELET) return RTX_ERR_INVALID_ARGUMENT;
-  if (opts->prefetch > RTX_PREFETCH_LEARNED) return RTX_ERR_INVALID_ARGUMENT;
+  if (opts->prefetch > RTX_PREFETCH_LATTER) return RTX_ERR_INVALID_ARGUMENT;
   if (opts->emit_records && !offsets) return RTX_ERR_INVALID_ARGUMENT;
   if (n > 0 ? opts->claim_start >= n : opts->claim_start != 0) return RTX_ERR_INVALID_ARGUMENT;
-  if (opts->record_slab > 1024) return RTX_ERR_INVALID_ARGUMENT;
+  if (opts->record_slab > 1024 || opts->record_layout > 1) return RTX_ERR_INVALID_ARGUMENT;
   if (!c->arena) return RTX_ERR_NO_SCENE;
+  if (opts->record_layout && (!c->part.valid || !c->part.layout_remap)) return RTX_ERR_NO_PARTITION;
   if ((opts->mode == RTX_MODE_TREELET || opts->prefetch != RTX_PREFETCH_NONE) && !c->part.valid)
     return RTX_ERR_NO_PARTITION;
   if (((uintptr_t)rays & 15) || ((uintptr_t)hits & 15) || ((uintptr_t)records & 15)) return RTX_ERR_INVALID_ARGUMENT;
@@ -586,6 +620,7 @@ rtx_status rtx_trace(rtx_ctx* c, const rtx_ray* rays, uint32_t n, const rtx_trac
   p.record_offsets = nullptr;
   p.claim_start = opts->claim_start;
   p.ray_list = nullptr;
+  p.layout_remap = opts->record_layout ? c->part.layout_remap : nullptr;
   p.owner = c->part.valid ? c->part.owner : nullptr;
   p.first_treelet = c->part.valid ? c->part.first_treelet : 0;
   p.spill = c->spill;
@@ -652,7 +687,7 @@ rtx_status rtx_trace(rtx_ctx* c, const rtx_ray* rays, uint32_t n, const rtx_trac
   HIP_TRY(hipEventRecord(c->evk1, c->stream));
   c->kernel_timed = true;
   // staging statistics of the main traversal (a second pass traces some rays again)
-  unsigned long long stats[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  unsigned long long stats[11] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
   if (stage) HIP_TRY(hipMemcpyAsync(stats, p.stats, sizeof(stats), hipMemcpyDeviceToHost, c->stream));
   if (opts->emit_records) {
     const size_t tiles = (n + 1023) / 1024;
@@ -706,6 +741,10 @@ rtx_status rtx_trace(rtx_ctx* c, const rtx_ray* rays, uint32_t n, const rtx_trac
   c->last_stats.predictions = stats[5];
   c->last_stats.prediction_hits = stats[6];
   c->last_stats.fallbacks = stats[7];
+  c->last_stats.timely_bytes = stats[8];
+  c->last_stats.too_early_bytes = stats[9];
+  c->last_stats.too_late_bytes = stats[10];
+  c->last_stats.late_bytes = 0;
   return launch_error();
 }
 

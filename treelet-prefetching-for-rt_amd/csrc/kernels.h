@@ -27,7 +27,7 @@ enum : int {
 };
 
 constexpr uint32_t kBlock = 256;        // threads per block (4 waves of 64)
-constexpr uint32_t kSlotWords = 32;     // control slot: [0] ray counter, [1] error, stats from word 16
+constexpr uint32_t kSlotWords = 64;     // control slot: [0] ray counter, [1] error, [2] overflow count, stats (u64) from word 16
 constexpr uint32_t kMaxStepsPerRay = 1u << 24;
 constexpr uint32_t kMaxBatchJobs = 1024;
 
@@ -57,6 +57,7 @@ struct TraceParams {
   uint64_t record_capacity;
   const uint64_t* record_offsets;
   uint32_t record_slab;    // slab records: ray r's first record_slab records at records[r * record_slab]
+  const uint64_t* layout_remap;  // nullable: records carry these layout offsets (arena slot -> offset)
   uint32_t claim_start;    // rays are claimed from this index on, wrapping to 0
   const uint32_t* ray_list;  // nullable: claim k traces ray ray_list[k] (n = list length)
   const uint32_t* owner;  // node slot -> dense treelet index (nullable in plain mode)

@@ -29,6 +29,11 @@ struct Partition {
   uint64_t images_total = 0;
   uint32_t max_image_bytes = 0;
   rtx_treelet_summary summary{};
+  // treelet layout (remapBVHToTreeletLayout), built on request
+  uint64_t* layout_remap = nullptr;    // [arena_bytes/64] layout offset of each node, ~0 for none
+  int32_t* layout_metadata = nullptr;  // [n * budget/64] entry places relative to the root's, -1 past a list
+  uint32_t layout_stride = 0;
+  uint64_t layout_mapped = 0;          // nodes placed
 };
 
 void partition_free(Partition& p);
@@ -40,6 +45,11 @@ rtx_status partition_install(const uint8_t* arena, uint64_t bytes, uint64_t tlas
                              const uint64_t* roots, const uint64_t* list_offsets, const uint64_t* list_nodes,
                              const uint32_t* owner, hipStream_t s, Partition& out);
 bool boxes_ordered(const uint8_t* arena, uint64_t bytes, uint64_t tlas, hipStream_t s);
+// remapBVHToTreeletLayout of the partition with -treelet_remap_stride `stride`
+// (and the per-treelet metadata table); RTX_ERR_UNSUPPORTED where the
+// reference asserts (a treelet root already placed by an earlier treelet).
+rtx_status partition_layout(const uint8_t* arena, uint64_t bytes, uint64_t tlas, uint32_t stride, hipStream_t s,
+                            Partition& p);
 rtx_status partition_validate(const uint8_t* arena, uint64_t bytes, uint64_t tlas, hipStream_t s,
                               uint64_t* n_nodes);
 

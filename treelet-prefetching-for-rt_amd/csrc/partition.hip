@@ -691,7 +691,73 @@ rtx_status build_images(const uint8_t* A, uint64_t bytes, hipStream_t s, const u
   return RTX_OK;
 }
 
+// ---- treelet layout: remapBVHToTreeletLayout (vulkan_ray_tracing.cc:1473-1509) ----
+// Treelet t's block starts at t * (budget + stride): its root first, then the
+// other nodes of its list in order, each advancing by its size (instance leaf
+// 128 bytes, every other node 64).  A node listed by several treelets keeps
+// the place of the first (lowest index, i.e. lowest root address): each list
+// entry proposes (treelet << 32 | place in block) and the slot keeps the
+// minimum.  A wave scans one treelet's list.
+__global__ __launch_bounds__(256) void layout_propose(uint32_t n, const uint64_t* roots, const uint64_t* lo,
+                                                      const uint64_t* ln, const uint8_t* kind,
+                                                      unsigned long long* key) {
+  const uint32_t lane = threadIdx.x & 63u;
+  const uint32_t wave = blockIdx.x * 4 + (threadIdx.x >> 6);
+  for (uint32_t t = wave; t < n; t += gridDim.x * 4) {
+    const uint32_t rs = (uint32_t)(roots[t] >> 6);
+    // the root sits at the block start; the others follow it
+    uint32_t place = kind[rs] == kSlotInstance ? 128u : 64u;
+    if (lane == 0) atomicMin(&key[rs], (unsigned long long)t << 32);
+    for (uint64_t j0 = lo[t]; j0 < lo[t + 1]; j0 += 64) {
+      const uint64_t j = j0 + lane;
+      uint32_t sz = 0, s = 0;
+      if (j < lo[t + 1]) {
+        s = (uint32_t)(ln[j] >> 6);
+        if (s != rs) sz = kind[s] == kSlotInstance ? 128u : 64u;
+      }
+      const uint32_t incl = wave_incl_scan<uint32_t>(sz, lane);
+      if (sz) atomicMin(&key[s], ((unsigned long long)t << 32) | (place + incl - sz));
+      place += __shfl(incl, 63, 64);
+    }
+  }
+}
+
+__global__ __launch_bounds__(256) void layout_place(uint64_t slots, const unsigned long long* key, uint64_t block,
+                                                    uint64_t* remap, unsigned long long* placed) {
+  uint64_t count = 0;
+  for (uint64_t s = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; s < slots; s += (uint64_t)gridDim.x * blockDim.x) {
+    const unsigned long long k = key[s];
+    remap[s] = k == ~0ull ? ~0ull : (k >> 32) * block + (k & 0xffffffffull);
+    count += k == ~0ull ? 0u : 1u;
+  }
+  for (int o = 32; o > 0; o >>= 1) count += __shfl_xor(count, o, 64);
+  if ((threadIdx.x & 63u) == 0 && count) atomicAdd(placed, (unsigned long long)count);
+}
+
+// metadata[t][j]: place of list entry j relative to the root's place; also
+// flags a root that an earlier treelet placed (the reference asserts there)
+__global__ __launch_bounds__(256) void layout_metadata_kernel(uint32_t n, uint32_t per, const uint64_t* roots,
+                                                              const uint64_t* lo, const uint64_t* ln,
+                                                              const unsigned long long* key, const uint64_t* remap,
+                                                              int32_t* meta, uint32_t* conflict) {
+  const uint32_t lane = threadIdx.x & 63u;
+  const uint32_t wave = blockIdx.x * 4 + (threadIdx.x >> 6);
+  for (uint32_t t = wave; t < n; t += gridDim.x * 4) {
+    const uint32_t rs = (uint32_t)(roots[t] >> 6);
+    if (lane == 0 && (uint32_t)(key[rs] >> 32) != t) atomicOr(conflict, 1u);
+    const int64_t r = (int64_t)remap[rs];
+    const uint64_t len = lo[t + 1] - lo[t];
+    for (uint32_t j = lane; j < per; j += 64) {
+      int32_t v = -1;
+      if (j < len) v = (int32_t)((int64_t)remap[ln[lo[t] + j] >> 6] - r);
+      meta[(uint64_t)t * per + j] = v;
+    }
+  }
+}
+
 void partition_free(Partition& p) {
+  hipFree(p.layout_remap);
+  hipFree(p.layout_metadata);
   hipFree(p.owner);
   hipFree(p.roots);
   hipFree(p.list_offsets);
@@ -702,6 +768,63 @@ void partition_free(Partition& p) {
   hipFree(p.stage_slot);
   hipFree(p.image_slots);
   p = Partition{};
+}
+
+rtx_status partition_layout(const uint8_t* A, uint64_t bytes, uint64_t tlas, uint32_t stride, hipStream_t s,
+                            Partition& p) {
+  if (!p.valid) return RTX_ERR_NO_PARTITION;
+  const uint64_t slots = bytes / 64;
+  const uint32_t per = p.budget / 64;
+  hipFree(p.layout_remap);
+  hipFree(p.layout_metadata);
+  p.layout_remap = nullptr;
+  p.layout_metadata = nullptr;
+  Scratch sc;
+  uint8_t* kind;
+  unsigned long long* key;
+  unsigned long long* placed;
+  uint32_t* conflict;
+  CS(sc.alloc(&kind, slots));
+  CS(sc.alloc(&key, slots));
+  CS(sc.alloc(&placed, 1));
+  CS(sc.alloc(&conflict, 1));
+  CS(run_validate(A, bytes, tlas, s, sc, kind, nullptr, nullptr));
+  uint64_t* remap;
+  int32_t* meta;
+  CS(dmalloc(&remap, slots));
+  rtx_status st = dmalloc(&meta, (size_t)p.n * per);
+  if (st != RTX_OK) {
+    hipFree(remap);
+    return st;
+  }
+  auto fail = [&](rtx_status e) {
+    hipFree(remap);
+    hipFree(meta);
+    return e;
+  };
+  if (hipMemsetAsync(key, 0xff, slots * 8, s) != hipSuccess || hipMemsetAsync(placed, 0, 8, s) != hipSuccess ||
+      hipMemsetAsync(conflict, 0, 4, s) != hipSuccess)
+    return fail(RTX_ERR_HIP);
+  const uint32_t g = std::max(1u, std::min<uint32_t>(4096, (p.n + 3) / 4));
+  hipLaunchKernelGGL(layout_propose, dim3(g), dim3(256), 0, s, p.n, p.roots, p.list_offsets, p.list_nodes, kind, key);
+  const uint32_t gs = (uint32_t)std::min<uint64_t>(8192, (slots + 255) / 256);
+  hipLaunchKernelGGL(layout_place, dim3(std::max(gs, 1u)), dim3(256), 0, s, slots, key,
+                     (uint64_t)p.budget + stride, remap, placed);
+  hipLaunchKernelGGL(layout_metadata_kernel, dim3(g), dim3(256), 0, s, p.n, per, p.roots, p.list_offsets,
+                     p.list_nodes, key, remap, meta, conflict);
+  if (hipGetLastError() != hipSuccess) return fail(RTX_ERR_HIP);
+  uint32_t bad = 0;
+  unsigned long long mapped = 0;
+  if (hipMemcpyAsync(&bad, conflict, 4, hipMemcpyDeviceToHost, s) != hipSuccess ||
+      hipMemcpyAsync(&mapped, placed, 8, hipMemcpyDeviceToHost, s) != hipSuccess ||
+      hipStreamSynchronize(s) != hipSuccess)
+    return fail(RTX_ERR_HIP);
+  if (bad) return fail(RTX_ERR_UNSUPPORTED);
+  p.layout_remap = remap;
+  p.layout_metadata = meta;
+  p.layout_stride = stride;
+  p.layout_mapped = mapped;
+  return RTX_OK;
 }
 
 // Whether every internal node reachable from the TLAS has, for each child it

@@ -173,7 +173,13 @@ void trace_kernel(TraceParams p) {
   // slot table of the staging area: treelet, units held, iteration of the last
   // read served, staged on a prediction
   constexpr uint32_t MAXSLOTS = STAGE ? kMaxStageSlots : 1;
-  __shared__ uint32_t sl_t[MAXSLOTS], sl_units[MAXSLOTS], sl_last[MAXSLOTS], sl_pred[MAXSLOTS];
+  __shared__ uint32_t sl_t[MAXSLOTS], sl_units[MAXSLOTS], sl_last[MAXSLOTS], sl_pred[MAXSLOTS], sl_first[MAXSLOTS];
+  // prefetch effectiveness (src/gpgpu-sim/gpu-cache.cc:1960-2117): node slots
+  // of staged units evicted unread (a later global read of one is TOO_EARLY)
+  // and of recent global reads (staging one afterwards is TOO_LATE), each in a
+  // direct-mapped table (a collision forgets the older slot)
+  constexpr uint32_t EV = STAGE ? 128 : 1;
+  __shared__ uint32_t ev_key[EV], dm_key[EV];
 
   const uint32_t tid = threadIdx.x;
   const uint32_t lane = tid & 63u;
@@ -222,7 +228,8 @@ void trace_kernel(TraceParams p) {
   // staging state (wave-uniform)
   uint32_t cooldown = 0, iter = 0, n_staged = 0;
   uint64_t st_treelets = 0, st_bytes = 0, st_wasted = 0, st_pred = 0, st_pred_hits = 0, st_fallback = 0;
-  uint32_t reads = 0, lds_hits = 0;
+  uint64_t st_timely = 0;
+  uint32_t reads = 0, lds_hits = 0, too_early = 0, too_late = 0;
   uint32_t octant = 0;  // direction octant of the lane's ray (learned policy)
   // slots of `cap` units each (powers of two): nslots treelets stay resident
   uint32_t cap_shift = 0, nslots = 1;
@@ -232,11 +239,13 @@ void trace_kernel(TraceParams p) {
     while ((1u << cap_shift) < cap) cap_shift++;
     nslots = min(MAXSLOTS, UNITS >> cap_shift);
     for (uint32_t i = lane; i < HSIZE; i += 64) hkey[i] = 0;
+    for (uint32_t i = lane; i < EV; i += 64) ev_key[i] = dm_key[i] = 0;
     for (uint32_t i = lane; i < MAXSLOTS; i += 64) {
       sl_t[i] = kNone;
       sl_units[i] = 0;
       sl_last[i] = 0;
       sl_pred[i] = 0;
+      sl_first[i] = 0;
     }
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
@@ -251,9 +260,10 @@ void trace_kernel(TraceParams p) {
       // by a second pass over the rays that had more
       if (REC == kRecSlab ? n_rec < p.record_slab : idx < p.record_capacity) {
         uint32_t tl = owner ? owner[off >> 6] : RTX_NO_TREELET;
+        const uint64_t roff = p.layout_remap ? p.layout_remap[off >> 6] : off;
         uint4 v;
-        v.x = (uint32_t)off;
-        v.y = (uint32_t)(off >> 32);
+        v.x = (uint32_t)roff;
+        v.y = (uint32_t)(roff >> 32);
         v.z = tl;
         v.w = size | (type << 16) | ((lvl > 255u ? 255u : lvl) << 24);
         reinterpret_cast<uint4*>(p.records)[idx] = v;
@@ -317,6 +327,11 @@ void trace_kernel(TraceParams p) {
         for (int k = 0; k < count; ++k) w[k] = q[k];
         return;
       }
+      // a demand read from global memory: TOO_EARLY when the node's unit
+      // was staged and evicted unread; remembered for TOO_LATE
+      const uint32_t s = (uint32_t)(off >> 6), h = slot_hash(s) & (EV - 1);
+      if (ev_key[h] == s + 1 && atomicCAS(&ev_key[h], s + 1, 0u) == s + 1) too_early++;
+      dm_key[h] = s + 1;
     }
     const uint4* q = reinterpret_cast<const uint4*>(A + off + first);
     for (int k = 0; k < count; ++k) w[k] = q[k];
@@ -366,7 +381,14 @@ void trace_kernel(TraceParams p) {
       for (uint32_t i = lane; i < units; i += 64) u += used[base + i] ? 1u : 0u;
       u = wave_sum(u);
       st_wasted += (uint64_t)(units - u) * 64;
+      st_timely += (uint64_t)u * 64;
       if (sl_pred[j] && u > 0) st_pred_hits++;
+      const uint64_t img = p.image_offset[sl_t[j]] / 64 + sl_first[j];
+      for (uint32_t i = lane; i < units; i += 64) {
+        if (used[base + i]) continue;
+        const uint32_t s = p.image_slots[img + i];
+        if (s != kNone) ev_key[slot_hash(s) & (EV - 1)] = s + 1;
+      }
       wave_sync();
       if (lane == 0) {
         sl_t[j] = kNone;
@@ -382,23 +404,25 @@ void trace_kernel(TraceParams p) {
       for (uint32_t j = 0; j < nslots; ++j) evict_slot(j);
   };
 
-  // Copy treelet t (its first `max_units` 64-byte units) into a slot
-  // (wave-uniform): the slot already holding it, else an empty one, else the
-  // least recently read one that served no read in this iteration.  The hash
-  // table is rebuilt over every resident slot.  Returns whether it copied.
-  auto stage = [&](uint32_t t, uint32_t max_units, bool predicted) -> bool {
+  // Copy treelet t (at most `max_units` 64-byte units of its image: the first
+  // ones, or the last ones when `latter`) into a slot (wave-uniform): the slot
+  // already holding it, else an empty one, else the least recently read one
+  // that served no read in this iteration.  The hash table is rebuilt over
+  // every resident slot.  Returns whether it copied.
+  auto stage = [&](uint32_t t, uint32_t max_units, bool predicted, bool latter = false) -> bool {
     if constexpr (STAGE > 0) {
       const uint32_t tt = __builtin_amdgcn_readfirstlane(t);
-      uint32_t units = p.image_bytes[tt] / 64;
-      units = min(units, min(1u << cap_shift, max_units));
+      const uint32_t total = p.image_bytes[tt] / 64;
+      uint32_t units = min(total, min(1u << cap_shift, max_units));
       units = __builtin_amdgcn_readfirstlane(units);
       if (units == 0) return false;
-      // victim: the slot holding t, else an empty slot, else the least
-      // recently read slot that served no read in this iteration
+      const uint32_t first = latter ? total - units : 0u;
+      // victim: the slot holding that part of t, else an empty slot, else the
+      // least recently read slot that served no read in this iteration
       uint32_t victim = kNone, empty = kNone, lru = kNone, lru_it = 0;
       for (uint32_t j = 0; j < nslots; ++j) {
         const uint32_t sj = sl_t[j];
-        if (sj == tt) victim = j;
+        if (sj == tt && sl_first[j] == first) victim = j;
         if (sj == kNone) {
           if (empty == kNone) empty = j;
         } else if (sl_last[j] < iter && (lru == kNone || sl_last[j] < lru_it)) {
@@ -419,7 +443,7 @@ void trace_kernel(TraceParams p) {
       victim = __builtin_amdgcn_readfirstlane(victim);
       evict_slot(victim);
       const uint32_t base = victim << cap_shift;
-      const uint4* src = reinterpret_cast<const uint4*>(p.images + p.image_offset[tt]);
+      const uint4* src = reinterpret_cast<const uint4*>(p.images + p.image_offset[tt]) + first * 4;
       for (uint32_t i = lane; i < units * 4; i += 64) stage_img[base * 4 + i] = src[i];
       for (uint32_t i = lane; i < units; i += 64) used[base + i] = 0;
       for (uint32_t i = lane; i < HSIZE; i += 64) hkey[i] = 0;
@@ -429,6 +453,7 @@ void trace_kernel(TraceParams p) {
         sl_units[victim] = units;
         sl_last[victim] = iter;
         sl_pred[victim] = predicted ? 1u : 0u;
+        sl_first[victim] = first;
       }
       wave_sync();
       // insert every resident unit: unit i of the staging area is unit
@@ -437,8 +462,12 @@ void trace_kernel(TraceParams p) {
         const uint32_t j = i >> cap_shift, u = i & ((1u << cap_shift) - 1);
         const uint32_t sj = sl_t[j];
         if (sj == kNone || u >= sl_units[j]) continue;
-        const uint32_t s = p.image_slots[p.image_offset[sj] / 64 + u];
+        const uint32_t s = p.image_slots[p.image_offset[sj] / 64 + sl_first[j] + u];
         if (s == kNone) continue;
+        if (j == victim) {
+          const uint32_t hd = slot_hash(s) & (EV - 1);
+          if (dm_key[hd] == s + 1 && atomicCAS(&dm_key[hd], s + 1, 0u) == s + 1) too_late++;
+        }
         uint32_t h = slot_hash(s) & (HSIZE - 1);
         for (uint32_t probe = 0; probe < HSIZE; ++probe) {
           if (atomicCAS(&hkey[h], 0u, s + 1) == 0u) {
@@ -655,14 +684,20 @@ void trace_kernel(TraceParams p) {
         if (cooldown > 0) {
           cooldown--;
         } else if (nact > 0 && 2 * nmiss > nact) {
+          // The RT unit's vote (src/gpgpu-sim/shader.cc:3424-3446): the
+          // treelet of every lane's next node is counted, and the most
+          // popular one wins, ties going to the lowest root offset (the
+          // ordered map's first maximum; dense treelet indices follow root
+          // offsets).  Each round takes one distinct treelet, so the loop
+          // ends after at most 64 rounds.
           const uint32_t own = (want && off + 64 <= bytes) ? owner[slot] : kNone;
-          const uint32_t t = pos == kNone ? own : kNone;
-          uint64_t rem = __ballot(t != kNone);
+          uint64_t rem = __ballot(own != kNone);
+          const uint32_t voters = (uint32_t)__popcll(rem);
           uint32_t best = kNone, bestc = 0;
-          for (int c = 0; c < 4 && rem; ++c) {
+          while (rem) {
             const uint32_t lead = (uint32_t)__ffsll((long long)rem) - 1u;
-            const uint32_t cand = __shfl(t, (int)lead, 64);
-            const uint64_t m = __ballot(t == cand);
+            const uint32_t cand = __shfl(own, (int)lead, 64);
+            const uint64_t m = __ballot(own == cand);
             const uint32_t cnt = (uint32_t)__popcll(m);
             if (cnt > bestc || (cnt == bestc && cand < best)) {
               best = cand;
@@ -670,19 +705,21 @@ void trace_kernel(TraceParams p) {
             }
             rem &= ~m;
           }
-          const float share = (float)bestc / (float)nact;
-          bool ok = best != kNone && bestc >= 2 && bestc > nact - nmiss;
+          // heuristics 0-3 (:3437-3559): always; share >= threshold; the
+          // first round(n * share) units; the last round(n * share) units
+          const float share = voters ? (float)bestc / (float)voters : 0.f;
+          bool ok = best != kNone;
           if (p.prefetch == RTX_PREFETCH_POPULARITY && share < p.threshold) ok = false;
+          bool staged = false;
           if (ok) {
             uint32_t maxu = UNITS;
-            if (p.prefetch == RTX_PREFETCH_PARTIAL) {
-              const uint32_t units = p.image_bytes[best] / 64;
-              maxu = (uint32_t)((float)units * share + 0.5f);
-            }
-            stage(best, maxu, false);
-          } else {
-            cooldown = 8;
+            const bool part = p.prefetch == RTX_PREFETCH_PARTIAL || p.prefetch == RTX_PREFETCH_LATTER;
+            if (part) maxu = (uint32_t)((float)(p.image_bytes[best] / 64) * share + 0.5f);
+            // a treelet already resident is not staged again (the "same root as
+            // the last prefetch" check of :3564)
+            staged = stage(best, maxu, false, p.prefetch == RTX_PREFETCH_LATTER);
           }
+          if (!staged) cooldown = 8;
           if (p.prefetch == RTX_PREFETCH_LEARNED) {
             // Each lane's predicted next treelets for (its treelet, its
             // octant).  A predicted treelet (its first pred_fraction of units)
@@ -989,7 +1026,11 @@ void trace_kernel(TraceParams p) {
   if constexpr (STAGE > 0) {
     evict_all();
     const uint32_t r = wave_sum(reads), hl = wave_sum(lds_hits);
+    const uint32_t te = wave_sum(too_early), tl = wave_sum(too_late);
     if (lane == 0) {
+      atomicAdd(&p.stats[8], (unsigned long long)st_timely);
+      atomicAdd(&p.stats[9], (unsigned long long)te * 64ull);
+      atomicAdd(&p.stats[10], (unsigned long long)tl * 64ull);
       atomicAdd(&p.stats[0], (unsigned long long)r);
       atomicAdd(&p.stats[1], (unsigned long long)hl);
       atomicAdd(&p.stats[2], (unsigned long long)st_treelets);
@@ -1526,19 +1567,32 @@ TraceFn pick_trace(uint32_t mode, int records, uint32_t lds_stack, uint32_t stag
 }
 
 // ---- slab records -> CSR ----------------------------------------------------
-// Thread t copies record t % slab of ray t / slab when the ray has that many;
-// the thread of a ray's record `slab` (one past the slab) lists the ray for
-// the second pass instead.
+// A wave takes 64 rays at a time: lane i loads ray i's record count and CSR
+// offset, then the wave copies the rays' slab records one ray after another,
+// lane k moving record k (16-byte loads and stores, coalesced).  A ray with
+// more records than its slab is listed for the second pass instead.
 __global__ __launch_bounds__(256) void compact_records(const rtx_hit* hits, uint32_t n, const uint4* slabs,
                                                        uint32_t slab, const uint64_t* offsets, uint4* records,
                                                        uint32_t* overflow_list, uint32_t* overflow_count) {
-  // the host keeps n * slab below 2^31
-  const uint32_t total = n * slab;
-  for (uint32_t t = blockIdx.x * blockDim.x + threadIdx.x; t < total; t += gridDim.x * blockDim.x) {
-    const uint32_t r = t / slab, k = t - r * slab;
-    const uint32_t cnt = hits[r].n_records;
-    if (k < cnt) records[offsets[r] + k] = slabs[t];
-    if (k == 0 && cnt > slab) overflow_list[atomicAdd(overflow_count, 1u)] = r;
+  const uint32_t lane = threadIdx.x & 63u;
+  const uint32_t wave = blockIdx.x * 4 + (threadIdx.x >> 6);
+  const uint32_t waves = gridDim.x * 4;
+  for (uint32_t base = wave * 64; base < n; base += waves * 64) {
+    const uint32_t r = base + lane;
+    uint32_t cnt = 0;
+    uint64_t off = 0;
+    if (r < n) {
+      cnt = hits[r].n_records;
+      off = offsets[r];
+      if (cnt > slab) overflow_list[atomicAdd(overflow_count, 1u)] = r;
+    }
+    const uint32_t m = min(64u, n - base);
+    for (uint32_t i = 0; i < m; ++i) {
+      const uint32_t c = min(__shfl(cnt, (int)i, 64), slab);
+      const uint64_t o = __shfl(off, (int)i, 64);
+      const uint4* src = slabs + (uint64_t)(base + i) * slab;
+      for (uint32_t k = lane; k < c; k += 64) records[o + k] = src[k];
+    }
   }
 }
 
@@ -1594,8 +1648,7 @@ hipError_t launch_compact_records(const rtx_hit* hits, uint32_t n, const rtx_rec
                                   const uint64_t* offsets, rtx_record* records, uint32_t* overflow_list,
                                   uint32_t* overflow_count, hipStream_t s) {
   if (n == 0 || slab == 0) return hipSuccess;
-  const uint64_t total = (uint64_t)n * slab;
-  const uint32_t grid = (uint32_t)std::min<uint64_t>(65536, (total + 255) / 256);
+  const uint32_t grid = std::max(1u, std::min<uint32_t>(4096, (n + 255) / 256));
   hipLaunchKernelGGL(compact_records, dim3(grid), dim3(256), 0, s, hits, n, reinterpret_cast<const uint4*>(slabs),
                      slab, offsets, reinterpret_cast<uint4*>(records), overflow_list, overflow_count);
   return hipGetLastError();

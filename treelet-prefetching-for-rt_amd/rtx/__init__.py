@@ -32,6 +32,7 @@ STATUS_NAMES = {
 }
 MODE_PLAIN, MODE_TREELET = 0, 1
 PREFETCH_NONE, PREFETCH_ALWAYS, PREFETCH_POPULARITY, PREFETCH_PARTIAL, PREFETCH_LEARNED = 0, 1, 2, 3, 4
+PREFETCH_LATTER = 5
 RAY_FLAG_OPAQUE, RAY_FLAG_TERMINATE_ON_FIRST_HIT, RAY_FLAG_SKIP_CLOSEST_HIT_SHADER = 0x1, 0x4, 0x8
 TX_BVH_STRUCTURE, TX_BVH_INTERNAL_NODE, TX_BVH_INSTANCE_LEAF = 0, 1, 2
 TX_BVH_PRIMITIVE_LEAF_DESCRIPTOR, TX_BVH_QUAD_LEAF, TX_BVH_QUAD_LEAF_HIT, TX_BVH_PROCEDURAL_LEAF = 3, 4, 5, 6
@@ -64,12 +65,19 @@ class _TraceOptions(ctypes.Structure):
                 ("lds_stack", ctypes.c_uint32), ("stage_bytes", ctypes.c_uint32), ("refill", ctypes.c_uint32),
                 ("slot_bytes", ctypes.c_uint32), ("min_score", ctypes.c_float), ("pred_fraction", ctypes.c_float),
                 ("lanes_per_ray", ctypes.c_uint32), ("claim_start", ctypes.c_uint32),
-                ("record_slab", ctypes.c_uint32)]
+                ("record_layout", ctypes.c_uint32), ("record_slab", ctypes.c_uint32)]
+
+
+class _LayoutSummary(ctypes.Structure):
+    _fields_ = [("stride", ctypes.c_uint32), ("block_bytes", ctypes.c_uint32), ("layout_bytes", ctypes.c_uint64),
+                ("mapped_nodes", ctypes.c_uint64), ("metadata_entries", ctypes.c_uint32),
+                ("reserved", ctypes.c_uint32)]
 
 
 class _PrefetchStats(ctypes.Structure):
     _fields_ = [(n, ctypes.c_uint64) for n in ("node_reads", "lds_hits", "staged_treelets", "staged_bytes",
-                                               "wasted_bytes", "predictions", "prediction_hits", "fallbacks")]
+                                               "wasted_bytes", "predictions", "prediction_hits", "fallbacks",
+                                               "timely_bytes", "too_early_bytes", "too_late_bytes", "late_bytes")]
 
 
 class _TreeletSummary(ctypes.Structure):
@@ -126,6 +134,8 @@ _sig("rtx_get_treelets", _i32, _vp, _vp, _vp, _vp, _vp)
 _sig("rtx_set_treelets", _i32, _vp, _u32, _u32, _vp, _vp, _vp, _vp)
 _sig("rtx_trace", _i32, _vp, _vp, _u32, _P(_TraceOptions), _vp, _vp, _u64, _vp)
 _sig("rtx_trace_static", _i32, _vp, _vp, _u32, _P(_TraceOptions), _vp)
+_sig("rtx_build_treelet_layout", _i32, _vp, _u32, _P(_LayoutSummary))
+_sig("rtx_get_treelet_layout", _i32, _vp, _vp, _vp)
 _sig("rtx_adapter_create", _i32, _i32, _vp, _u64, _u64, _u32, _P(_vp))
 _sig("rtx_adapter_destroy", _i32, _vp)
 _sig("rtx_adapter_trace_ray", _i32, _vp, _i32, _u32, _u32, _u32, _u32, _u32, _P(_f32), _f32, _P(_f32), _f32, _vp, _vp,
@@ -178,7 +188,7 @@ EXPORTED_SYMBOLS = [
     "rtx_status_string", "rtx_version", "rtx_device_count", "rtx_create", "rtx_destroy", "rtx_synchronize",
     "rtx_malloc", "rtx_free", "rtx_memcpy_h2d", "rtx_memcpy_d2h", "rtx_memcpy_d2d", "rtx_memset", "rtx_timer_start",
     "rtx_timer_stop", "rtx_device_info", "rtx_set_scene", "rtx_set_scene_device", "rtx_validate_scene",
-    "rtx_build_treelets", "rtx_get_treelets", "rtx_set_treelets", "rtx_trace", "rtx_trace_static", "rtx_get_prefetch_stats",
+    "rtx_build_treelets", "rtx_get_treelets", "rtx_set_treelets", "rtx_build_treelet_layout", "rtx_get_treelet_layout", "rtx_trace", "rtx_trace_static", "rtx_get_prefetch_stats",
     "rtx_last_kernel_ms", "rtx_add_scene", "rtx_build_scene_treelets", "rtx_trace_batch", "rtx_batch_graph_create",
     "rtx_batch_graph_launch", "rtx_batch_graph_destroy", "rtx_autotune_batch", "rtx_autotune_batch_treelets",
     "rtx_set_prefetch_table", "rtx_mlp_create", "rtx_mlp_destroy", "rtx_mlp_param_count", "rtx_mlp_get_params",
@@ -579,6 +589,21 @@ class Context:
         return {"roots": roots, "list_offsets": lo, "list_nodes": ln[: self.summary["n_list_entries"]],
                 "owner": owner}
 
+    def build_treelet_layout(self, stride=0):
+        """remapBVHToTreeletLayout of the current partition; returns the summary as a dict."""
+        s = _LayoutSummary()
+        _check(_lib.rtx_build_treelet_layout(self._h, stride, ctypes.byref(s)), "rtx_build_treelet_layout")
+        self.layout = {f: getattr(s, f) for f, _ in _LayoutSummary._fields_}
+        return self.layout
+
+    def get_treelet_layout(self):
+        """(remap[arena/64], metadata[n_treelets, budget/64]) of the last built layout."""
+        n, per = self.summary["n_treelets"], self.layout["metadata_entries"]
+        remap = np.empty(self.arena_bytes // 64, np.uint64)
+        meta = np.empty((n, per), np.int32)
+        _check(_lib.rtx_get_treelet_layout(self._h, _ptr(remap), _ptr(meta)), "rtx_get_treelet_layout")
+        return remap, meta
+
     def set_treelets(self, budget, roots, list_offsets, list_nodes, owner):
         r = np.ascontiguousarray(roots, np.uint64)
         lo = np.ascontiguousarray(list_offsets, np.uint64)
@@ -593,11 +618,11 @@ class Context:
     def trace_device(self, rays_buf, n, hits_buf, mode=MODE_PLAIN, prefetch=PREFETCH_NONE, threshold=0.5,
                      records_buf=None, record_capacity=0, offsets_buf=None, waves_per_cu=0, lds_stack=0,
                      stage_bytes=0, refill=0, slot_bytes=0, min_score=0.0, pred_fraction=0.0, lanes_per_ray=0,
-                     claim_start=0, record_slab=0):
+                     claim_start=0, record_slab=0, record_layout=0):
         """rtx_trace on device buffers; returns the rtx_status (0 = ok)."""
         o = _TraceOptions(mode, prefetch, threshold, 1 if offsets_buf is not None else 0, waves_per_cu, lds_stack,
                           stage_bytes, refill, slot_bytes, min_score, pred_fraction, lanes_per_ray, claim_start,
-                          record_slab)
+                          record_layout, record_slab)
         return _lib.rtx_trace(self._h, rays_buf.ptr, n, ctypes.byref(o), hits_buf.ptr,
                               records_buf.ptr if records_buf is not None else None, record_capacity,
                               offsets_buf.ptr if offsets_buf is not None else None)
@@ -606,7 +631,7 @@ class Context:
                             lanes_per_ray=0, claim_start=0):
         """rtx_trace_static (hits only, no allocation) on device buffers; returns the rtx_status."""
         o = _TraceOptions(mode, PREFETCH_NONE, 0.0, 0, waves_per_cu, lds_stack, 0, refill, 0, 0.0, 0.0, lanes_per_ray,
-                          claim_start, 0)
+                          claim_start, 0, 0)
         return _lib.rtx_trace_static(self._h, rays_buf.ptr, n, ctypes.byref(o), hits_buf.ptr)
 
     def trace(self, rays, mode=MODE_PLAIN, prefetch=PREFETCH_NONE, threshold=0.5, records=False,
@@ -685,7 +710,7 @@ class Context:
 
     @staticmethod
     def _opts(mode=MODE_PLAIN, waves_per_cu=0, lds_stack=0, refill=0):
-        return _TraceOptions(mode, PREFETCH_NONE, 0.0, 0, waves_per_cu, lds_stack, 0, refill, 0, 0.0, 0.0, 0, 0, 0)
+        return _TraceOptions(mode, PREFETCH_NONE, 0.0, 0, waves_per_cu, lds_stack, 0, refill, 0, 0.0, 0.0, 0, 0, 0, 0)
 
     def trace_batch(self, jobs, **opts):
         arr = self._jobs(jobs)

@@ -213,3 +213,95 @@ class LearnedPrefetcher:
         if self.mlp is not None:
             self.mlp.close()
             self.mlp = None
+
+
+# ---- one network over several scenes -----------------------------------------------------
+def scene_transitions(ctx, arena, tlas, budget, rays, mode=MODE_PLAIN):
+    """Set the scene and its partition on ctx, trace `rays` with records and return the
+    scene's treelet context and its transitions (cur, next, octant)."""
+    ctx.set_scene(arena, tlas)
+    ctx.build_treelets(budget)
+    part = ctx.get_treelets()
+    parent, level = node_tree(arena, tlas)
+    root_level, anc = treelet_context(part["roots"], part["owner"], parent, level)
+    rays = np.ascontiguousarray(rays, RAY_DTYPE)
+    _, off, rec = ctx.trace(rays, mode=mode, records=True)
+    cur, nxt, rid = transitions(rays, off, rec)
+    return {"T": len(part["roots"]), "root_level": root_level, "anc": anc, "cur": cur, "nxt": nxt,
+            "oct": octants(rays)[rid]}
+
+
+class JointPrefetcher:
+    """One next-treelet network over the treelets of several scenes: scene k's treelet
+    t is embedding row offset[k] + t, the trunk is shared, and the classes are the
+    n_classes - 1 next treelets (of any scene) seen most often.  `sources` are
+    scene_transitions() results."""
+
+    def __init__(self, ctx, sources, n_classes=16384, seed=1):
+        self.ctx, self.seed, self.n_classes = ctx, seed, int(n_classes)
+        self.offset = np.cumsum([0] + [s["T"] for s in sources])[:-1].astype(np.int64)
+        self.T = int(sum(s["T"] for s in sources))
+        self.sizes = [s["T"] for s in sources]
+        self.root_level = np.concatenate([s["root_level"] for s in sources]).astype(np.float32)
+        ancs = []
+        for s, o in zip(sources, self.offset):
+            a = s["anc"].astype(np.int64)
+            ancs.append(np.where(s["anc"] == NO_TREELET, NO_TREELET, a + o).astype(np.uint32))
+        self.anc = np.concatenate(ancs)
+        self.cur = np.concatenate([s["cur"].astype(np.int64) + o for s, o in zip(sources, self.offset)])
+        self.nxt = np.concatenate([s["nxt"].astype(np.int64) + o for s, o in zip(sources, self.offset)])
+        self.oct = np.concatenate([s["oct"] for s in sources]).astype(np.uint32)
+        vals, cnt = np.unique(self.nxt, return_counts=True)
+        order = np.argsort(-cnt, kind="stable")
+        self.class_treelet = vals[order[: self.n_classes - 1]].astype(np.int64)
+        self.seen = np.zeros(self.T, bool)
+        self.seen[self.cur] = True
+        self.mlp = None
+
+    def _samples(self, cur, octs, nxt):
+        cls = np.full(self.T, self.n_classes - 1, np.uint32)
+        cls[self.class_treelet] = np.arange(len(self.class_treelet), dtype=np.uint32)
+        return make_samples(cur.astype(np.uint32), octs, self.root_level, self.anc, cls[nxt])
+
+    def fit(self, epochs=2, batch=1024, lr=2e-3, max_samples=3_000_000):
+        s = self._samples(self.cur, self.oct, self.nxt)
+        rng = np.random.default_rng(self.seed)
+        s = s[rng.permutation(len(s))[:max_samples]]
+        self.mlp = self.ctx.mlp(self.T, self.n_classes, self.seed)
+        buf = self.mlp.upload(s)
+        self.n_train = len(s)
+        self.losses = [self.mlp.train(buf, len(s), epochs=1, batch=batch, lr=lr) for _ in range(epochs)]
+        return self.losses
+
+    def accuracy(self, k, cur, nxt, octs):
+        """Top-1 / top-4 accuracy on transitions (scene-local ids) of scene k."""
+        o = self.offset[k]
+        s = self._samples(cur.astype(np.int64) + o, octs.astype(np.uint32), nxt.astype(np.int64) + o)
+        top, _ = self.mlp.predict(s)
+        C1 = len(self.class_treelet)
+        pred = np.where(top < C1, self.class_treelet[np.minimum(top, C1 - 1).astype(np.int64)], -1)
+        hit = pred == (nxt.astype(np.int64) + o)[:, None]
+        return float(hit[:, 0].mean()), float(hit.any(axis=1).mean())
+
+    def table(self, k):
+        """Prediction table of scene k (its treelets and its classes only), uint32 [T_k * 8, 4]."""
+        o, n = int(self.offset[k]), self.sizes[k]
+        t_ids = np.nonzero(self.seen[o:o + n])[0].astype(np.int64)
+        cur = np.repeat(t_ids + o, OCTANTS)
+        octs = np.tile(np.arange(OCTANTS, dtype=np.uint32), len(t_ids))
+        s = make_samples(cur.astype(np.uint32), octs, self.root_level, self.anc, np.zeros(len(cur), np.uint32))
+        top, prob = self.mlp.predict(s)
+        C1 = len(self.class_treelet)
+        tre = np.where(top < C1, self.class_treelet[np.minimum(top, C1 - 1).astype(np.int64)], -1)
+        local = tre - o
+        ok = (tre >= 0) & (local >= 0) & (local < n)
+        q = np.clip(np.rint(prob * 255.0), 0, 255).astype(np.uint32)
+        ent = np.where(ok & (q > 0), (np.where(ok, local, 0).astype(np.uint32) + 1) | (q << 24), 0).astype(np.uint32)
+        out = np.zeros((n * OCTANTS, 4), np.uint32)
+        out[(cur - o) * OCTANTS + octs] = ent
+        return out
+
+    def close(self):
+        if self.mlp is not None:
+            self.mlp.close()
+            self.mlp = None
