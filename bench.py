@@ -161,18 +161,37 @@ def training_rays_diffuse(rtx, ctx, scene):
     return np.concatenate(train)
 
 
-def train_learned(ctx, scene, train, args):
-    """Train the next-treelet network on traces of `train` and install its table."""
+def train_learned(rtx, ctx, scene, train, args, budget):
+    """Train the next-treelet network and install its table for `scene`.  The
+    1M-triangle workload trains one network on traces of both benchmark scenes
+    (the 16K-triangle sphere over a quad, 80 % of its pixels, and the held-out
+    training rays of this scene); other workloads train on their own scene."""
     from rtx import learned
 
     t0 = time.perf_counter()
-    lp = learned.LearnedPrefetcher(ctx, scene.arena, scene.tlas_offset, n_classes=16384)
-    losses = lp.fit(train, epochs=args.learn_epochs, max_samples=args.learn_samples)
-    fit_s = time.perf_counter() - t0
-    t0 = time.perf_counter()
-    tab = lp.install()
-    info = {"train_rays": len(train), "samples": lp.n_train, "classes": lp.n_classes, "epochs": args.learn_epochs,
-            "losses": [round(x, 4) for x in losses], "fit_s": round(fit_s, 2),
+    if args.workload == "diffuse":
+        d1 = c1_desc(rtx)
+        s1 = rtx.workload_scene(d1)
+        p1 = rtx.primary_rays(d1)
+        p1 = p1[np.arange(len(p1)) % 5 != 0]
+        src = [learned.scene_transitions(ctx, s1.arena, s1.tlas_offset, budget, p1),
+               learned.scene_transitions(ctx, scene.arena, scene.tlas_offset, budget, train)]  # leaves this scene set
+        lp = learned.JointPrefetcher(ctx, src, n_classes=16384)
+        losses = lp.fit(epochs=args.learn_epochs, max_samples=args.learn_samples)
+        fit_s = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        tab = lp.table(1)
+        ctx.set_prefetch_table(tab)
+        what = "traces of the 16K-triangle scene and of this scene"
+    else:
+        lp = learned.LearnedPrefetcher(ctx, scene.arena, scene.tlas_offset, n_classes=16384)
+        losses = lp.fit(train, epochs=args.learn_epochs, max_samples=args.learn_samples)
+        fit_s = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        tab = lp.install()
+        what = "traces of this scene"
+    info = {"trained_on": what, "train_rays": len(train), "samples": lp.n_train, "classes": lp.n_classes,
+            "epochs": args.learn_epochs, "losses": [round(x, 4) for x in losses], "fit_s": round(fit_s, 2),
             "table_s": round(time.perf_counter() - t0, 2), "table_rows_filled": int((tab[:, 0] != 0).sum())}
     lp.close()
     log(f"learned prefetcher: {info}")
@@ -366,7 +385,7 @@ def main():
             batch_budget = cfg["budget"]
             log(f"batch scenes: treelets at {batch_budget} B built in {time.perf_counter() - t0:.2f} s")
         if cfg["prefetch"] == 4:
-            learned_info = train_learned(ctx, scene, train_rays, args)
+            learned_info = train_learned(rtx, ctx, scene, train_rays, args, cfg["budget"])
         kw = dict(mode=cfg["order"], prefetch=cfg["prefetch"], threshold=args.threshold, stage_bytes=cfg["stage"],
                   slot_bytes=cfg["slot"])
         graph = None

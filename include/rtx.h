@@ -179,8 +179,13 @@ typedef struct rtx_trace_options {
   uint32_t record_slab;  /* records, one traversal: each ray's first record_slab records go to a
                             per-ray slab in library memory and are compacted into the caller's CSR
                             buffer; only rays with more records are traced a second time.
-                            0 = 64; rtx_trace falls back to a counting pass plus a writing pass
-                            when n * record_slab * 16 bytes exceed 8 GiB */
+                            0 = 256, halved until n * record_slab * 16 bytes fit 32 GiB (at least
+                            16); when they do not fit, rtx_trace runs a counting pass and a
+                            writing pass instead */
+  const uint32_t* claim_order; /* device array of n ray indices or NULL: claim k traces ray
+                                  claim_order[k] (outputs stay at input indices), e.g. the order of
+                                  rtx_coherence_order; claim_start must then be 0.  An index >= n is
+                                  reported as RTX_ERR_OFFSET_OUT_OF_ARENA.  rtx_trace only */
 } rtx_trace_options;
 
 /* Prefetch accounting of the last rtx_trace call.  The effectiveness classes
@@ -302,6 +307,28 @@ rtx_status rtx_trace(rtx_ctx* ctx, const rtx_ray* rays_dev, uint32_t n, const rt
                      uint64_t* record_offsets_dev);
 rtx_status rtx_get_prefetch_stats(rtx_ctx* ctx, rtx_prefetch_stats* out);
 
+/* Run-level statistics of a traced batch, the functional RT counters the
+ * reference prints at the end of a kernel (src/gpgpu-sim/gpu-sim.cc:2168-2182;
+ * kept in src/cuda-sim/cuda-sim.h:155-165), reduced on the device from the
+ * hits and, when given, the records.  Without records the primitive-leaf
+ * records are all counted as RTX_TX_BVH_QUAD_LEAF (the hit counters do not
+ * tell quad leaves, accepted quad leaves and procedural leaves apart). */
+typedef struct rtx_trace_stats {
+  uint64_t access_type[7];     /* "Ray tracing memory access distribution": records per RTX_TX_* */
+  uint64_t num_hits;           /* rt_num_hits: rays with hit_geometry */
+  uint64_t num_any_hits;       /* rt_num_any_hits: accepted hits of non-opaque rays */
+  uint64_t n_rays;             /* rt_n_total_rays */
+  uint64_t tot_nodes;          /* rt_tot_nodes_per_ray (sum of total_nodes_accessed) */
+  uint32_t max_nodes_per_ray;  /* rt_max_nodes_per_ray */
+  uint32_t max_tree_depth;     /* rt_max_tree_depth */
+  uint64_t fetch_bytes;        /* node bytes fetched, leaf descriptors excluded */
+  uint64_t accessed_bytes;     /* total accessed data size: every record (gpu-sim.cc:2008) */
+  uint32_t from_records;       /* 1 when the histogram comes from records */
+  uint32_t reserved;
+} rtx_trace_stats;
+rtx_status rtx_trace_stats_of(rtx_ctx* ctx, const rtx_hit* hits_dev, uint32_t n, const rtx_record* records_dev,
+                              uint64_t n_records, rtx_trace_stats* out);
+
 /* rtx_trace that allocates nothing.  No counterpart in the reference: it is for
  * callers that cannot allocate device memory while they trace (tight serving
  * loops, harnesses that forbid hipMalloc).  Hits only, no staging: the options
@@ -316,6 +343,38 @@ rtx_status rtx_get_prefetch_stats(rtx_ctx* ctx, rtx_prefetch_stats* out);
  * each call clears the control words it uses and waits for its kernel. */
 rtx_status rtx_trace_static(rtx_ctx* ctx, const rtx_ray* rays_dev, uint32_t n, const rtx_trace_options* opts,
                             rtx_hit* hits_dev);
+
+/* ---- ray coherence engine ------------------------------------------------------
+ * The RT unit's ray sorting (src/gpgpu-sim/ray_coherency_engine.cc; enabled by
+ * -gpgpu_rt_coherence_engine, gpu-sim.cc:354-360) as a claim order for
+ * rtx_trace.  Each ray is hashed (:393-474), rays of equal hash form a pool in
+ * input order, and packets of `packet` rays are taken from the largest pool
+ * first (ties: lowest hash; get_largest_packet :182-193).  order_dev[n]
+ * receives the ray indices in packet order; hashes_dev[n] (may be NULL) each
+ * ray's hash.  Hashes must fit 32 bits: francois_bits <= 4, grid_bits <= 10,
+ * sphere_bits <= 8. */
+enum {
+  RTX_COHERENCE_WORLD_REFERENCE = 0, /* as the reference sets it (vulkan_ray_tracing.cc:2452-2465): running
+                                        min from (0,0,0) over the TLAS root's child boxes, for the low
+                                        corner and (as written there) the high corner alike */
+  RTX_COHERENCE_WORLD_TLAS = 1,      /* the TLAS header bounds */
+  RTX_COHERENCE_WORLD_GIVEN = 2      /* world_min / world_max below */
+};
+typedef struct rtx_coherence_config {
+  char hash;               /* 'f' francois (:432-445), 'g' grid + spherical (:447-456),
+                              't' two point (:462-469), 'd' direction only (:471-474; the default) */
+  uint8_t world;           /* RTX_COHERENCE_WORLD_* */
+  uint16_t reserved;
+  uint32_t francois_bits;  /* hash_francois_bits */
+  uint32_t grid_bits;      /* hash_grid_bits */
+  uint32_t sphere_bits;    /* hash_sphere_bits */
+  float two_point_ratio;   /* hash_two_point_est_length_ratio */
+  uint32_t packet;         /* rays per packet (the reference's warp_size); 0 = 64, one wavefront */
+  float world_min[3];
+  float world_max[3];
+} rtx_coherence_config;
+rtx_status rtx_coherence_order(rtx_ctx* ctx, const rtx_ray* rays_dev, uint32_t n, const rtx_coherence_config* cfg,
+                               uint32_t* order_dev, uint32_t* hashes_dev);
 
 /* ---- learned next-treelet prefetch -----------------------------------------
  * No counterpart in the reference (its RT unit has popularity heuristics

@@ -76,8 +76,10 @@ def test_slab_records_capacity_and_staging(gpu, rtx, oracle, c1):
 
 
 def test_two_pass_records_when_slabs_exceed_the_limit(gpu, rtx, oracle, c1):
-    """786,432 rays with 1,024-record slabs would need 12 GiB of slabs: rtx_trace
+    """786,432 rays with 1,024-record slabs would need 48 GiB of slabs: rtx_trace
     falls back to a counting pass and a writing pass, with the same records."""
+    rtx_slab_limit = 32 << 30
+    assert 786432 * 1024 * 16 > rtx_slab_limit
     scene, _ = c1
     from conftest import c1_desc
 
@@ -247,3 +249,87 @@ def test_single_lane_adapter_matches_the_oracle(rtx, oracle, instanced):
             assert st == 5
     finally:
         assert lib.rtx_adapter_destroy(a) == 0
+
+
+@pytest.mark.parametrize("hash,world", [("d", 0), ("f", 0), ("g", 1), ("t", 1), ("t", 0)])
+def test_coherence_order_matches_the_oracle(gpu, rtx, oracle, instanced, hash, world):
+    """rtx_coherence_order: each ray's hash and the packet order equal the CPU
+    restatement of the reference's engine; tracing in that claim order leaves
+    every hit unchanged.  The spherical hashes go through acosf / atan2f, whose
+    GPU and C-library results may differ in the last bit: a ray whose CPU angle
+    lies within 1e-4 degrees of a bin edge may land in the neighbouring bin."""
+    import coherence
+
+    scene, rays = instanced
+    gpu.set_scene(scene.arena, scene.tlas_offset)
+    bounce, _ = scene.secondary_rays(rays, gpu.trace(rays), kind=0, seed=3)
+    batch = np.concatenate([rays, bounce])
+    cfg = dict(hash=hash, world=world, francois_bits=2, grid_bits=4, sphere_bits=4, two_point_ratio=0.5, packet=64)
+    order, hashes = gpu.coherence_order(batch, **cfg)
+    # the world box the engine used
+    A = scene.arena
+    if world == 1:
+        wmin = np.frombuffer(A[scene.tlas_offset + 8: scene.tlas_offset + 20].tobytes(), np.float32)
+        wmax = np.frombuffer(A[scene.tlas_offset + 20: scene.tlas_offset + 32].tobytes(), np.float32)
+    else:
+        root = scene.tlas_offset + int(np.frombuffer(A[scene.tlas_offset:scene.tlas_offset + 8].tobytes(), np.uint64)[0])
+        wmin, wmax = np.zeros(3, np.float32), np.zeros(3, np.float32)
+        for i in range(6):
+            if int(A[root + 22 + i]) & 3 == 0:
+                continue
+            lo, hi = oracle.set_child_bounds(A[root:root + 64], i)
+            wmin = np.where(lo < wmin, lo, wmin).astype(np.float32)
+            wmax = np.where(hi < wmax, hi, wmax).astype(np.float32)  # the reference's running min
+    ref = coherence.ray_hash(batch, hash, 2, 4, 4, 0.5, wmin, wmax).astype(np.uint64)
+    diff = np.nonzero(hashes.astype(np.uint64) != ref)[0]
+    assert len(diff) <= (len(batch) // 1000 if hash in "gd" else 0), f"{len(diff)} hashes differ"
+    # the schedule of the GPU's own hashes, restated on the CPU
+    assert np.array_equal(order, coherence.packet_order_sorted(hashes, 64))
+    assert np.array_equal(np.sort(order), np.arange(len(batch), dtype=np.uint32))
+    # tracing in that claim order: the same hits at the same indices
+    ob = gpu.to_device(order)
+    rb, hb = gpu.to_device(batch), gpu.buffer(len(batch) * rtx.HIT_DTYPE.itemsize)
+    assert gpu.trace_device(rb, len(batch), hb, claim_order=ob) == 0
+    assert_hits_equal(hb.download(rtx.HIT_DTYPE, len(batch)), gpu.trace(batch), f"coherence order {hash}")
+
+
+def test_claim_order_rejects_rays_outside_the_batch(gpu, rtx, c1):
+    scene, rays = c1
+    gpu.set_scene(scene.arena, scene.tlas_offset)
+    n = 500
+    rb, hb = gpu.to_device(rays[:n]), gpu.buffer(n * rtx.HIT_DTYPE.itemsize)
+    bad = np.arange(n, dtype=np.uint32)
+    bad[17] = n + 3
+    assert gpu.trace_device(rb, n, hb, claim_order=gpu.to_device(bad), lanes_per_ray=1) == 3
+    assert gpu.trace_device(rb, n, hb, claim_order=gpu.to_device(bad), lanes_per_ray=8) == 3
+    good = np.arange(n, dtype=np.uint32)[::-1].copy()
+    assert gpu.trace_device(rb, n, hb, claim_order=gpu.to_device(good)) == 0
+    assert gpu.trace_device(rb, n, hb, claim_order=gpu.to_device(good), claim_start=2) == 1
+
+
+@pytest.mark.parametrize("mode", [0, 1])
+def test_run_statistics_match_the_oracle(gpu, rtx, oracle, instanced, mode):
+    """rtx_trace_stats_of: the reference's end-of-kernel RT counters
+    (gpu-sim.cc:2168-2182) from the hits and records, against the same sums
+    over the oracle's output; without records the per-type counts follow from
+    the per-ray counters."""
+    scene, rays = instanced
+    gpu.set_scene(scene.arena, scene.tlas_offset)
+    gpu.build_treelets(1024)
+    part = oracle.Partition(scene.arena, scene.tlas_offset, 1024)
+    h_ref, o_ref, r_ref = oracle.trace(scene.arena, scene.tlas_offset, rays, mode=mode, partition=part)
+    n, total = len(rays), int(o_ref[-1])
+    rb, hb, ob = gpu.to_device(rays), gpu.buffer(n * rtx.HIT_DTYPE.itemsize), gpu.buffer((n + 1) * 8)
+    recb = gpu.buffer(total * rtx.RECORD_DTYPE.itemsize)
+    assert gpu.trace_device(rb, n, hb, mode=mode, records_buf=recb, record_capacity=total, offsets_buf=ob) == 0
+    s = gpu.trace_stats(hb, n, recb, total)
+    assert s["access_type"] == [int((r_ref["type"] == t).sum()) for t in range(7)]
+    assert s["num_hits"] == int(h_ref["hit"].sum()) and s["num_any_hits"] == int(h_ref["n_any_hits"].sum())
+    assert s["tot_nodes"] == int(h_ref["n_nodes"].sum()) and s["max_nodes_per_ray"] == int(h_ref["n_nodes"].max())
+    assert s["max_tree_depth"] == int(h_ref["max_level"].max()) and s["n_rays"] == n
+    assert s["accessed_bytes"] == int(r_ref["size"].astype(np.int64).sum())
+    assert s["fetch_bytes"] == int(h_ref["fetch_bytes"].astype(np.int64).sum()) and s["from_records"] == 1
+    d = gpu.trace_stats(hb, n)
+    leaves = [int((r_ref["type"] == t).sum()) for t in (4, 5, 6)]
+    assert d["access_type"][:4] == s["access_type"][:4] and d["access_type"][4] == sum(leaves)
+    assert d["from_records"] == 0 and d["accessed_bytes"] == s["accessed_bytes"]

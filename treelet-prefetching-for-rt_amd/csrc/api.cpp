@@ -8,6 +8,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cmath>
 #include <cstdio>
 #include <cstring>
 #include <new>
@@ -22,6 +23,8 @@ using namespace rtxk;
 
 // bytes of the control block cleared before and read after a traversal
 static constexpr uint32_t kSlots = 64;
+// most device memory the per-ray record slabs may take
+static constexpr uint64_t kSlabBytesMax = 32ull << 30;
 
 struct rtx_ctx {
   int device = 0;
@@ -535,6 +538,8 @@ rtx_status rtx_trace(rtx_ctx* c, const rtx_ray* rays, uint32_t n, const rtx_trac
   if (opts->prefetch > RTX_PREFETCH_LATTER) return RTX_ERR_INVALID_ARGUMENT;
   if (opts->emit_records && !offsets) return RTX_ERR_INVALID_ARGUMENT;
   if (n > 0 ? opts->claim_start >= n : opts->claim_start != 0) return RTX_ERR_INVALID_ARGUMENT;
+  if (opts->claim_order && (opts->claim_start != 0 || ((uintptr_t)opts->claim_order & 3)))
+    return RTX_ERR_INVALID_ARGUMENT;
   if (opts->record_slab > 1024 || opts->record_layout > 1) return RTX_ERR_INVALID_ARGUMENT;
   if (!c->arena) return RTX_ERR_NO_SCENE;
   if (opts->record_layout && (!c->part.valid || !c->part.layout_remap)) return RTX_ERR_NO_PARTITION;
@@ -578,10 +583,16 @@ rtx_status rtx_trace(rtx_ctx* c, const rtx_ray* rays, uint32_t n, const rtx_trac
     return RTX_ERR_INVALID_ARGUMENT;
   const bool wide_ok = !opts->emit_records && stage == 0;
   // records in one traversal: per-ray slabs of `slab` records, compacted into
-  // the caller's CSR buffer; only rays with more records are traced again
-  const uint32_t slab = opts->record_slab ? opts->record_slab : 64;
-  const bool use_slab = opts->emit_records && (uint64_t)n * slab * sizeof(rtx_record) <= (8ull << 30) &&
-                        (uint64_t)n * slab < (1ull << 31);
+  // the caller's CSR buffer; only rays with more records are traced again.
+  // The default slab is 256 records (a 1M-triangle diffuse batch has at most
+  // about 210 per ray), halved until the slabs fit 32 GiB; below 16 records a
+  // counting pass and a writing pass replace them.
+  uint32_t slab = opts->record_slab;
+  if (slab == 0) {
+    slab = 256;
+    while (slab > 16 && (uint64_t)n * slab * sizeof(rtx_record) > kSlabBytesMax) slab /= 2;
+  }
+  const bool use_slab = opts->emit_records && (uint64_t)n * slab * sizeof(rtx_record) <= kSlabBytesMax;
   const int main_rec = !opts->emit_records ? kRecNone : use_slab ? kRecSlab : kRecNone;
   if (opts->lanes_per_ray == kWideRayLanes && !wide_ok) return RTX_ERR_INVALID_ARGUMENT;
   const bool wide = wide_ok && (opts->lanes_per_ray == kWideRayLanes ||
@@ -619,7 +630,8 @@ rtx_status rtx_trace(rtx_ctx* c, const rtx_ray* rays, uint32_t n, const rtx_trac
   p.record_capacity = 0;
   p.record_offsets = nullptr;
   p.claim_start = opts->claim_start;
-  p.ray_list = nullptr;
+  p.ray_list = opts->claim_order;
+  p.n_rays = n;
   p.layout_remap = opts->record_layout ? c->part.layout_remap : nullptr;
   p.owner = c->part.valid ? c->part.owner : nullptr;
   p.first_treelet = c->part.valid ? c->part.first_treelet : 0;
@@ -756,6 +768,8 @@ rtx_status rtx_trace_static(rtx_ctx* c, const rtx_ray* rays, uint32_t n, const r
   if (opts->mode != RTX_MODE_PLAIN && opts->mode != RTX_MODE_TREELET) return RTX_ERR_INVALID_ARGUMENT;
   if (opts->prefetch != RTX_PREFETCH_NONE || opts->emit_records) return RTX_ERR_UNSUPPORTED;
   if (n > 0 ? opts->claim_start >= n : opts->claim_start != 0) return RTX_ERR_INVALID_ARGUMENT;
+  if (opts->claim_order && (opts->claim_start != 0 || ((uintptr_t)opts->claim_order & 3)))
+    return RTX_ERR_INVALID_ARGUMENT;
   if (opts->refill > 64) return RTX_ERR_INVALID_ARGUMENT;
   if (!c->arena) return RTX_ERR_NO_SCENE;
   if (opts->mode == RTX_MODE_TREELET && !c->part.valid) return RTX_ERR_NO_PARTITION;
@@ -808,6 +822,8 @@ rtx_status rtx_trace_static(rtx_ctx* c, const rtx_ray* rays, uint32_t n, const r
   p.spill = reinterpret_cast<uint64_t*>(c->static_scratch + kStaticControlBytes);
   p.spill_cap = spill_cap;
   p.claim_start = opts->claim_start;
+  p.ray_list = opts->claim_order;
+  p.n_rays = n;
   p.ray_counter = control;
   p.error = control + 1;
   p.host_err = c->err_host_dev;
@@ -827,6 +843,92 @@ rtx_status rtx_trace_static(rtx_ctx* c, const rtx_ray* rays, uint32_t n, const r
   uint32_t err = 0;
   HIP_TRY(hipMemcpy(&err, control + 1, sizeof(uint32_t), hipMemcpyDeviceToHost));
   return kernel_error_status(err);
+}
+
+rtx_status rtx_coherence_order(rtx_ctx* c, const rtx_ray* rays, uint32_t n, const rtx_coherence_config* cfg,
+                               uint32_t* order, uint32_t* hashes) {
+  if (!c || !cfg || (n > 0 && (!rays || !order))) return RTX_ERR_INVALID_ARGUMENT;
+  const char h = cfg->hash ? cfg->hash : 'd';
+  if (h != 'f' && h != 'g' && h != 't' && h != 'd') return RTX_ERR_INVALID_ARGUMENT;
+  // every hash below 2^32 (and the spherical shifts non-negative)
+  if (h == 'f' && (cfg->francois_bits < 1 || cfg->francois_bits > 4)) return RTX_ERR_INVALID_ARGUMENT;
+  if ((h == 'g' || h == 't') && (cfg->grid_bits < 1 || cfg->grid_bits > 10)) return RTX_ERR_INVALID_ARGUMENT;
+  if ((h == 'g' || h == 'd') && (cfg->sphere_bits < 1 || cfg->sphere_bits > 8)) return RTX_ERR_INVALID_ARGUMENT;
+  if (cfg->world > RTX_COHERENCE_WORLD_GIVEN || ((uintptr_t)rays & 15)) return RTX_ERR_INVALID_ARGUMENT;
+  float wmin[3] = {0.f, 0.f, 0.f}, wmax[3] = {0.f, 0.f, 0.f};
+  if (cfg->world == RTX_COHERENCE_WORLD_GIVEN) {
+    for (int a = 0; a < 3; ++a) {
+      wmin[a] = cfg->world_min[a];
+      wmax[a] = cfg->world_max[a];
+    }
+  } else {
+    if (!c->arena) return RTX_ERR_NO_SCENE;
+    HIP_TRY(hipSetDevice(c->device));
+    uint8_t hdr[64], node[64];
+    HIP_TRY(hipMemcpy(hdr, c->arena + c->tlas, 64, hipMemcpyDeviceToHost));
+    if (cfg->world == RTX_COHERENCE_WORLD_TLAS) {
+      memcpy(wmin, hdr + 8, 12);
+      memcpy(wmax, hdr + 20, 12);
+    } else {
+      // vulkan_ray_tracing.cc:2452-2465: both corners take a running min from
+      // (0,0,0) over the TLAS root's child boxes (set_child_bounds: origin +
+      // ldexpf(q, exponent - 8), exact)
+      uint64_t root_off = 0;
+      memcpy(&root_off, hdr, 8);
+      const uint64_t root = c->tlas + root_off;
+      if ((root & 63) || root + 64 > c->arena_bytes) return RTX_ERR_OFFSET_OUT_OF_ARENA;
+      HIP_TRY(hipMemcpy(node, c->arena + root, 64, hipMemcpyDeviceToHost));
+      float origin[3];
+      memcpy(origin, node, 12);
+      for (int i = 0; i < 6; ++i) {
+        if ((node[22 + i] & 3u) == 0) continue;
+        for (int a = 0; a < 3; ++a) {
+          const int e = (int)(int8_t)node[18 + a];
+          const float lo = origin[a] + std::ldexp((float)node[28 + 12 * a + i], e - 8);
+          const float hi = origin[a] + std::ldexp((float)node[34 + 12 * a + i], e - 8);
+          wmin[a] = lo < wmin[a] ? lo : wmin[a];  // std::min(world, lo)
+          wmax[a] = hi < wmax[a] ? hi : wmax[a];  // std::min(world, hi), as the reference writes it
+        }
+      }
+    }
+  }
+  HIP_TRY(hipSetDevice(c->device));
+  rtx_coherence_config k = *cfg;
+  k.hash = h;
+  if (n == 0) return RTX_OK;
+  const hipError_t e = coherence_order(rays, n, k, wmin, wmax, order, hashes, c->stream);
+  if (e == hipErrorOutOfMemory) return RTX_ERR_OUT_OF_MEMORY;
+  return e == hipSuccess ? RTX_OK : RTX_ERR_HIP;
+}
+
+rtx_status rtx_trace_stats_of(rtx_ctx* c, const rtx_hit* hits, uint32_t n, const rtx_record* records,
+                              uint64_t n_records, rtx_trace_stats* out) {
+  if (!c || !out || (n > 0 && !hits) || (n_records > 0 && !records)) return RTX_ERR_INVALID_ARGUMENT;
+  HIP_TRY(hipSetDevice(c->device));
+  unsigned long long* acc = nullptr;
+  if (hipMalloc(&acc, 16 * sizeof(unsigned long long)) != hipSuccess) {
+    (void)hipGetLastError();
+    return RTX_ERR_OUT_OF_MEMORY;
+  }
+  unsigned long long h[16];
+  hipError_t e = hipMemsetAsync(acc, 0, sizeof(h), c->stream);
+  if (e == hipSuccess) e = launch_trace_stats(hits, n, n_records ? records : nullptr, n_records, acc, c->stream);
+  if (e == hipSuccess) e = hipMemcpyAsync(h, acc, sizeof(h), hipMemcpyDeviceToHost, c->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+  hipFree(acc);
+  if (e != hipSuccess) return RTX_ERR_HIP;
+  memset(out, 0, sizeof(*out));
+  for (int k = 0; k < 7; ++k) out->access_type[k] = h[k];
+  out->num_hits = h[7];
+  out->num_any_hits = h[8];
+  out->n_rays = n;
+  out->tot_nodes = h[9];
+  out->max_nodes_per_ray = (uint32_t)h[10];
+  out->max_tree_depth = (uint32_t)h[11];
+  out->fetch_bytes = h[12];
+  out->accessed_bytes = h[13];
+  out->from_records = n_records ? 1u : 0u;
+  return RTX_OK;
 }
 
 rtx_status rtx_set_prefetch_table(rtx_ctx* c, const uint32_t* table, uint64_t n_rows) {

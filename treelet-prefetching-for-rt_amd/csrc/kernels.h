@@ -60,6 +60,7 @@ struct TraceParams {
   const uint64_t* layout_remap;  // nullable: records carry these layout offsets (arena slot -> offset)
   uint32_t claim_start;    // rays are claimed from this index on, wrapping to 0
   const uint32_t* ray_list;  // nullable: claim k traces ray ray_list[k] (n = list length)
+  uint32_t n_rays;           // rays in the ray / hit arrays (ray_list entries must lie below)
   const uint32_t* owner;  // node slot -> dense treelet index (nullable in plain mode)
   uint32_t first_treelet; // dense index of the treelet rooted at the TLAS header
   uint64_t* spill;        // global stack spill, spill_cap entries per stack per lane
@@ -125,6 +126,14 @@ hipError_t launch_scan_records(const rtx_hit* hits, uint32_t n, uint64_t* offset
 // Copy each ray's slab records (at most `slab`) to its CSR position; rays with
 // more than `slab` records are appended to overflow_list (count in
 // overflow_count, which must start at 0).
+// Statistics of a traced batch: acc[0..6] records per type, [7] hits, [8] any
+// hits, [9] nodes, [10] max nodes, [11] max level, [12] fetch bytes, [13]
+// accessed bytes (acc must start at 0; records may be null).
+hipError_t launch_trace_stats(const rtx_hit* hits, uint32_t n, const rtx_record* records, uint64_t n_records,
+                              unsigned long long* acc, hipStream_t s);
+// The coherence engine's claim order of n rays (coherence.hip).
+hipError_t coherence_order(const rtx_ray* rays, uint32_t n, const rtx_coherence_config& cfg, const float wmin[3],
+                           const float wmax[3], uint32_t* order, uint32_t* hashes_out, hipStream_t s);
 hipError_t launch_compact_records(const rtx_hit* hits, uint32_t n, const rtx_record* slabs, uint32_t slab,
                                   const uint64_t* offsets, rtx_record* records, uint32_t* overflow_list,
                                   uint32_t* overflow_count, hipStream_t s);

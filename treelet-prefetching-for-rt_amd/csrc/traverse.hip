@@ -571,8 +571,12 @@ void trace_kernel(TraceParams p) {
       if ((uint64_t)base + nidle >= p.n) exhausted = true;
       if (!active) {
         const uint32_t k = base + lanes_below(idle);
-        if (k < p.n) {
-          const uint32_t r = MULTI ? k : claimed_ray(p, k);
+        const uint32_t r = k < p.n ? (MULTI ? k : claimed_ray(p, k)) : 0u;
+        if (k < p.n && !MULTI && r >= p.n_rays) {
+          // a listed ray outside the batch: reported, never read
+          atomicOr(p.error, kErrRange);
+          if (p.host_err) *reinterpret_cast<volatile uint32_t*>(p.host_err) = 1u;
+        } else if (k < p.n) {
           active = true;
           local_id = r;
           if constexpr (MULTI) {
@@ -1289,8 +1293,13 @@ __global__ __launch_bounds__(kWideBlock) void trace_wide(TraceParams p, uint32_t
       if ((uint64_t)base + nidle >= p.n) exhausted = true;
       if (!active) {
         const uint32_t k = base + (uint32_t)__popcll(idle & ((1ull << gbase) - 1ull));
-        if (k < p.n) {
-          const uint32_t r = claimed_ray(p, k);
+        const uint32_t r = k < p.n ? claimed_ray(p, k) : 0u;
+        if (k < p.n && r >= p.n_rays) {
+          if (j == 0) {
+            atomicOr(p.error, kErrRange);
+            if (p.host_err) *reinterpret_cast<volatile uint32_t*>(p.host_err) = 1u;
+          }
+        } else if (k < p.n) {
           active = true;
           local_id = r;
           const uint4* rq = reinterpret_cast<const uint4*>(p.rays + r);
@@ -1567,10 +1576,13 @@ TraceFn pick_trace(uint32_t mode, int records, uint32_t lds_stack, uint32_t stag
 }
 
 // ---- slab records -> CSR ----------------------------------------------------
-// A wave takes 64 rays at a time: lane i loads ray i's record count and CSR
-// offset, then the wave copies the rays' slab records one ray after another,
-// lane k moving record k (16-byte loads and stores, coalesced).  A ray with
-// more records than its slab is listed for the second pass instead.
+// A wave takes 64 consecutive rays, whose records fill one contiguous CSR
+// range: lane i holds ray i's record count and the inclusive prefix of the
+// counts, and output position f of the range (f = lane, lane + 64, ...)
+// belongs to the ray whose prefix first exceeds f (a binary search over the
+// wave's prefixes with shuffles).  Loads and stores are 16 bytes and the
+// stores are contiguous.  Records past a ray's slab are left to the second
+// pass, which lists the ray.
 __global__ __launch_bounds__(256) void compact_records(const rtx_hit* hits, uint32_t n, const uint4* slabs,
                                                        uint32_t slab, const uint64_t* offsets, uint4* records,
                                                        uint32_t* overflow_list, uint32_t* overflow_count) {
@@ -1580,18 +1592,31 @@ __global__ __launch_bounds__(256) void compact_records(const rtx_hit* hits, uint
   for (uint32_t base = wave * 64; base < n; base += waves * 64) {
     const uint32_t r = base + lane;
     uint32_t cnt = 0;
-    uint64_t off = 0;
     if (r < n) {
       cnt = hits[r].n_records;
-      off = offsets[r];
       if (cnt > slab) overflow_list[atomicAdd(overflow_count, 1u)] = r;
     }
-    const uint32_t m = min(64u, n - base);
-    for (uint32_t i = 0; i < m; ++i) {
-      const uint32_t c = min(__shfl(cnt, (int)i, 64), slab);
-      const uint64_t o = __shfl(off, (int)i, 64);
-      const uint4* src = slabs + (uint64_t)(base + i) * slab;
-      for (uint32_t k = lane; k < c; k += 64) records[o + k] = src[k];
+    uint32_t incl = cnt;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+      const uint32_t y = __shfl_up(incl, d, 64);
+      if (lane >= (uint32_t)d) incl += y;
+    }
+    const uint32_t total = __shfl(incl, 63, 64);
+    const uint64_t out = offsets[base];
+    // wave-uniform loop: every lane takes part in the shuffles
+    for (uint32_t f0 = 0; f0 < total; f0 += 64) {
+      const uint32_t f = f0 + lane;
+      // first ray i with prefix(i) > f
+      uint32_t lo = 0;
+#pragma unroll
+      for (uint32_t step = 32; step > 0; step >>= 1) {
+        const uint32_t pre = __shfl(incl, (int)(lo + step - 1), 64);
+        if (pre <= f) lo += step;
+      }
+      const uint32_t before = __shfl(incl, (int)(lo ? lo - 1 : 0), 64);
+      const uint32_t k = f - (lo ? before : 0u);
+      if (f < total && k < slab) records[out + f] = slabs[(uint64_t)(base + lo) * slab + k];
     }
   }
 }
@@ -1643,6 +1668,76 @@ hipError_t launch_fill_zero(uint32_t* dst, uint32_t words, hipStream_t s) {
 }
 
 uint32_t trace_block_size(uint32_t stage_bytes) { return stage_bytes ? 64u : 256u; }
+
+namespace {
+// per-ray counters -> run totals; with no records the per-type counts follow
+// from the counters: a internal nodes, b instance leaves, c primitive leaves
+// (fetch = 64 + 64a + 192b + 64c, nodes = a + b + c, records = 1 + a + 2b + 2c)
+__global__ __launch_bounds__(256) void k_stats_hits(const rtx_hit* hits, uint32_t n, bool derive_types,
+                                                    unsigned long long* acc) {
+  unsigned long long t[7] = {0, 0, 0, 0, 0, 0, 0}, nh = 0, na = 0, nodes = 0, fb = 0, ab = 0;
+  uint32_t mx_nodes = 0, mx_level = 0;
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    const rtx_hit& h = hits[i];
+    nh += h.hit;
+    na += h.n_any_hits;
+    nodes += h.n_nodes;
+    fb += h.fetch_bytes;
+    ab += h.accessed_bytes;
+    mx_nodes = max(mx_nodes, h.n_nodes);
+    mx_level = max(mx_level, h.max_level);
+    if (derive_types && h.n_records) {
+      const uint32_t b = ((h.fetch_bytes >> 6) - 1u - h.n_nodes) >> 1;
+      const uint32_t c = h.n_records - 1u - h.n_nodes - b;
+      const uint32_t a = h.n_nodes - b - c;
+      t[RTX_TX_BVH_STRUCTURE] += 1u + b;
+      t[RTX_TX_BVH_INTERNAL_NODE] += a;
+      t[RTX_TX_BVH_INSTANCE_LEAF] += b;
+      t[RTX_TX_BVH_PRIMITIVE_LEAF_DESCRIPTOR] += c;
+      t[RTX_TX_BVH_QUAD_LEAF] += c;
+    }
+  }
+  const unsigned long long v[14] = {t[0], t[1], t[2], t[3], t[4], t[5], t[6], nh, na, nodes, 0, 0, fb, ab};
+  for (int k = 0; k < 14; ++k) {
+    if (k == 10 || k == 11) continue;
+    unsigned long long x = v[k];
+    for (int o = 32; o > 0; o >>= 1) x += __shfl_xor(x, o, 64);
+    if ((threadIdx.x & 63u) == 0 && x) atomicAdd(&acc[k], x);
+  }
+  for (int o = 32; o > 0; o >>= 1) {
+    mx_nodes = max(mx_nodes, (uint32_t)__shfl_xor(mx_nodes, o, 64));
+    mx_level = max(mx_level, (uint32_t)__shfl_xor(mx_level, o, 64));
+  }
+  if ((threadIdx.x & 63u) == 0) {
+    atomicMax(&acc[10], (unsigned long long)mx_nodes);
+    atomicMax(&acc[11], (unsigned long long)mx_level);
+  }
+}
+
+__global__ __launch_bounds__(256) void k_stats_records(const rtx_record* rec, uint64_t n, unsigned long long* acc) {
+  unsigned long long t[7] = {0, 0, 0, 0, 0, 0, 0};
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x) {
+    const uint32_t ty = rec[i].type;
+    if (ty < 7) t[ty]++;
+  }
+  for (int k = 0; k < 7; ++k) {
+    unsigned long long x = t[k];
+    for (int o = 32; o > 0; o >>= 1) x += __shfl_xor(x, o, 64);
+    if ((threadIdx.x & 63u) == 0 && x) atomicAdd(&acc[k], x);
+  }
+}
+}  // namespace
+
+hipError_t launch_trace_stats(const rtx_hit* hits, uint32_t n, const rtx_record* records, uint64_t n_records,
+                              unsigned long long* acc, hipStream_t s) {
+  const uint32_t g = std::max(1u, std::min<uint32_t>(2048, (n + 255) / 256));
+  if (n) hipLaunchKernelGGL(k_stats_hits, dim3(g), dim3(256), 0, s, hits, n, records == nullptr, acc);
+  if (records && n_records) {
+    const uint32_t gr = (uint32_t)std::min<uint64_t>(4096, (n_records + 255) / 256);
+    hipLaunchKernelGGL(k_stats_records, dim3(gr), dim3(256), 0, s, records, n_records, acc);
+  }
+  return hipGetLastError();
+}
 
 hipError_t launch_compact_records(const rtx_hit* hits, uint32_t n, const rtx_record* slabs, uint32_t slab,
                                   const uint64_t* offsets, rtx_record* records, uint32_t* overflow_list,

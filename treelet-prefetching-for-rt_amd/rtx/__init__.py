@@ -65,7 +65,25 @@ class _TraceOptions(ctypes.Structure):
                 ("lds_stack", ctypes.c_uint32), ("stage_bytes", ctypes.c_uint32), ("refill", ctypes.c_uint32),
                 ("slot_bytes", ctypes.c_uint32), ("min_score", ctypes.c_float), ("pred_fraction", ctypes.c_float),
                 ("lanes_per_ray", ctypes.c_uint32), ("claim_start", ctypes.c_uint32),
-                ("record_layout", ctypes.c_uint32), ("record_slab", ctypes.c_uint32)]
+                ("record_layout", ctypes.c_uint32), ("record_slab", ctypes.c_uint32),
+                ("claim_order", ctypes.c_void_p)]
+
+
+class _TraceStats(ctypes.Structure):
+    _fields_ = [("access_type", ctypes.c_uint64 * 7), ("num_hits", ctypes.c_uint64), ("num_any_hits", ctypes.c_uint64),
+                ("n_rays", ctypes.c_uint64), ("tot_nodes", ctypes.c_uint64), ("max_nodes_per_ray", ctypes.c_uint32),
+                ("max_tree_depth", ctypes.c_uint32), ("fetch_bytes", ctypes.c_uint64),
+                ("accessed_bytes", ctypes.c_uint64), ("from_records", ctypes.c_uint32), ("reserved", ctypes.c_uint32)]
+
+
+class _CoherenceConfig(ctypes.Structure):
+    _fields_ = [("hash", ctypes.c_char), ("world", ctypes.c_uint8), ("reserved", ctypes.c_uint16),
+                ("francois_bits", ctypes.c_uint32), ("grid_bits", ctypes.c_uint32), ("sphere_bits", ctypes.c_uint32),
+                ("two_point_ratio", ctypes.c_float), ("packet", ctypes.c_uint32),
+                ("world_min", ctypes.c_float * 3), ("world_max", ctypes.c_float * 3)]
+
+
+COHERENCE_WORLD_REFERENCE, COHERENCE_WORLD_TLAS, COHERENCE_WORLD_GIVEN = 0, 1, 2
 
 
 class _LayoutSummary(ctypes.Structure):
@@ -135,6 +153,8 @@ _sig("rtx_set_treelets", _i32, _vp, _u32, _u32, _vp, _vp, _vp, _vp)
 _sig("rtx_trace", _i32, _vp, _vp, _u32, _P(_TraceOptions), _vp, _vp, _u64, _vp)
 _sig("rtx_trace_static", _i32, _vp, _vp, _u32, _P(_TraceOptions), _vp)
 _sig("rtx_build_treelet_layout", _i32, _vp, _u32, _P(_LayoutSummary))
+_sig("rtx_coherence_order", _i32, _vp, _vp, _u32, _P(_CoherenceConfig), _vp, _vp)
+_sig("rtx_trace_stats_of", _i32, _vp, _vp, _u32, _vp, _u64, _P(_TraceStats))
 _sig("rtx_get_treelet_layout", _i32, _vp, _vp, _vp)
 _sig("rtx_adapter_create", _i32, _i32, _vp, _u64, _u64, _u32, _P(_vp))
 _sig("rtx_adapter_destroy", _i32, _vp)
@@ -188,7 +208,7 @@ EXPORTED_SYMBOLS = [
     "rtx_status_string", "rtx_version", "rtx_device_count", "rtx_create", "rtx_destroy", "rtx_synchronize",
     "rtx_malloc", "rtx_free", "rtx_memcpy_h2d", "rtx_memcpy_d2h", "rtx_memcpy_d2d", "rtx_memset", "rtx_timer_start",
     "rtx_timer_stop", "rtx_device_info", "rtx_set_scene", "rtx_set_scene_device", "rtx_validate_scene",
-    "rtx_build_treelets", "rtx_get_treelets", "rtx_set_treelets", "rtx_build_treelet_layout", "rtx_get_treelet_layout", "rtx_trace", "rtx_trace_static", "rtx_get_prefetch_stats",
+    "rtx_build_treelets", "rtx_get_treelets", "rtx_set_treelets", "rtx_build_treelet_layout", "rtx_get_treelet_layout", "rtx_trace", "rtx_trace_static", "rtx_coherence_order", "rtx_trace_stats_of", "rtx_get_prefetch_stats",
     "rtx_last_kernel_ms", "rtx_add_scene", "rtx_build_scene_treelets", "rtx_trace_batch", "rtx_batch_graph_create",
     "rtx_batch_graph_launch", "rtx_batch_graph_destroy", "rtx_autotune_batch", "rtx_autotune_batch_treelets",
     "rtx_set_prefetch_table", "rtx_mlp_create", "rtx_mlp_destroy", "rtx_mlp_param_count", "rtx_mlp_get_params",
@@ -618,11 +638,12 @@ class Context:
     def trace_device(self, rays_buf, n, hits_buf, mode=MODE_PLAIN, prefetch=PREFETCH_NONE, threshold=0.5,
                      records_buf=None, record_capacity=0, offsets_buf=None, waves_per_cu=0, lds_stack=0,
                      stage_bytes=0, refill=0, slot_bytes=0, min_score=0.0, pred_fraction=0.0, lanes_per_ray=0,
-                     claim_start=0, record_slab=0, record_layout=0):
-        """rtx_trace on device buffers; returns the rtx_status (0 = ok)."""
+                     claim_start=0, record_slab=0, record_layout=0, claim_order=None):
+        """rtx_trace on device buffers; returns the rtx_status (0 = ok).  claim_order: a
+        DeviceBuffer of uint32 ray indices (rtx_coherence_order) or None."""
         o = _TraceOptions(mode, prefetch, threshold, 1 if offsets_buf is not None else 0, waves_per_cu, lds_stack,
                           stage_bytes, refill, slot_bytes, min_score, pred_fraction, lanes_per_ray, claim_start,
-                          record_layout, record_slab)
+                          record_layout, record_slab, claim_order.ptr.value if claim_order is not None else None)
         return _lib.rtx_trace(self._h, rays_buf.ptr, n, ctypes.byref(o), hits_buf.ptr,
                               records_buf.ptr if records_buf is not None else None, record_capacity,
                               offsets_buf.ptr if offsets_buf is not None else None)
@@ -631,8 +652,34 @@ class Context:
                             lanes_per_ray=0, claim_start=0):
         """rtx_trace_static (hits only, no allocation) on device buffers; returns the rtx_status."""
         o = _TraceOptions(mode, PREFETCH_NONE, 0.0, 0, waves_per_cu, lds_stack, 0, refill, 0, 0.0, 0.0, lanes_per_ray,
-                          claim_start, 0, 0)
+                          claim_start, 0, 0, None)
         return _lib.rtx_trace_static(self._h, rays_buf.ptr, n, ctypes.byref(o), hits_buf.ptr)
+
+    def trace_stats(self, hits_buf, n, records_buf=None, n_records=0):
+        """rtx_trace_stats_of on device buffers -> dict (access_type as a list)."""
+        s = _TraceStats()
+        _check(_lib.rtx_trace_stats_of(self._h, hits_buf.ptr, n, records_buf.ptr if records_buf is not None else None,
+                                       n_records, ctypes.byref(s)), "rtx_trace_stats_of")
+        out = {f: getattr(s, f) for f, _ in _TraceStats._fields_}
+        out["access_type"] = list(s.access_type)
+        return out
+
+    def coherence_order_device(self, rays_buf, n, order_buf, hash="d", world=COHERENCE_WORLD_REFERENCE,
+                               francois_bits=2, grid_bits=4, sphere_bits=4, two_point_ratio=0.5, packet=0,
+                               world_min=(0.0, 0.0, 0.0), world_max=(1.0, 1.0, 1.0), hashes_buf=None):
+        """rtx_coherence_order on device buffers; returns the rtx_status."""
+        c = _CoherenceConfig(hash.encode(), world, 0, francois_bits, grid_bits, sphere_bits, two_point_ratio, packet,
+                             (ctypes.c_float * 3)(*world_min), (ctypes.c_float * 3)(*world_max))
+        return _lib.rtx_coherence_order(self._h, rays_buf.ptr, n, ctypes.byref(c), order_buf.ptr,
+                                        hashes_buf.ptr if hashes_buf is not None else None)
+
+    def coherence_order(self, rays, **cfg):
+        """Host rays in; (order, hashes) out."""
+        rays = np.ascontiguousarray(rays, dtype=RAY_DTYPE)
+        n = len(rays)
+        rb, ob, hb = self.to_device(rays), self.buffer(max(n, 1) * 4), self.buffer(max(n, 1) * 4)
+        _check(self.coherence_order_device(rb, n, ob, hashes_buf=hb, **cfg), "rtx_coherence_order")
+        return ob.download(np.uint32, n), hb.download(np.uint32, n)
 
     def trace(self, rays, mode=MODE_PLAIN, prefetch=PREFETCH_NONE, threshold=0.5, records=False,
               records_per_ray=48, **kw):
@@ -710,7 +757,7 @@ class Context:
 
     @staticmethod
     def _opts(mode=MODE_PLAIN, waves_per_cu=0, lds_stack=0, refill=0):
-        return _TraceOptions(mode, PREFETCH_NONE, 0.0, 0, waves_per_cu, lds_stack, 0, refill, 0, 0.0, 0.0, 0, 0, 0, 0)
+        return _TraceOptions(mode, PREFETCH_NONE, 0.0, 0, waves_per_cu, lds_stack, 0, refill, 0, 0.0, 0.0, 0, 0, 0, 0, None)
 
     def trace_batch(self, jobs, **opts):
         arr = self._jobs(jobs)

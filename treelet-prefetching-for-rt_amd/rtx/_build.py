@@ -21,7 +21,7 @@ BUILD = os.path.join(ROOT, "_build")
 LIB = os.path.join(PKG, "librtx.so")
 
 SOURCES = ["api.cpp", "adapter.cpp", "builder.cpp", "scene.cpp", "formats.cpp", "traverse.hip", "partition.hip",
-           "mlp.hip", "gbuild.hip", "static_scratch.hip"]
+           "mlp.hip", "gbuild.hip", "static_scratch.hip", "coherence.hip"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("RTX_OFFLOAD_ARCH", "gfx950")
