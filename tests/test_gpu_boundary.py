@@ -76,21 +76,22 @@ def test_slab_records_capacity_and_staging(gpu, rtx, oracle, c1):
 
 
 def test_two_pass_records_when_slabs_exceed_the_limit(gpu, rtx, oracle, c1):
-    """786,432 rays with 1,024-record slabs would need 48 GiB of slabs: rtx_trace
+    """2,359,296 rays with 1,024-record slabs would need 36 GiB of slabs: rtx_trace
     falls back to a counting pass and a writing pass, with the same records."""
     rtx_slab_limit = 32 << 30
-    assert 786432 * 1024 * 16 > rtx_slab_limit
+    assert 2048 * 1152 * 1024 * 16 > rtx_slab_limit
     scene, _ = c1
     from conftest import c1_desc
 
-    d = c1_desc(rtx, 1024, 768)
+    d = c1_desc(rtx, 2048, 1152)
     rays = rtx.primary_rays(d)
+    assert len(rays) == 2048 * 1152
     gpu.set_scene(scene.arena, scene.tlas_offset)
     h2, o2, r2 = gpu.trace(rays, mode=0, records=True, record_slab=1024)
     h1, o1, r1 = gpu.trace(rays, mode=0, records=True)
     assert_hits_equal(h2, h1, "two-pass vs slab")
     assert_records_equal(o2, r2, o1, r1, "two-pass vs slab")
-    sample = np.arange(0, len(rays), 97)
+    sample = np.arange(0, len(rays), 293)
     h_ref, o_ref, r_ref = oracle.trace(scene.arena, scene.tlas_offset, rays[sample], mode=0)
     assert_hits_equal(h1[sample], h_ref, "slab vs oracle sample")
     got = np.concatenate([r1[o1[i]:o1[i + 1]] for i in sample])
