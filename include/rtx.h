@@ -48,9 +48,11 @@ typedef enum rtx_status {
   RTX_ERR_NO_PARTITION = 10,
   RTX_ERR_UNSUPPORTED = 11,
   RTX_ERR_OUT_OF_MEMORY = 12,
-  RTX_ERR_TIMEOUT = 13,
-  RTX_ERR_STALE = 14                   /* a batch graph whose scenes' partitions were rebuilt, or whose
+  RTX_ERR_TIMEOUT = 13,                /* a ray passed the per-ray limit of 2^24 steps: its traversal
+                                          would not end (a cyclic BVH, where the reference loops) */
+  RTX_ERR_STALE = 14,                  /* a batch graph whose scenes' partitions were rebuilt, or whose
                                           context was destroyed, after it was captured */
+  RTX_ERR_CYCLE = 15                   /* rtx_validate_scene: a node the TLAS reaches reaches itself */
 } rtx_status;
 
 /* Traversal order. PLAIN follows traceRay, TREELET follows traceRayWithTreelets. */
@@ -123,7 +125,12 @@ typedef struct rtx_hit {
   uint32_t n_nodes;         /* total_nodes_accessed */
   uint32_t fetch_bytes;     /* sum of record sizes, leaf-descriptor records excluded */
   uint32_t n_any_hits;      /* accepted hits of a non-opaque ray (any-hit shader calls) */
-  uint32_t max_level;       /* deepest tree level assigned (tree_level_map) */
+  uint32_t max_level;       /* the ray's level: the maximum over its final tree_level_map
+                               (vulkan_ray_tracing.cc:2274-2279, 3062-3067), where a node
+                               assigned again keeps its last value; equal to the deepest level
+                               assigned unless a node is reachable at two levels (a BLAS under
+                               instance leaves of different depths), for which rtx_trace runs
+                               a second traversal that keeps the map */
   uint64_t instance_leaf;   /* arena offset of closest_instanceLeaf (vulkan_ray_tracing.cc:2856), the
                                instance of the last accepted hit; Hit_data's worldToObjectMatrix and
                                objectToWorldMatrix (:3002-3003) are its rows at +16 and +80.
@@ -139,8 +146,10 @@ typedef struct rtx_record {
   uint32_t treelet; /* dense treelet index of the node's owner, or RTX_NO_TREELET */
   uint16_t size;    /* bytes: 64, 128 or 8 */
   uint8_t type;     /* RTX_TX_* */
-  uint8_t level;    /* tree level of the node (tree_level_map, vulkan_ray_tracing.cc:2469,2569,2659):
-                       0 for the TLAS header, saturating at 255 */
+  uint8_t level;    /* depth of the node along the path that reached it (the tree_level_map value
+                       assigned with it, vulkan_ray_tracing.cc:2470,2569,2668): TLAS root 1, a child
+                       its parent's + 1, a BLAS root and its header record their instance leaf's;
+                       0 for the TLAS header; saturating at 255 */
 } rtx_record; /* 16 bytes */
 
 typedef struct rtx_trace_options {

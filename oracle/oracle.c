@@ -685,6 +685,85 @@ int orc_treelet_layout(const uint8_t* A, uint64_t bytes, uint64_t tlas, const or
 /* ------------------------------------------------------------------------ */
 enum { TX_STRUCT = 0, TX_INTERNAL = 1, TX_INSTANCE = 2, TX_DESC = 3, TX_QUAD = 4, TX_QUAD_HIT = 5, TX_PROC = 6 };
 
+/* ---- tree_level_map (vulkan_ray_tracing.cc:2424, 1662) --------------------
+ * A per-ray map from node address (arena offset here) to tree level.  The
+ * reference assigns a child `map[node] + 1` with the node's value when the
+ * node is processed, a BLAS root its instance leaf's value, and takes the ray's
+ * level as the maximum over the final map (:2274-2279, :3062-3067): a node that
+ * a later assignment overwrites keeps only its last value.  Open addressing
+ * with the keys of the current ray listed, so clearing and the final maximum
+ * cost one step per key. */
+typedef struct {
+  uint64_t* key; /* offset + 1, 0 = empty */
+  uint32_t* val;
+  uint32_t* used; /* table positions written by the current ray */
+  size_t cap, n;
+} lmap;
+
+static size_t lm_pos(const lmap* m, uint64_t k) {
+  size_t h = (size_t)((k * 0x9E3779B97F4A7C15ull) >> 20) & (m->cap - 1);
+  while (m->key[h] != 0 && m->key[h] != k + 1) h = (h + 1) & (m->cap - 1);
+  return h;
+}
+static int lm_grow(lmap* m) {
+  size_t nc = m->cap ? 2 * m->cap : 1024;
+  uint64_t* nk = (uint64_t*)calloc(nc, sizeof(uint64_t));
+  uint32_t* nv = (uint32_t*)malloc(nc * sizeof(uint32_t));
+  uint32_t* nu = (uint32_t*)malloc(nc / 2 * sizeof(uint32_t));
+  if (!nk || !nv || !nu) {
+    free(nk);
+    free(nv);
+    free(nu);
+    return 0;
+  }
+  lmap o = *m;
+  m->key = nk;
+  m->val = nv;
+  m->used = nu;
+  m->cap = nc;
+  m->n = 0;
+  for (size_t i = 0; i < o.n; i++) {
+    size_t h = lm_pos(m, o.key[o.used[i]] - 1);
+    m->key[h] = o.key[o.used[i]];
+    m->val[h] = o.val[o.used[i]];
+    m->used[m->n++] = (uint32_t)h;
+  }
+  free(o.key);
+  free(o.val);
+  free(o.used);
+  return 1;
+}
+static int lm_set(lmap* m, uint64_t k, uint32_t v) {
+  if (2 * (m->n + 1) > m->cap && !lm_grow(m)) return 0;
+  size_t h = lm_pos(m, k);
+  if (m->key[h] == 0) {
+    m->key[h] = k + 1;
+    m->used[m->n++] = (uint32_t)h;
+  }
+  m->val[h] = v;
+  return 1;
+}
+static uint32_t lm_get(const lmap* m, uint64_t k) { /* the reference asserts the key is present */
+  if (m->cap == 0) return 0;
+  size_t h = lm_pos(m, k);
+  return m->key[h] ? m->val[h] : 0;
+}
+static uint32_t lm_max_and_clear(lmap* m) {
+  uint32_t mx = 0;
+  for (size_t i = 0; i < m->n; i++) {
+    if (m->val[m->used[i]] > mx) mx = m->val[m->used[i]];
+    m->key[m->used[i]] = 0;
+  }
+  m->n = 0;
+  return mx;
+}
+static void lm_free(lmap* m) {
+  free(m->key);
+  free(m->val);
+  free(m->used);
+  memset(m, 0, sizeof(*m));
+}
+
 typedef struct {
   const uint8_t* A;
   uint64_t bytes;
@@ -696,6 +775,7 @@ typedef struct {
   uint32_t fetch_bytes;
   uint32_t accessed_bytes; /* accessedDataSize += transaction.size (vulkan_ray_tracing.cc:2260) */
   int overflow;
+  lmap* lm; /* the ray's tree_level_map */
 } sink_t;
 
 static void emit(sink_t* s, uint64_t off, uint32_t size, uint32_t type, uint32_t level) {
@@ -744,8 +824,9 @@ static int ss_push(sstack* s, sent e) {
 
 typedef struct {
   int err;
-  uint32_t max_level;
+  uint32_t max_level; /* deepest path level assigned (checked against the map's maximum in tests) */
 } tstate;
+
 
 #define CHECK_RANGE(off, len)                  \
   do {                                         \
@@ -809,7 +890,8 @@ static int trace_plain(sink_t* s, const uint64_t tlas, const orc_ray* in, orc_hi
   hdr_t top = unpack_hdr(A + tlas);
   emit(s, tlas, 64, TX_STRUCT, 0); /* :2444-2449 */
   uint64_t topRoot = tlas + top.root_offset;
-  level_set(ts, 1); /* tree_level_map[topRootAddr] = 1 (:2469) */
+  level_set(ts, 1);
+  if (!lm_set(s->lm, topRoot, 1)) goto oom; /* tree_level_map[topRootAddr] = 1 (:2470) */
   {
     float thit;
     if (ray_box_test(top.bmin, top.bmax, calculate_idir(ray.d), ray.o, ray.tmin, ray.tmax, &thit))
@@ -858,6 +940,7 @@ static int trace_plain(sink_t* s, const uint64_t tlas, const orc_ray* in, orc_hi
           child_hit[i] = 0;
       }
       uint64_t child = node_addr + (int64_t)node.child_offset * 64;
+      const uint32_t node_map = lm_get(s->lm, node_addr);
       for (int i = 0; i < 6; i++) {
         if (child_hit[i]) {
           if (node.type[i] != 0) {
@@ -874,6 +957,7 @@ static int trace_plain(sink_t* s, const uint64_t tlas, const orc_ray* in, orc_hi
             if (!ss_push(&st, (sent){child, 1, 0, node_level + 1})) goto oom;
           }
           level_set(ts, node_level + 1);
+          if (!lm_set(s->lm, child, node_map + 1)) goto oom; /* :2569, 2576, 2581 */
         }
         child += node.size[i] * 64;
       }
@@ -896,6 +980,7 @@ static int trace_plain(sink_t* s, const uint64_t tlas, const orc_ray* in, orc_hi
       uint64_t botRoot = blas + bh.root_offset;
       if (!ss_push(&st, (sent){botRoot, 0, 0, le.level})) goto oom;
       level_set(ts, le.level);
+      if (!lm_set(s->lm, botRoot, lm_get(s->lm, leaf_addr))) goto oom; /* :2668 */
 
       while (st.n > 0 && !st.e[st.n - 1].top) { /* bottom level, :2679-2986 */
         sent be = st.e[--st.n];
@@ -923,6 +1008,7 @@ static int trace_plain(sink_t* s, const uint64_t tlas, const orc_ray* in, orc_hi
               child_hit[i] = 0;
           }
           uint64_t child = node_addr + (int64_t)node.child_offset * 64;
+          const uint32_t node_map = lm_get(s->lm, node_addr);
           for (int i = 0; i < 6; i++) {
             if (child_hit[i]) {
               if (node.type[i] != 0) {
@@ -935,6 +1021,7 @@ static int trace_plain(sink_t* s, const uint64_t tlas, const orc_ray* in, orc_hi
                 if (!ss_push(&st, (sent){child, 0, 0, node_level + 1})) goto oom;
               }
               level_set(ts, node_level + 1);
+              if (!lm_set(s->lm, child, node_map + 1)) goto oom; /* :2756, 2763, 2767 */
             }
             child += node.size[i] * 64;
           }
@@ -984,7 +1071,8 @@ static int trace_plain(sink_t* s, const uint64_t tlas, const orc_ray* in, orc_hi
   h->fetch_bytes = s->fetch_bytes;
   h->accessed_bytes = s->accessed_bytes;
   h->n_any_hits = n_any;
-  h->max_level = ts->max_level;
+  h->max_level = lm_max_and_clear(s->lm);
+  h->path_level = ts->max_level;
   free(st.e);
   return ORC_OK;
 oom:
@@ -1074,6 +1162,7 @@ static int trace_treelet(sink_t* s, const uint64_t tlas, const orc_ray* in, orc_
   uint64_t current_root = tlas;
   int host_root = 0;
   level_set(ts, 1);
+  if (!lm_set(s->lm, topRoot, 1)) goto oom; /* tree_level_map[topRootAddr] = 1 (:1710) */
   tent z;
   memset(&z, 0, sizeof(z));
   {
@@ -1117,6 +1206,7 @@ static int trace_treelet(sink_t* s, const uint64_t tlas, const orc_ray* in, orc_
           child_hit[i] = 0;
       }
       uint64_t child = cn.addr + (int64_t)node.child_offset * 64;
+      const uint32_t node_map = lm_get(s->lm, cn.addr); /* the node's value now, not when it was pushed */
       for (int i = 0; i < 6; i++) {
         if (child_hit[i]) {
           if (node.type[i] != 0 && node.type[i] != 1) {
@@ -1135,6 +1225,7 @@ static int trace_treelet(sink_t* s, const uint64_t tlas, const orc_ray* in, orc_
             goto oom;
           }
           level_set(ts, cn.level + 1);
+          if (!lm_set(s->lm, child, node_map + 1)) goto oom; /* :1837, 1858 */
         }
         child += node.size[i] * 64;
       }
@@ -1162,6 +1253,7 @@ static int trace_treelet(sink_t* s, const uint64_t tlas, const orc_ray* in, orc_
         goto oom;
       }
       level_set(ts, cn.level);
+      if (!lm_set(s->lm, e.addr, lm_get(s->lm, cn.addr))) goto oom; /* :1944 */
     } else if (!cn.top && !cn.leaf) { /* :1957-2072 */
       CHECK_RANGE(cn.addr, 64);
       inode_t node = unpack_inode(A + cn.addr);
@@ -1180,6 +1272,7 @@ static int trace_treelet(sink_t* s, const uint64_t tlas, const orc_ray* in, orc_
           child_hit[i] = 0;
       }
       uint64_t child = cn.addr + (int64_t)node.child_offset * 64;
+      const uint32_t node_map = lm_get(s->lm, cn.addr);
       for (int i = 0; i < 6; i++) {
         if (child_hit[i]) {
           CHECK_RANGE(child, 64);
@@ -1193,6 +1286,7 @@ static int trace_treelet(sink_t* s, const uint64_t tlas, const orc_ray* in, orc_
             goto oom;
           }
           level_set(ts, cn.level + 1);
+          if (!lm_set(s->lm, child, node_map + 1)) goto oom; /* :2034, 2055 */
         }
         child += node.size[i] * 64;
       }
@@ -1236,7 +1330,8 @@ static int trace_treelet(sink_t* s, const uint64_t tlas, const orc_ray* in, orc_
   h->fetch_bytes = s->fetch_bytes;
   h->accessed_bytes = s->accessed_bytes;
   h->n_any_hits = 0;
-  h->max_level = ts->max_level;
+  h->max_level = lm_max_and_clear(s->lm);
+  h->path_level = ts->max_level;
   return ORC_OK;
 oom:
   ts->err = ORC_ERR_OUT_OF_MEMORY;
@@ -1248,6 +1343,7 @@ static int trace_range(const uint8_t* A, uint64_t bytes, uint64_t tlas, const or
                        const orc_ray* rays, uint32_t begin, uint32_t end, orc_hit* hits, orc_record* records,
                        uint64_t cap, uint64_t* offsets, uint64_t* cursor_io, int* overflow) {
   tdeque cur = {0}, other = {0};
+  lmap lm = {0};
   int st = ORC_OK;
   uint64_t cursor = cursor_io ? *cursor_io : 0;
   if (mode == 1 && (!td_grow(&cur) || !td_grow(&other))) {
@@ -1263,6 +1359,8 @@ static int trace_range(const uint8_t* A, uint64_t bytes, uint64_t tlas, const or
     s.out = records;
     s.cap = cap;
     s.base = cursor;
+    s.lm = &lm;
+    (void)lm_max_and_clear(&lm); /* a fresh map per ray */
     st = mode == 0 ? trace_plain(&s, tlas, &rays[i], &hits[i]) : trace_treelet(&s, tlas, &rays[i], &hits[i], &cur, &other);
     if (st != ORC_OK) goto out;
     if (s.overflow) *overflow = 1;
@@ -1273,6 +1371,7 @@ out:
   if (cursor_io) *cursor_io = cursor;
   free(cur.e);
   free(other.e);
+  lm_free(&lm);
   return st;
 }
 

@@ -45,7 +45,7 @@ typedef struct orc_hit {
   uint32_t n_records, n_nodes, fetch_bytes, n_any_hits, max_level;
   uint64_t instance_leaf;  /* closest_instanceLeaf (vulkan_ray_tracing.cc:2856), ~0 on a miss */
   uint32_t accessed_bytes; /* accessedDataSize of the ray (:2260): every record size */
-  uint32_t reserved;
+  uint32_t path_level;     /* deepest level along the traversal paths (max_level is the map's maximum) */
 } orc_hit;
 
 typedef struct orc_record {
@@ -53,7 +53,9 @@ typedef struct orc_record {
   uint32_t treelet;
   uint16_t size;
   uint8_t type;
-  uint8_t level; /* tree level of the node (tree_level_map), 0 for the TLAS header, saturating at 255 */
+  uint8_t level; /* depth of the node along the path that reached it (TLAS root 1, a BLAS root and
+                    its header its instance leaf's, a child its parent's + 1; 0 for the TLAS
+                    header), saturating at 255 */
 } orc_record;
 
 typedef struct orc_partition orc_partition;

@@ -30,7 +30,7 @@ HIT_DTYPE = np.dtype([("hit", "<u4"), ("primitive_index", "<u4"), ("geometry_ind
                       ("instance_index", "<u4"), ("t", "<f4"), ("p", "<f4", 3), ("bary", "<f4", 3),
                       ("n_records", "<u4"), ("n_nodes", "<u4"), ("fetch_bytes", "<u4"), ("n_any_hits", "<u4"),
                       ("max_level", "<u4"), ("instance_leaf", "<u8"), ("accessed_bytes", "<u4"),
-                      ("reserved", "<u4")])
+                      ("path_level", "<u4")])
 RECORD_DTYPE = np.dtype([("offset", "<u8"), ("treelet", "<u4"), ("size", "<u2"), ("type", "u1"),
                          ("level", "u1")])
 

@@ -108,6 +108,30 @@ def tiny_shared_arena():
     return np.frombuffer(bytes(A), dtype=np.uint8).copy()
 
 
+def tiny_two_depth_arena():
+    """One BLAS under instance leaves at two tree levels (1,024 bytes):
+        0   TLAS header     RootNodeOffset 64
+       64   TLAS root       children at 128: I1 (instance, ChildSize 2), T1 (internal, ChildSize 1)
+      128   I1              identity, BVHAddress 320 (-> 448), InstanceID 7   (level 2)
+      256   T1              one child at 320: I2 (instance, ChildSize 2)
+      320   I2              identity, BVHAddress 128 (-> 448), InstanceID 8   (level 3)
+      448.. the tiny arena's BLAS (bytes 256..832 of tiny_arena), moved up by 192
+    Every child box is x, y in [-1, 3], z in [1, 5].
+    """
+    src = tiny_arena()
+    A = bytearray(1024)
+    A[0:64] = _header(64, (-1, -1, 0), (3, 3, 6))
+    full = (0, 128, 0, 128, 0, 128)
+    A[64:128] = _internal((-1, -1, 1), (3, 3, 3), 1, [(2, 1, full), (1, 0, full)])
+    A[256:320] = _internal((-1, -1, 1), (3, 3, 3), 1, [(2, 1, full)])
+    for at, bvh, iid in ((128, 320, 7), (320, 128, 8)):
+        leaf = bytearray(src[128:256].tobytes())
+        struct.pack_into("<QII", leaf, 64, bvh, iid, 0)
+        A[at:at + 128] = leaf
+    A[448:1024] = src[256:832].tobytes()
+    return np.frombuffer(bytes(A), dtype=np.uint8).copy()
+
+
 def tiny_ray(ray_dtype, flags=1):
     r = np.zeros(1, dtype=ray_dtype)
     r["org"] = (0.25, 0.25, 0.0)

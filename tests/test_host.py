@@ -28,7 +28,7 @@ def test_library_exports_every_declared_symbol(rtx, header):
 
 
 def test_status_strings(rtx):
-    for s in range(15):
+    for s in range(16):
         assert rtx._lib.rtx_status_string(s)
     assert rtx.version().startswith("rtx-mi355x")
 

@@ -244,6 +244,55 @@ def test_tiny_treelet(oracle, case, budget):
         assert roots[int(r["treelet"])] == owner_root
 
 
+@pytest.mark.parametrize("mode,case", [(0, "plain_miss"), (1, "treelet_miss_8192")])
+def test_tree_level_map_keeps_the_last_assignment(oracle, mode, case):
+    """One BLAS under instance leaves at levels 3 and 2: the second instance
+    reassigns every BLAS node one level lower, and the ray's level is the
+    maximum over the final map (vulkan_ray_tracing.cc:2274-2279), 4, not the
+    deepest assignment, 5.  Records and counters are hand-derived."""
+    exp = tiny_scene.expected()["two_depth"]
+    arena = tiny_scene.tiny_two_depth_arena()
+    ray = tiny_scene.tiny_ray(oracle.RAY_DTYPE)
+    ray["org"] = exp["ray_miss"]["org"]
+    part = oracle.Partition(arena, 0, 8192) if mode else None
+    if part is not None:
+        assert part.n_treelets == 1  # every push goes to the current list
+    h, off, rec = oracle.trace(arena, 0, ray, mode=mode, partition=part)
+    got = [[int(r["offset"]), int(r["size"]), int(r["type"]), int(r["level"])] for r in rec]
+    e = exp[case]
+    assert got == e["records"]
+    for k in ("hit", "n_nodes", "fetch_bytes", "max_level", "path_level"):
+        assert h[0][k] == e[k], k
+    # the opaque centre ray culls every box under the second instance
+    h, _, _ = oracle.trace(arena, 0, tiny_scene.tiny_ray(oracle.RAY_DTYPE), mode=0)
+    assert h[0]["max_level"] == exp["plain_opaque_centre"]["max_level"]
+    assert h[0]["path_level"] == exp["plain_opaque_centre"]["path_level"]
+
+
+def test_map_level_is_the_path_level_when_every_node_has_one_level(oracle, c1, instanced, instanced_shared):
+    """Where each node is reachable at one level only (tree-shaped scenes, and a
+    shared BLAS whose instance leaves sit at one depth), every assignment of a
+    node carries the same value, so the final map's maximum is the deepest path
+    level; the two-depth arena is where they part."""
+    for scene, rays in (c1, instanced, instanced_shared):
+        for mode in (0, 1):
+            part = oracle.Partition(scene.arena, scene.tlas_offset, 1024) if mode else None
+            h, _, _ = oracle.trace(scene.arena, scene.tlas_offset, rays[:2000], mode=mode, partition=part)
+            assert np.array_equal(h["max_level"], h["path_level"])
+    arena = tiny_scene.tiny_two_depth_arena()
+    rng = np.random.default_rng(5)
+    rays = np.zeros(400, dtype=oracle.RAY_DTYPE)
+    rays["org"][:, 0] = rng.uniform(-1, 3, 400)
+    rays["org"][:, 1] = rng.uniform(-1, 3, 400)
+    rays["dir"] = (0, 0, 1)
+    rays["tmax"] = 100
+    rays["flags"] = rng.integers(0, 2, 400)
+    rays["cull_mask"] = 0xFF
+    h, _, _ = oracle.trace(arena, 0, rays, mode=0)
+    assert (h["max_level"] <= h["path_level"]).all()
+    assert (h["max_level"] < h["path_level"]).any() and (h["max_level"] == h["path_level"]).any()
+
+
 @pytest.mark.parametrize("budget", ["8192", "256", "192"])
 def test_tiny_partition(oracle, budget):
     exp = tiny_scene.expected()["partitions"][budget]

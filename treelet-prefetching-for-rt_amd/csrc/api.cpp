@@ -40,7 +40,10 @@ struct rtx_ctx {
   uint8_t* arena_owned = nullptr;
   uint64_t arena_bytes = 0;
   uint64_t tlas = 0;
-  bool ordered = false;  // every child box the TLAS reaches has lo <= hi (boxes_ordered)
+  bool ordered = false;  // every child box the TLAS reaches has lo <= hi (scene_check)
+  // every reachable node has one tree level (scene_check); otherwise hits take
+  // their max_level from the level pass
+  bool levels_consistent = false;
   // partition
   Partition part;
   // scratch
@@ -69,6 +72,11 @@ struct rtx_ctx {
   size_t slab_entries = 0;
   uint32_t* overflow_list = nullptr;
   size_t overflow_entries = 0;
+  uint32_t* rec_counts = nullptr;  // per-ray record counts of a slab traversal
+  size_t rec_count_entries = 0;
+  // level pass: per-lane tree_level_maps, zeroed when allocated (the pass
+  // leaves them zeroed)
+  uint8_t* lm_mem = nullptr;
   rtx_prefetch_stats last_stats{};
   // learned prefetch table of the current partition
   uint4* pred_table = nullptr;
@@ -78,7 +86,8 @@ struct rtx_ctx {
     uint8_t* arena = nullptr;
     uint64_t bytes = 0;
     uint64_t tlas = 0;
-    bool ordered = false;  // boxes_ordered() of the arena
+    bool ordered = false;            // scene_check() of the arena
+    bool levels_consistent = false;  // scene_check() of the arena
     Partition part;
     uint64_t part_gen = 0;  // bumped whenever the scene's partition is rebuilt
   };
@@ -134,7 +143,7 @@ rtx_status kernel_error_status(uint32_t bits) {
   if (bits & kErrRecords) return RTX_ERR_TRACE_OVERFLOW;
   if (bits & kErrBudget) return RTX_ERR_NODE_LARGER_THAN_BUDGET;
   if (bits & kErrInstSlot) return RTX_ERR_UNSUPPORTED;
-  if (bits & kErrSteps) return RTX_ERR_BAD_NODE_TYPE;
+  if (bits & kErrSteps) return RTX_ERR_TIMEOUT;
   return RTX_ERR_HIP;
 }
 
@@ -144,6 +153,7 @@ void free_scene(rtx_ctx* c) {
   c->arena = nullptr;
   c->arena_bytes = 0;
   c->ordered = false;
+  c->levels_consistent = false;
 }
 
 rtx_status ensure_spill(rtx_ctx* c, size_t entries) {
@@ -154,6 +164,28 @@ rtx_status ensure_spill(rtx_ctx* c, size_t entries) {
   HIP_TRY(hipMalloc(&c->spill, entries * sizeof(uint64_t)));
   c->spill_entries = entries;
   return RTX_OK;
+}
+
+// The level pass's memory: one 256-lane workgroup per CU, each lane a map of
+// kLevelMapCap entries and its position list (2.5 GiB on 256 CUs), zeroed
+// once; the pass leaves every map zeroed for the next.
+rtx_status ensure_level_mem(rtx_ctx* c) {
+  if (c->lm_mem) return RTX_OK;
+  const uint64_t bytes = (uint64_t)c->n_cus * 256 * level_pass_lane_bytes(kLevelMapCap);
+  if (hipMalloc(&c->lm_mem, bytes) != hipSuccess) {
+    (void)hipGetLastError();
+    c->lm_mem = nullptr;
+    return RTX_ERR_OUT_OF_MEMORY;
+  }
+  HIP_TRY(hipMemsetAsync(c->lm_mem, 0, bytes, c->stream));
+  return RTX_OK;
+}
+
+// Point p's level-pass fields at `mem`, laid out for `lanes` lanes.
+void set_level_maps(TraceParams& p, uint8_t* mem, uint64_t lanes) {
+  p.lm_table = reinterpret_cast<uint64_t*>(mem);
+  p.lm_list = reinterpret_cast<uint32_t*>(mem + lanes * kLevelMapCap * sizeof(uint64_t));
+  p.lm_cap = kLevelMapCap;
 }
 
 // occupancy-cache keys of the eight-lanes-per-ray kernels: this plus the
@@ -234,6 +266,7 @@ const char* rtx_status_string(rtx_status s) {
     case RTX_ERR_OUT_OF_MEMORY: return "out of memory";
     case RTX_ERR_TIMEOUT: return "timeout";
     case RTX_ERR_STALE: return "batch graph is stale (partition rebuilt or context destroyed)";
+    case RTX_ERR_CYCLE: return "the BVH has a cycle";
   }
   return "unknown status";
 }
@@ -313,6 +346,8 @@ rtx_status rtx_destroy(rtx_ctx* c) {
   if (c->scan_tmp) hipFree(c->scan_tmp);
   if (c->slabs) hipFree(c->slabs);
   if (c->overflow_list) hipFree(c->overflow_list);
+  if (c->rec_counts) hipFree(c->rec_counts);
+  if (c->lm_mem) hipFree(c->lm_mem);
   for (auto& sc : c->scenes) {
     if (sc.arena) hipFree(sc.arena);
     partition_free(sc.part);
@@ -440,7 +475,9 @@ rtx_status rtx_set_scene(rtx_ctx* c, const void* host, size_t bytes, uint64_t tl
   c->arena = d;
   c->arena_bytes = bytes;
   c->tlas = tlas;
-  c->ordered = boxes_ordered(d, bytes, tlas, c->stream);
+  const SceneCheck ck = scene_check(d, bytes, tlas, c->stream);
+  c->ordered = ck.valid && ck.ordered;
+  c->levels_consistent = ck.valid && ck.level_consistent;
   return RTX_OK;
 }
 
@@ -457,7 +494,9 @@ rtx_status rtx_set_scene_device(rtx_ctx* c, const void* dev, size_t bytes, uint6
   c->arena = (const uint8_t*)dev;
   c->arena_bytes = bytes;
   c->tlas = tlas;
-  c->ordered = boxes_ordered(c->arena, bytes, tlas, c->stream);
+  const SceneCheck ck = scene_check(c->arena, bytes, tlas, c->stream);
+  c->ordered = ck.valid && ck.ordered;
+  c->levels_consistent = ck.valid && ck.level_consistent;
   return RTX_OK;
 }
 
@@ -685,12 +724,16 @@ rtx_status rtx_trace(rtx_ctx* c, const rtx_ray* rays, uint32_t n, const rtx_trac
   if (use_slab) {
     st = ensure_buffer(c->slabs, c->slab_entries, (size_t)n * slab);
     if (st != RTX_OK) return st;
+    st = ensure_buffer(c->rec_counts, c->rec_count_entries, (size_t)n);
+    if (st != RTX_OK) return st;
     p.records = c->slabs;
     p.record_slab = slab;
+    p.rec_counts = c->rec_counts;
   }
   st = use_slot();
   if (st != RTX_OK) return st;
   uint32_t* main_slot = p.ray_counter;
+  const TraceParams scene_params = p;  // for the level pass
   HIP_TRY(hipEventRecord(c->evk0, c->stream));
   if (wide)
     HIP_TRY(launch_trace_wide(p, opts->mode, lds_stack + spill_cap, grid, c->stream));
@@ -705,7 +748,19 @@ rtx_status rtx_trace(rtx_ctx* c, const rtx_ray* rays, uint32_t n, const rtx_trac
     const size_t tiles = (n + 1023) / 1024;
     st = ensure_scan_tmp(c, tiles + 1);
     if (st != RTX_OK) return st;
-    HIP_TRY(launch_scan_records(hits, n, offsets, c->scan_tmp, c->stream));
+    HIP_TRY(launch_scan_records(hits, use_slab ? c->rec_counts : nullptr, n, offsets, c->scan_tmp, c->stream));
+    uint32_t second = n;  // rays the writing pass traces
+    if (use_slab) {
+      // slab records to their CSR places, queued behind the scan (the kernel
+      // copies nothing when the total exceeds the capacity); word 2 of the
+      // main launch's control slot (cleared with it) counts the rays whose
+      // records overflowed their slab
+      st = ensure_buffer(c->overflow_list, c->overflow_entries, (size_t)n);
+      if (st != RTX_OK) return st;
+      HIP_TRY(launch_compact_records(c->rec_counts, n, c->slabs, slab, offsets, records, records ? capacity : 0,
+                                     c->overflow_list, main_slot + 2, c->stream));
+      HIP_TRY(hipMemcpyAsync(&second, main_slot + 2, sizeof(uint32_t), hipMemcpyDeviceToHost, c->stream));
+    }
     uint64_t total = 0;
     HIP_TRY(hipMemcpyAsync(&total, offsets + n, sizeof(uint64_t), hipMemcpyDeviceToHost, c->stream));
     HIP_TRY(hipStreamSynchronize(c->stream));
@@ -718,20 +773,12 @@ rtx_status rtx_trace(rtx_ctx* c, const rtx_ray* rays, uint32_t n, const rtx_trac
     p.records = records;
     p.record_capacity = capacity;
     p.record_offsets = offsets;
-    uint32_t second = n;  // rays the writing pass traces
     if (use_slab) {
-      // slab records to their CSR places; word 2 of the main launch's control
-      // slot (cleared with it) counts the rays whose records overflowed
-      st = ensure_buffer(c->overflow_list, c->overflow_entries, (size_t)n);
-      if (st != RTX_OK) return st;
-      HIP_TRY(launch_compact_records(hits, n, c->slabs, slab, offsets, records, c->overflow_list, main_slot + 2,
-                                     c->stream));
-      HIP_TRY(hipMemcpyAsync(&second, main_slot + 2, sizeof(uint32_t), hipMemcpyDeviceToHost, c->stream));
-      HIP_TRY(hipStreamSynchronize(c->stream));
       p.ray_list = c->overflow_list;
       p.n = second;
       p.claim_start = 0;
       p.record_slab = 0;
+      p.rec_counts = nullptr;
     }
     c->slots_clean = true;
     if (second > 0) {
@@ -742,6 +789,34 @@ rtx_status rtx_trace(rtx_ctx* c, const rtx_ray* rays, uint32_t n, const rtx_trac
       if (st != RTX_OK) return st;
       HIP_TRY(launch_trace(p, opts->mode, kRecCsr, lds_stack, stage, g2, c->stream));
     }
+  }
+  if (!c->levels_consistent) {
+    // a node is reachable at two levels: the rays again, keeping the
+    // reference's tree_level_map, for max_level
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    st = launch_error();
+    if (st != RTX_OK) return st;
+    st = ensure_level_mem(c);
+    if (st != RTX_OK) return st;
+    TraceParams lp = scene_params;
+    const uint32_t lm_grid = std::max(1u, std::min<uint32_t>(c->n_cus, (n + 255) / 256));
+    st = ensure_spill(c, (size_t)lm_grid * 256 * spill_cap * ns);
+    if (st != RTX_OK) return st;
+    lp.spill = c->spill;
+    lp.records = nullptr;
+    lp.record_slab = 0;
+    lp.rec_counts = nullptr;
+    lp.prefetch = RTX_PREFETCH_NONE;
+    lp.ray_list = nullptr;
+    lp.claim_start = 0;
+    lp.n = n;
+    set_level_maps(lp, c->lm_mem, (uint64_t)c->n_cus * 256);
+    st = use_slot();
+    if (st != RTX_OK) return st;
+    lp.ray_counter = p.ray_counter;
+    lp.error = p.error;
+    lp.stats = p.stats;
+    HIP_TRY(launch_level_pass(lp, opts->mode, lm_grid, c->stream));
   }
   HIP_TRY(hipStreamSynchronize(c->stream));
   c->slots_clean = true;
@@ -839,6 +914,26 @@ rtx_status rtx_trace_static(rtx_ctx* c, const rtx_ray* rays, uint32_t n, const r
   HIP_TRY(hipEventRecord(c->evk1, c->stream));
   c->kernel_timed = true;
   HIP_TRY(hipStreamSynchronize(c->stream));
+  if (*herr == 0 && !c->levels_consistent) {
+    // The level pass in the same array, once the traversal is done with it:
+    // per lane a zeroed map, its position list and 128 spill entries per list.
+    const uint64_t lane_bytes = level_pass_lane_bytes(kLevelMapCap) + 128ull * 8 * ns;
+    const uint32_t lm_grid = (uint32_t)std::max<uint64_t>(
+        1, std::min<uint64_t>((spill_bytes / lane_bytes) / 256, ((uint64_t)n + 255) / 256));
+    const uint64_t lanes = (uint64_t)lm_grid * 256;
+    uint8_t* base = c->static_scratch + kStaticControlBytes;
+    TraceParams lp = p;
+    lp.ray_list = nullptr;
+    lp.claim_start = 0;
+    set_level_maps(lp, base, lanes);
+    lp.spill = reinterpret_cast<uint64_t*>(base + lanes * level_pass_lane_bytes(kLevelMapCap));
+    lp.spill_cap = 128;
+    HIP_TRY(launch_fill_zero(control, 2, c->stream));
+    HIP_TRY(launch_fill_zero(reinterpret_cast<uint32_t*>(base),
+                             (uint32_t)(lanes * level_pass_lane_bytes(kLevelMapCap) / 4), c->stream));
+    HIP_TRY(launch_level_pass(lp, opts->mode, lm_grid, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+  }
   if (*herr == 0) return RTX_OK;
   uint32_t err = 0;
   HIP_TRY(hipMemcpy(&err, control + 1, sizeof(uint32_t), hipMemcpyDeviceToHost));
@@ -1048,6 +1143,48 @@ TraceParams batch_params(rtx_ctx* c, const BatchJob* jobs_dev, uint32_t n_jobs, 
   return p;
 }
 
+// Whether any job of the batch traces a scene that needs the level pass.
+bool batch_needs_levels(const rtx_ctx* c, const rtx_batch_job* jobs, uint32_t n_jobs) {
+  for (uint32_t j = 0; j < n_jobs; ++j)
+    if (jobs[j].n_rays > 0 && !c->scenes[jobs[j].scene].levels_consistent) return true;
+  return false;
+}
+
+// The level pass of every job whose scene needs it, queued after the batch
+// launch (capturable: the ray counter is cleared by a kernel, and the error
+// word, counters[1], collects the passes' errors with the batch's).  `spill`
+// holds kBatchSpillCap entries per list for min(n_cus, rays / 256) workgroups.
+hipError_t batch_level_passes(rtx_ctx* c, const std::vector<BatchJob>& hj, const rtx_batch_job* jobs,
+                              uint32_t n_jobs, uint32_t mode, uint64_t* spill, uint32_t* counters, hipStream_t s) {
+  for (uint32_t j = 0; j < n_jobs; ++j) {
+    const auto& sc = c->scenes[jobs[j].scene];
+    if (hj[j].n == 0 || sc.levels_consistent) continue;
+    TraceParams p{};
+    p.arena = sc.arena;
+    p.arena_bytes = sc.bytes;
+    p.tlas = sc.tlas;
+    p.rays = hj[j].rays;
+    p.hits = hj[j].hits;
+    p.n = hj[j].n;
+    p.n_rays = hj[j].n;
+    p.refill = 40;
+    p.ordered_boxes = sc.ordered ? 1u : 0u;
+    p.owner = hj[j].owner;
+    p.first_treelet = hj[j].first_treelet;
+    p.spill = spill;
+    p.spill_cap = kBatchSpillCap;
+    p.ray_counter = counters;
+    p.error = counters + 1;
+    p.stats = c->stats;
+    set_level_maps(p, c->lm_mem, (uint64_t)c->n_cus * kBlock);
+    const uint32_t grid = std::max(1u, std::min<uint32_t>(c->n_cus, (hj[j].n + kBlock - 1) / kBlock));
+    hipError_t e = launch_fill_zero(counters, 1, s);
+    if (e == hipSuccess) e = launch_level_pass(p, mode, grid, s);
+    if (e != hipSuccess) return e;
+  }
+  return hipSuccess;
+}
+
 }  // namespace
 
 extern "C" {
@@ -1068,7 +1205,9 @@ rtx_status rtx_add_scene(rtx_ctx* c, const void* host, size_t bytes, uint64_t tl
   }
   sc.bytes = bytes;
   sc.tlas = tlas;
-  sc.ordered = boxes_ordered(sc.arena, bytes, tlas, c->stream);
+  const SceneCheck ck = scene_check(sc.arena, bytes, tlas, c->stream);
+  sc.ordered = ck.valid && ck.ordered;
+  sc.levels_consistent = ck.valid && ck.level_consistent;
   c->scenes.push_back(sc);
   *id = (uint32_t)(c->scenes.size() - 1);
   return RTX_OK;
@@ -1102,12 +1241,18 @@ rtx_status rtx_trace_batch(rtx_ctx* c, const rtx_batch_job* jobs, uint32_t n_job
   if (st != RTX_OK) return st;
   HIP_TRY(hipMemcpyAsync(c->jobs_dev, hj.data(), n_jobs * sizeof(BatchJob), hipMemcpyHostToDevice, c->stream));
   HIP_TRY(hipMemsetAsync(c->counters, 0, 2 * sizeof(uint32_t), c->stream));
+  const bool levels = batch_needs_levels(c, jobs, n_jobs);
+  if (levels) {
+    st = ensure_level_mem(c);
+    if (st != RTX_OK) return st;
+  }
   TraceParams p = batch_params(c, c->jobs_dev, n_jobs, total, s, c->spill, c->counters);
   c->kernel_timed = false;
   HIP_TRY(hipEventRecord(c->evk0, c->stream));
   HIP_TRY(launch_trace_batch(p, o->mode, s.lds_stack, s.grid, c->stream));
   HIP_TRY(hipEventRecord(c->evk1, c->stream));
   c->kernel_timed = true;
+  if (levels) HIP_TRY(batch_level_passes(c, hj, jobs, n_jobs, o->mode, c->spill, c->counters, c->stream));
   uint32_t err = 0;
   HIP_TRY(hipMemcpyAsync(&err, c->counters + 1, sizeof(uint32_t), hipMemcpyDeviceToHost, c->stream));
   HIP_TRY(hipStreamSynchronize(c->stream));
@@ -1145,12 +1290,19 @@ rtx_status rtx_batch_graph_create(rtx_ctx* c, const rtx_batch_job* jobs, uint32_
   }
   if (hipMemcpy(g->jobs, hj.data(), n_jobs * sizeof(BatchJob), hipMemcpyHostToDevice) != hipSuccess)
     return fail(RTX_ERR_HIP);
+  const bool levels = batch_needs_levels(c, jobs, n_jobs);
+  if (levels) {
+    st = ensure_level_mem(c);  // zeroed before the capture, never inside it
+    if (st != RTX_OK) return fail(st);
+  }
   if (hipStreamSynchronize(c->stream) != hipSuccess) return fail(RTX_ERR_HIP);
   TraceParams p = batch_params(c, g->jobs, n_jobs, total, s, g->spill, g->counters);
   if (hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal) != hipSuccess) return fail(RTX_ERR_HIP);
   // counters are cleared by a kernel: no memset node inside the graph
   hipError_t e1 = launch_fill_zero(g->counters, 2, c->stream);
   hipError_t e2 = e1 == hipSuccess ? launch_trace_batch(p, o->mode, s.lds_stack, s.grid, c->stream) : e1;
+  if (e2 == hipSuccess && levels)
+    e2 = batch_level_passes(c, hj, jobs, n_jobs, o->mode, g->spill, g->counters, c->stream);
   hipError_t e3 = hipStreamEndCapture(c->stream, &g->graph);
   if (e1 != hipSuccess || e2 != hipSuccess || e3 != hipSuccess) return fail(RTX_ERR_HIP);
   if (hipGraphInstantiate(&g->exec, g->graph, nullptr, nullptr, 0) != hipSuccess) return fail(RTX_ERR_HIP);

@@ -57,6 +57,12 @@ struct TraceParams {
   uint64_t record_capacity;
   const uint64_t* record_offsets;
   uint32_t record_slab;    // slab records: ray r's first record_slab records at records[r * record_slab]
+  uint32_t* rec_counts;    // slab records: ray r's record count at rec_counts[r]
+  // level pass: per lane a tree_level_map of lm_cap u64 entries (zero between
+  // rays) at lm_table + lane * lm_cap, and lm_cap / 2 positions at lm_list
+  uint64_t* lm_table;
+  uint32_t* lm_list;
+  uint32_t lm_cap;
   const uint64_t* layout_remap;  // nullable: records carry these layout offsets (arena slot -> offset)
   uint32_t claim_start;    // rays are claimed from this index on, wrapping to 0
   const uint32_t* ray_list;  // nullable: claim k traces ray ray_list[k] (n = list length)
@@ -119,13 +125,20 @@ uint32_t trace_batch_blocks_per_cu(uint32_t mode, uint32_t lds_stack);
 constexpr uint32_t kWideRayLanes = 8;
 hipError_t launch_trace_wide(const TraceParams& p, uint32_t mode, uint32_t cap, uint32_t grid, hipStream_t s);
 uint32_t trace_wide_blocks_per_cu(uint32_t mode);
+// The level pass over p.n rays (p.ray_list / claim order as in launch_trace):
+// each hit's max_level becomes the maximum of the reference's final
+// tree_level_map.  p.lm_table must hold grid * 256 zeroed maps of p.lm_cap
+// (a power of two) entries.
+hipError_t launch_level_pass(const TraceParams& p, uint32_t mode, uint32_t grid, hipStream_t s);
+// Bytes of level-pass memory per lane: the map and its position list.
+constexpr uint64_t level_pass_lane_bytes(uint32_t lm_cap) { return (uint64_t)lm_cap * 8 + (uint64_t)lm_cap / 2 * 4; }
+constexpr uint32_t kLevelMapCap = 4096;  // map entries per lane: at most 2,048 nodes assigned per ray
 // Clears `words` 32-bit words with a kernel (usable inside graph capture).
 hipError_t launch_fill_zero(uint32_t* dst, uint32_t words, hipStream_t s);
-hipError_t launch_scan_records(const rtx_hit* hits, uint32_t n, uint64_t* offsets, uint64_t* block_tmp,
-                               hipStream_t s);
-// Copy each ray's slab records (at most `slab`) to its CSR position; rays with
-// more than `slab` records are appended to overflow_list (count in
-// overflow_count, which must start at 0).
+// CSR offsets[0..n] of the rays' record counts: hits[i].n_records, or
+// counts[i] when `counts` is given.
+hipError_t launch_scan_records(const rtx_hit* hits, const uint32_t* counts, uint32_t n, uint64_t* offsets,
+                               uint64_t* block_tmp, hipStream_t s);
 // Statistics of a traced batch: acc[0..6] records per type, [7] hits, [8] any
 // hits, [9] nodes, [10] max nodes, [11] max level, [12] fetch bytes, [13]
 // accessed bytes (acc must start at 0; records may be null).
@@ -134,8 +147,12 @@ hipError_t launch_trace_stats(const rtx_hit* hits, uint32_t n, const rtx_record*
 // The coherence engine's claim order of n rays (coherence.hip).
 hipError_t coherence_order(const rtx_ray* rays, uint32_t n, const rtx_coherence_config& cfg, const float wmin[3],
                            const float wmax[3], uint32_t* order, uint32_t* hashes_out, hipStream_t s);
-hipError_t launch_compact_records(const rtx_hit* hits, uint32_t n, const rtx_record* slabs, uint32_t slab,
-                                  const uint64_t* offsets, rtx_record* records, uint32_t* overflow_list,
-                                  uint32_t* overflow_count, hipStream_t s);
+// Copy each ray's slab records (at most `slab`) to its CSR position; rays with
+// more than `slab` records are appended to overflow_list (count in
+// overflow_count, which must start at 0).  Nothing is copied when the total
+// (offsets[n]) exceeds `capacity`.
+hipError_t launch_compact_records(const uint32_t* counts, uint32_t n, const rtx_record* slabs, uint32_t slab,
+                                  const uint64_t* offsets, rtx_record* records, uint64_t capacity,
+                                  uint32_t* overflow_list, uint32_t* overflow_count, hipStream_t s);
 
 }  // namespace rtxk

@@ -44,12 +44,22 @@ rtx_status partition_copy_out(const Partition& p, uint64_t arena_bytes, hipStrea
 rtx_status partition_install(const uint8_t* arena, uint64_t bytes, uint64_t tlas, uint32_t budget, uint32_t n,
                              const uint64_t* roots, const uint64_t* list_offsets, const uint64_t* list_nodes,
                              const uint32_t* owner, hipStream_t s, Partition& out);
-bool boxes_ordered(const uint8_t* arena, uint64_t bytes, uint64_t tlas, hipStream_t s);
+// What the structural walk of a scene found.  On a malformed arena (or no
+// scratch memory) `valid` is false and the rest keeps its safe values: the
+// compare-select slab tests, and the level pass.
+struct SceneCheck {
+  bool valid = false;
+  bool ordered = false;           // every child box the TLAS reaches has lo <= hi bytes on all axes
+  bool level_consistent = false;  // every reachable node has one tree level (every path agrees)
+  bool cyclic = false;            // some node reaches itself
+};
+SceneCheck scene_check(const uint8_t* arena, uint64_t bytes, uint64_t tlas, hipStream_t s);
 // remapBVHToTreeletLayout of the partition with -treelet_remap_stride `stride`
 // (and the per-treelet metadata table); RTX_ERR_UNSUPPORTED where the
 // reference asserts (a treelet root already placed by an earlier treelet).
 rtx_status partition_layout(const uint8_t* arena, uint64_t bytes, uint64_t tlas, uint32_t stride, hipStream_t s,
                             Partition& p);
+// The structural walk alone: RTX_ERR_CYCLE when a reachable node reaches itself.
 rtx_status partition_validate(const uint8_t* arena, uint64_t bytes, uint64_t tlas, hipStream_t s,
                               uint64_t* n_nodes);
 

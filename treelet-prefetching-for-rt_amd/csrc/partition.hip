@@ -398,7 +398,19 @@ struct ValParams {
   uint32_t* shared;
   uint32_t* error;
   uint32_t* unordered;  // set when a valid child box has a lower bound byte above its upper one
+  // tree levels (tree_level_map values): nodes processed in this round are at
+  // level round + 1 (the TLAS root at 1), their children and a BLAS root
+  // reached through an instance leaf at round + 2; lvl[slot] keeps the first
+  // level seen and `inconsistent` is set when a node is reached at another
+  uint32_t* lvl;
+  uint32_t round;
+  uint32_t* inconsistent;
 };
+
+__device__ __forceinline__ void level_ref(const ValParams& p, uint64_t off, uint32_t level) {
+  const uint32_t prev = atomicCAS(&p.lvl[off >> 6], 0u, level);
+  if (prev != 0u && prev != level) atomicOr(p.inconsistent, 1u);
+}
 
 __device__ void val_ref(const ValParams& p, uint64_t off, uint8_t k, bool push, uint32_t qkind) {
   const uint32_t s = (uint32_t)(off >> 6);
@@ -422,6 +434,7 @@ __global__ void validate_level(ValParams p) {
     const uint8_t* nd = p.A + off;
     const int32_t co = *reinterpret_cast<const int32_t*>(nd + 12);
     uint64_t c = off + (uint64_t)((int64_t)co * 64);
+    const uint32_t child_level = p.round + 2;
     for (int k = 0; k < 6; ++k) {
       const uint32_t info = nd[22 + k] & 0x3fu;
       const uint32_t size = info & 3u, type = info >> 2;
@@ -433,6 +446,7 @@ __global__ void validate_level(ValParams p) {
         atomicOr(p.error, kErrRange);
         return;
       }
+      level_ref(p, c, child_level);
       if (type == 0) {
         val_ref(p, c, kSlotInternal, true, kind);
       } else if (kind == kTopInternal) {
@@ -451,6 +465,7 @@ __global__ void validate_level(ValParams p) {
           return;
         }
         val_ref(p, c, kSlotInstance, false, 0);
+        level_ref(p, r, child_level);  // tree_level_map[botLevelRootAddr] = tree_level_map[leaf_addr]
         const uint32_t hs = (uint32_t)(h >> 6);
         if (atomicAdd(&p.refs[hs], 1u) == 0u) {
           p.kind[hs] = kSlotHeader;
@@ -468,6 +483,60 @@ __global__ void validate_level(ValParams p) {
       c += len;
     }
   }
+}
+
+// ---- cycle check: Kahn's peeling over the graph the walk above counted ------
+// refs[s] holds each reachable slot's in-degree from the walk (every reference
+// an expanded node made; a BLAS header one per instance leaf, its root one from
+// the header).  A round takes the internal nodes whose in-degree reached zero
+// and removes their edges in the same order the walk added them; the graph has
+// a cycle exactly when some count never reaches zero.
+struct KahnParams {
+  const uint8_t* A;
+  const uint64_t* in;
+  uint32_t n_in;
+  uint64_t* out;
+  uint32_t* n_out;
+  uint32_t* refs;
+};
+
+__device__ __forceinline__ void kahn_push(const KahnParams& p, uint64_t off, uint32_t kind) {
+  const uint32_t o = atomicAdd(p.n_out, 1u);
+  p.out[o] = qentry((uint32_t)(off >> 6), kind);
+}
+
+__global__ void kahn_level(KahnParams p) {
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < p.n_in; i += gridDim.x * blockDim.x) {
+    const uint64_t e = p.in[i];
+    const uint64_t off = (uint64_t)q_slot(e) << 6;
+    const uint32_t kind = q_kind(e);
+    const uint8_t* nd = p.A + off;
+    const int32_t co = *reinterpret_cast<const int32_t*>(nd + 12);
+    uint64_t c = off + (uint64_t)((int64_t)co * 64);
+    for (int k = 0; k < 6; ++k) {
+      const uint32_t info = nd[22 + k] & 0x3fu;
+      const uint32_t size = info & 3u, type = info >> 2;
+      if (size == 0) continue;
+      if (type == 0) {
+        if (atomicSub(&p.refs[c >> 6], 1u) == 1u) kahn_push(p, c, kind);
+      } else if (kind == kTopInternal) {
+        atomicSub(&p.refs[c >> 6], 1u);
+        const uint64_t h = c + *reinterpret_cast<const uint64_t*>(p.A + c + 64);
+        const uint64_t r = h + *reinterpret_cast<const uint64_t*>(p.A + h);
+        if (atomicSub(&p.refs[h >> 6], 1u) == 1u && atomicSub(&p.refs[r >> 6], 1u) == 1u) kahn_push(p, r, kBotInternal);
+      } else {
+        atomicSub(&p.refs[c >> 6], 1u);
+      }
+      c += (uint64_t)size * 64;
+    }
+  }
+}
+
+__global__ void count_nonzero(const uint32_t* v, uint64_t n, uint32_t* count) {
+  uint32_t local = 0;
+  for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
+    local += v[i] != 0u ? 1u : 0u;
+  if (local) atomicAdd(count, local);
 }
 
 template <typename T>
@@ -518,18 +587,21 @@ rtx_status map_err(uint32_t bits) {
 
 // Walks the DAG; fills kind[] and reports whether any node is referenced twice.
 rtx_status run_validate(const uint8_t* A, uint64_t bytes, uint64_t tlas, hipStream_t s, Scratch& sc, uint8_t* kind,
-                        uint32_t* shared_out, uint64_t* n_nodes, uint32_t* unordered_out = nullptr) {
+                        uint32_t* shared_out, uint64_t* n_nodes, uint32_t* unordered_out = nullptr,
+                        SceneCheck* check = nullptr) {
   const uint64_t slots = bytes / 64;
-  uint32_t* refs;
+  uint32_t *refs, *lvl;
   uint64_t *qa, *qb;
-  uint32_t* ctr;  // [0] n_out, [1] error, [2] shared, [3] unordered
+  uint32_t* ctr;  // [0] n_out, [1] error, [2] shared, [3] unordered, [4] inconsistent, [5] nonzero
   CS(sc.alloc(&refs, slots));
+  CS(sc.alloc(&lvl, slots));
   CS(sc.alloc(&qa, slots));
   CS(sc.alloc(&qb, slots));
-  CS(sc.alloc(&ctr, 4));
+  CS(sc.alloc(&ctr, 8));
   CK(hipMemsetAsync(refs, 0, slots * 4, s));
+  CK(hipMemsetAsync(lvl, 0, slots * 4, s));
   CK(hipMemsetAsync(kind, 0, slots, s));
-  CK(hipMemsetAsync(ctr, 0, 16, s));
+  CK(hipMemsetAsync(ctr, 0, 32, s));
   uint64_t hdr[1];
   CK(hipMemcpyAsync(hdr, A + tlas, 8, hipMemcpyDeviceToHost, s));
   CK(hipStreamSynchronize(s));
@@ -540,14 +612,16 @@ rtx_status run_validate(const uint8_t* A, uint64_t bytes, uint64_t tlas, hipStre
   CK(hipMemcpyAsync(refs + tlas / 64, &one, 4, hipMemcpyHostToDevice, s));
   CK(hipMemcpyAsync(kind + tlas / 64, &khdr, 1, hipMemcpyHostToDevice, s));
   CK(hipMemcpyAsync(refs + root / 64, &one, 4, hipMemcpyHostToDevice, s));
+  CK(hipMemcpyAsync(lvl + root / 64, &one, 4, hipMemcpyHostToDevice, s));  // tree_level_map[topRootAddr] = 1
   CK(hipMemcpyAsync(kind + root / 64, &kint, 1, hipMemcpyHostToDevice, s));
   uint64_t e0 = (root / 64) | ((uint64_t)kTopInternal << 32);
   CK(hipMemcpyAsync(qa, &e0, 8, hipMemcpyHostToDevice, s));
   uint32_t n_in = 1;
   uint64_t total = 2;
+  uint32_t round = 0;
   while (n_in > 0) {
     CK(hipMemsetAsync(ctr, 0, 4, s));
-    ValParams vp{A, bytes, qa, n_in, qb, ctr, refs, kind, ctr + 2, ctr + 1, ctr + 3};
+    ValParams vp{A, bytes, qa, n_in, qb, ctr, refs, kind, ctr + 2, ctr + 1, ctr + 3, lvl, round++, ctr + 4};
     uint32_t grid = std::min<uint32_t>(4096, (n_in + 255) / 256);
     hipLaunchKernelGGL(validate_level, dim3(grid), dim3(256), 0, s, vp);
     CK(hipGetLastError());
@@ -560,11 +634,45 @@ rtx_status run_validate(const uint8_t* A, uint64_t bytes, uint64_t tlas, hipStre
     std::swap(qa, qb);
   }
   // count every referenced slot (leaves and headers included)
-  uint32_t c[4];
-  CK(hipMemcpyAsync(c, ctr, 16, hipMemcpyDeviceToHost, s));
+  uint32_t c[5];
+  CK(hipMemcpyAsync(c, ctr, 20, hipMemcpyDeviceToHost, s));
   CK(hipStreamSynchronize(s));
   if (shared_out) *shared_out = c[2];
   if (unordered_out) *unordered_out = c[3];
+  if (check) {
+    check->ordered = c[3] == 0;
+    check->level_consistent = c[4] == 0;
+    check->cyclic = false;
+    // A cycle reaches a node at two levels, so only inconsistent scenes are peeled.
+    if (!check->level_consistent) {
+      uint32_t r0 = 0;
+      CK(hipMemcpyAsync(&r0, refs + root / 64, 4, hipMemcpyDeviceToHost, s));
+      CK(hipStreamSynchronize(s));
+      r0 -= 1;  // the walk's seed reference
+      CK(hipMemcpyAsync(refs + root / 64, &r0, 4, hipMemcpyHostToDevice, s));
+      const uint32_t zero = 0;
+      CK(hipMemcpyAsync(refs + tlas / 64, &zero, 4, hipMemcpyHostToDevice, s));
+      n_in = r0 == 0 ? 1u : 0u;  // an edge into the TLAS root closes a cycle
+      CK(hipMemcpyAsync(qa, &e0, 8, hipMemcpyHostToDevice, s));
+      while (n_in > 0) {
+        CK(hipMemsetAsync(ctr, 0, 4, s));
+        KahnParams kp{A, qa, n_in, qb, ctr, refs};
+        uint32_t grid = std::min<uint32_t>(4096, (n_in + 255) / 256);
+        hipLaunchKernelGGL(kahn_level, dim3(grid), dim3(256), 0, s, kp);
+        CK(hipGetLastError());
+        CK(hipMemcpyAsync(&n_in, ctr, 4, hipMemcpyDeviceToHost, s));
+        CK(hipStreamSynchronize(s));
+        std::swap(qa, qb);
+      }
+      const uint32_t g = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(4096, (slots + 255) / 256));
+      hipLaunchKernelGGL(count_nonzero, dim3(g), dim3(256), 0, s, refs, slots, ctr + 5);
+      CK(hipGetLastError());
+      uint32_t left = 0;
+      CK(hipMemcpyAsync(&left, ctr + 5, 4, hipMemcpyDeviceToHost, s));
+      CK(hipStreamSynchronize(s));
+      check->cyclic = left != 0;
+    }
+  }
   if (n_nodes) {
     std::vector<uint8_t> k(slots);
     CK(hipMemcpyAsync(k.data(), kind, slots, hipMemcpyDeviceToHost, s));
@@ -832,24 +940,28 @@ rtx_status partition_layout(const uint8_t* A, uint64_t bytes, uint64_t tlas, uin
 // the child's decoded box has lo <= hi).  False as well when the walk finds a
 // malformed arena or cannot allocate its scratch: traversal then keeps the
 // compare-select slab tests, and a malformed arena still reports at trace time.
-bool boxes_ordered(const uint8_t* A, uint64_t bytes, uint64_t tlas, hipStream_t s) {
+SceneCheck scene_check(const uint8_t* A, uint64_t bytes, uint64_t tlas, hipStream_t s) {
+  SceneCheck out;
   Scratch sc;
   uint8_t* kind;
-  if (sc.alloc(&kind, bytes / 64) != RTX_OK) return false;
-  uint32_t unordered = 1;
-  if (run_validate(A, bytes, tlas, s, sc, kind, nullptr, nullptr, &unordered) != RTX_OK) {
+  if (sc.alloc(&kind, bytes / 64) != RTX_OK) return out;
+  SceneCheck ck;
+  if (run_validate(A, bytes, tlas, s, sc, kind, nullptr, nullptr, nullptr, &ck) != RTX_OK) {
     (void)hipGetLastError();
     (void)hipStreamSynchronize(s);
-    return false;
+    return out;
   }
-  return unordered == 0;
+  ck.valid = true;
+  return ck;
 }
 
 rtx_status partition_validate(const uint8_t* A, uint64_t bytes, uint64_t tlas, hipStream_t s, uint64_t* n_nodes) {
   Scratch sc;
   uint8_t* kind;
   CS(sc.alloc(&kind, bytes / 64));
-  return run_validate(A, bytes, tlas, s, sc, kind, nullptr, n_nodes);
+  SceneCheck ck;
+  CS(run_validate(A, bytes, tlas, s, sc, kind, nullptr, n_nodes, nullptr, &ck));
+  return ck.cyclic ? RTX_ERR_CYCLE : RTX_OK;
 }
 
 rtx_status build_images(const uint8_t* A, uint64_t bytes, hipStream_t s, const uint8_t* kind, Partition& p);

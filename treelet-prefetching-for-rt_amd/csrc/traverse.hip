@@ -157,10 +157,17 @@ __device__ __forceinline__ uint32_t wave_sum(uint32_t v) {
 
 // Non-staged kernels are compiled for five waves per SIMD (at most 96 VGPRs);
 // staged kernels are bounded by their LDS image instead.
-template <int MODE, int REC, int K, int STAGE, bool MULTI = false>
-__global__ __launch_bounds__(STAGE ? 64 : 256) __attribute__((amdgpu_waves_per_eu(STAGE ? 1 : REC ? 4 : 5)))
+// LM: the level pass.  The rays are traced again with the reference's
+// tree_level_map kept per lane in global memory, and only each hit's max_level
+// is written: the maximum over the final map (vulkan_ray_tracing.cc:2274-2279),
+// where a node reassigned later keeps its last value.  Needed only for scenes
+// in which a node is reachable at two levels; elsewhere every assignment of a
+// node carries the same value and the traversal's running maximum is exact.
+template <int MODE, int REC, int K, int STAGE, bool MULTI = false, bool LM = false>
+__global__ __launch_bounds__(STAGE ? 64 : 256) __attribute__((amdgpu_waves_per_eu(STAGE ? 1 : (REC == kRecCsr || (MODE == 1 && REC) || LM) ? 4 : 5)))
 void trace_kernel(TraceParams p) {
   static_assert(!MULTI || (!REC && STAGE == 0), "batch kernels write hits only");
+  static_assert(!LM || (!REC && STAGE == 0 && !MULTI), "the level pass writes max_level only");
   constexpr uint32_t BS = STAGE ? 64u : 256u;
   constexpr int NS = MODE == 1 ? 2 : 1;
   constexpr uint32_t UNITS = STAGE / 64;
@@ -255,7 +262,8 @@ void trace_kernel(TraceParams p) {
   auto emit = [&](uint64_t off, uint32_t size, uint32_t type, uint32_t lvl) {
     if (type != RTX_TX_BVH_PRIMITIVE_LEAF_DESCRIPTOR) fetch_bytes += size;
     if (REC) {
-      uint64_t idx = rec_base + n_rec;
+      // a slab's base follows from the ray index (no register held for it)
+      uint64_t idx = (REC == kRecSlab ? (uint64_t)local_id * p.record_slab : rec_base) + n_rec;
       // a slab keeps the ray's first record_slab records; the rest are written
       // by a second pass over the rays that had more
       if (REC == kRecSlab ? n_rec < p.record_slab : idx < p.record_capacity) {
@@ -281,6 +289,37 @@ void trace_kernel(TraceParams p) {
     else
       ok = st_push<K, BS>(sb, e, ldsB, gB, p.spill_cap);
     if (!ok) err |= kErrStack;
+  };
+
+  // ---- the level pass's per-lane tree_level_map: open addressing over
+  // lm_cap entries ((slot + 1) << 32 | level, 0 = empty), at most half full,
+  // and the positions this ray wrote (for the final maximum and the clear)
+  uint64_t* lm_t = nullptr;
+  uint32_t* lm_l = nullptr;
+  uint32_t lm_n = 0;
+  if constexpr (LM) {
+    lm_t = p.lm_table + lane_slot * (uint64_t)p.lm_cap;
+    lm_l = p.lm_list + lane_slot * (uint64_t)(p.lm_cap / 2);
+  }
+  auto lm_pos = [&](uint32_t slot) -> uint32_t {
+    uint32_t h = slot_hash(slot) & (p.lm_cap - 1);
+    while (true) {
+      const uint64_t v = lm_t[h];
+      if (v == 0 || (uint32_t)(v >> 32) == slot + 1) return h;
+      h = (h + 1) & (p.lm_cap - 1);
+    }
+  };
+  auto lm_get = [&](uint32_t slot) -> uint32_t { return (uint32_t)lm_t[lm_pos(slot)]; };
+  auto lm_set = [&](uint32_t slot, uint32_t level) {
+    const uint32_t h = lm_pos(slot);
+    if (lm_t[h] == 0) {
+      if (lm_n >= p.lm_cap / 2) {  // the map is full: the ray reports a stack overflow
+        err |= kErrStack;
+        return;
+      }
+      lm_l[lm_n++] = h;
+    }
+    lm_t[h] = ((uint64_t)(slot + 1) << 32) | level;
   };
 
   // TLAS header slot of the lane's scene (instance leaves are stored relative to it)
@@ -507,6 +546,22 @@ void trace_kernel(TraceParams p) {
   // to the wave's refill so the write path runs once per batch of completed
   // rays instead of in nearly every iteration.
   auto write_hit = [&]() {
+    if constexpr (LM) {
+      // the maximum over the final map; the entries are cleared for the next ray
+      uint32_t mx = 0;
+      for (uint32_t i = 0; i < lm_n; ++i) {
+        const uint32_t h = lm_l[i];
+        mx = max(mx, (uint32_t)lm_t[h]);
+        lm_t[h] = 0;
+      }
+      lm_n = 0;
+      hits_out[local_id].max_level = mx;
+      if (err) {
+        atomicOr(p.error, err);
+        if (p.host_err) *reinterpret_cast<volatile uint32_t*>(p.host_err) = 1u;
+      }
+      return;
+    }
     uint4 o0 = make_uint4(0, 0, 0, 0), o1 = make_uint4(0, 0, 0, 0), o2 = make_uint4(0, 0, 0, 0);
     uint4 o4 = make_uint4(0xffffffffu, 0xffffffffu, 0, 0);
     if (err == 0 && min_thit < rtmax && closest_leaf != kNoSlot) {
@@ -542,6 +597,7 @@ void trace_kernel(TraceParams p) {
     hq[2] = o2;
     hq[3] = make_uint4(n_nodes, fetch_bytes, n_any, max_level);
     hq[4] = o4;
+    if constexpr (REC == kRecSlab) p.rec_counts[local_id] = n_rec;
     if (err) {
       atomicOr(p.error, err);
       if (p.host_err) *reinterpret_cast<volatile uint32_t*>(p.host_err) = 1u;
@@ -616,7 +672,7 @@ void trace_kernel(TraceParams p) {
           cur_inst = kNoSlot;
           n_rec = n_nodes = fetch_bytes = n_any = 0;
           max_level = 1;  // tree_level_map[topRootAddr] = 1
-          rec_base = REC == kRecCsr ? p.record_offsets[r] : REC == kRecSlab ? (uint64_t)r * p.record_slab : 0;
+          if constexpr (REC == kRecCsr) rec_base = p.record_offsets[r];
           sa = Stack{0, 0, 0};
           sb = Stack{0, 0, 0};
           phase2 = false;
@@ -640,6 +696,8 @@ void trace_kernel(TraceParams p) {
             F3 lo = mk3(__uint_as_float(h0.z), __uint_as_float(h0.w), __uint_as_float(h1.x));
             F3 hi = mk3(__uint_as_float(h1.y), __uint_as_float(h1.z), __uint_as_float(h1.w));
             float th;
+            // tree_level_map[topRootAddr] = 1, before the bounds test (:2470)
+            if constexpr (LM) lm_set((uint32_t)(root >> 6), 1u);
             if (ray_box(lo, hi, ridir, ro, rtmin, rtmax, th)) {
               if ((root & 63) || root + 64 > bytes) {
                 err |= kErrRange;
@@ -810,6 +868,7 @@ void trace_kernel(TraceParams p) {
             } else {
               const uint32_t rs = (uint32_t)(root >> 6);
               if (level > max_level) max_level = level;
+              if constexpr (LM) lm_set(rs, lm_get(slot));  // :2668 / :1944
               const bool to_a = MODE == 0 || (!phase2 && (first_bits(slot) & 1u));
               // The BLAS root is popped next when it goes on top of the list
               // the next pop takes: always in plain order; in treelet order
@@ -988,6 +1047,9 @@ void trace_kernel(TraceParams p) {
             if (hitmask) {
               const uint32_t lvl = level + 1;
               if (lvl > max_level) max_level = lvl;
+              // the node's map value when it is processed (:2569-2581, :2756-2767,
+              // :1837-1858, :2034-2055)
+              const uint32_t lm_child = LM ? lm_get(slot) + 1 : 0u;
               const uint32_t inst = top ? 0u : e_inst_rel(e);
               const uint32_t base_flags = top ? kFlTop : 0u;
               int deferred = -1;
@@ -1008,6 +1070,7 @@ void trace_kernel(TraceParams p) {
                   continue;
                 }
                 const uint32_t cs = (uint32_t)(c >> 6);
+                if constexpr (LM) lm_set(cs, lm_child);
                 if (MODE == 0 && !leaf && deferred < 0) {
                   deferred = i;
                   deferred_cs = cs;
@@ -1069,16 +1132,25 @@ __device__ __forceinline__ uint64_t block_exclusive_scan(uint64_t v, uint64_t* s
   return wsum + x - v;
 }
 
-__global__ __launch_bounds__(kScanBlock) void scan_tiles(const rtx_hit* hits, uint32_t n, uint64_t* out,
-                                                         uint64_t* tile_sums) {
+__global__ __launch_bounds__(kScanBlock) void scan_tiles(const rtx_hit* hits, const uint32_t* counts, uint32_t n,
+                                                         uint64_t* out, uint64_t* tile_sums) {
   __shared__ uint64_t smem[4];
   const uint64_t base = (uint64_t)blockIdx.x * kScanTile + threadIdx.x * kScanItems;
   uint64_t v[kScanItems];
   uint64_t s = 0;
+  if (counts && base + kScanItems <= n) {
+    const uint4 c = *reinterpret_cast<const uint4*>(counts + base);  // base is a multiple of 4
+    v[0] = c.x;
+    v[1] = c.y;
+    v[2] = c.z;
+    v[3] = c.w;
+    s = v[0] + v[1] + v[2] + v[3];
+  } else {
 #pragma unroll
-  for (uint32_t k = 0; k < kScanItems; ++k) {
-    v[k] = base + k < n ? hits[base + k].n_records : 0;
-    s += v[k];
+    for (uint32_t k = 0; k < kScanItems; ++k) {
+      v[k] = base + k < n ? (counts ? counts[base + k] : hits[base + k].n_records) : 0;
+      s += v[k];
+    }
   }
   uint64_t total;
   uint64_t ex = block_exclusive_scan(s, smem, total);
@@ -1580,20 +1652,23 @@ TraceFn pick_trace(uint32_t mode, int records, uint32_t lds_stack, uint32_t stag
 // range: lane i holds ray i's record count and the inclusive prefix of the
 // counts, and output position f of the range (f = lane, lane + 64, ...)
 // belongs to the ray whose prefix first exceeds f (a binary search over the
-// wave's prefixes with shuffles).  Loads and stores are 16 bytes and the
-// stores are contiguous.  Records past a ray's slab are left to the second
-// pass, which lists the ray.
-__global__ __launch_bounds__(256) void compact_records(const rtx_hit* hits, uint32_t n, const uint4* slabs,
+// wave's prefixes with shuffles).  Loads and stores are 16 bytes, the stores
+// contiguous, and each lane has four loads in flight.  Records past a ray's
+// slab are left to the second pass, which lists the ray.
+__global__ __launch_bounds__(256) void compact_records(const uint32_t* counts, uint32_t n, const uint4* slabs,
                                                        uint32_t slab, const uint64_t* offsets, uint4* records,
-                                                       uint32_t* overflow_list, uint32_t* overflow_count) {
+                                                       uint64_t capacity, uint32_t* overflow_list,
+                                                       uint32_t* overflow_count) {
+  if (offsets[n] > capacity) return;  // the caller reports RTX_ERR_TRACE_OVERFLOW
   const uint32_t lane = threadIdx.x & 63u;
   const uint32_t wave = blockIdx.x * 4 + (threadIdx.x >> 6);
   const uint32_t waves = gridDim.x * 4;
+  constexpr uint32_t kUnroll = 4;
   for (uint32_t base = wave * 64; base < n; base += waves * 64) {
     const uint32_t r = base + lane;
     uint32_t cnt = 0;
     if (r < n) {
-      cnt = hits[r].n_records;
+      cnt = counts[r];
       if (cnt > slab) overflow_list[atomicAdd(overflow_count, 1u)] = r;
     }
     uint32_t incl = cnt;
@@ -1605,18 +1680,29 @@ __global__ __launch_bounds__(256) void compact_records(const rtx_hit* hits, uint
     const uint32_t total = __shfl(incl, 63, 64);
     const uint64_t out = offsets[base];
     // wave-uniform loop: every lane takes part in the shuffles
-    for (uint32_t f0 = 0; f0 < total; f0 += 64) {
-      const uint32_t f = f0 + lane;
-      // first ray i with prefix(i) > f
-      uint32_t lo = 0;
+    for (uint32_t f0 = 0; f0 < total; f0 += 64 * kUnroll) {
+      uint4 v[kUnroll];
+      uint32_t fs[kUnroll];
+      bool ok[kUnroll];
 #pragma unroll
-      for (uint32_t step = 32; step > 0; step >>= 1) {
-        const uint32_t pre = __shfl(incl, (int)(lo + step - 1), 64);
-        if (pre <= f) lo += step;
+      for (uint32_t u = 0; u < kUnroll; ++u) {
+        const uint32_t f = f0 + u * 64 + lane;
+        // first ray i with prefix(i) > f
+        uint32_t lo = 0;
+#pragma unroll
+        for (uint32_t step = 32; step > 0; step >>= 1) {
+          const uint32_t pre = __shfl(incl, (int)(lo + step - 1), 64);
+          if (pre <= f) lo += step;
+        }
+        const uint32_t before = __shfl(incl, (int)(lo ? lo - 1 : 0), 64);
+        const uint32_t k = f - (lo ? before : 0u);
+        fs[u] = f;
+        ok[u] = f < total && k < slab;
+        if (ok[u]) v[u] = slabs[(uint64_t)(base + lo) * slab + k];
       }
-      const uint32_t before = __shfl(incl, (int)(lo ? lo - 1 : 0), 64);
-      const uint32_t k = f - (lo ? before : 0u);
-      if (f < total && k < slab) records[out + f] = slabs[(uint64_t)(base + lo) * slab + k];
+#pragma unroll
+      for (uint32_t u = 0; u < kUnroll; ++u)
+        if (ok[u]) records[out + fs[u]] = v[u];
     }
   }
 }
@@ -1739,14 +1825,21 @@ hipError_t launch_trace_stats(const rtx_hit* hits, uint32_t n, const rtx_record*
   return hipGetLastError();
 }
 
-hipError_t launch_compact_records(const rtx_hit* hits, uint32_t n, const rtx_record* slabs, uint32_t slab,
-                                  const uint64_t* offsets, rtx_record* records, uint32_t* overflow_list,
-                                  uint32_t* overflow_count, hipStream_t s) {
+hipError_t launch_compact_records(const uint32_t* counts, uint32_t n, const rtx_record* slabs, uint32_t slab,
+                                  const uint64_t* offsets, rtx_record* records, uint64_t capacity,
+                                  uint32_t* overflow_list, uint32_t* overflow_count, hipStream_t s) {
   if (n == 0 || slab == 0) return hipSuccess;
   const uint32_t grid = std::max(1u, std::min<uint32_t>(4096, (n + 255) / 256));
-  hipLaunchKernelGGL(compact_records, dim3(grid), dim3(256), 0, s, hits, n, reinterpret_cast<const uint4*>(slabs),
-                     slab, offsets, reinterpret_cast<uint4*>(records), overflow_list, overflow_count);
+  hipLaunchKernelGGL(compact_records, dim3(grid), dim3(256), 0, s, counts, n, reinterpret_cast<const uint4*>(slabs),
+                     slab, offsets, reinterpret_cast<uint4*>(records), capacity, overflow_list, overflow_count);
   return hipGetLastError();
+}
+
+hipError_t launch_level_pass(const TraceParams& p, uint32_t mode, uint32_t grid, hipStream_t s) {
+  if (!p.lm_table || !p.lm_list || p.lm_cap < 2 || (p.lm_cap & (p.lm_cap - 1))) return hipErrorInvalidValue;
+  TraceFn f = mode == RTX_MODE_PLAIN ? trace_kernel<0, kRecNone, 8, 0, false, true>
+                                     : trace_kernel<1, kRecNone, 8, 0, false, true>;
+  return launch_fn(f, p, grid, 256, s);
 }
 
 hipError_t launch_trace_wide(const TraceParams& p, uint32_t mode, uint32_t cap, uint32_t grid, hipStream_t s) {
@@ -1775,11 +1868,12 @@ uint32_t trace_blocks_per_cu(uint32_t mode, int records, uint32_t lds_stack, uin
   return blocks_per_cu(pick_trace(mode, records, lds_stack, stage_bytes), trace_block_size(stage_bytes));
 }
 
-hipError_t launch_scan_records(const rtx_hit* hits, uint32_t n, uint64_t* offsets, uint64_t* tile_sums,
-                               hipStream_t s) {
+hipError_t launch_scan_records(const rtx_hit* hits, const uint32_t* counts, uint32_t n, uint64_t* offsets,
+                               uint64_t* tile_sums, hipStream_t s) {
+  static_assert(kScanItems == 4, "scan_tiles reads four counts as one uint4");
   const uint32_t tiles = (n + kScanTile - 1) / kScanTile;
   if (tiles > 0) {
-    hipLaunchKernelGGL(scan_tiles, dim3(tiles), dim3(kScanBlock), 0, s, hits, n, offsets, tile_sums);
+    hipLaunchKernelGGL(scan_tiles, dim3(tiles), dim3(kScanBlock), 0, s, hits, counts, n, offsets, tile_sums);
   }
   hipLaunchKernelGGL(scan_sums, dim3(1), dim3(kScanBlock), 0, s, tile_sums, tiles, offsets, n);
   if (tiles > 0) {

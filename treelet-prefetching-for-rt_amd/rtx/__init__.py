@@ -29,6 +29,7 @@ STATUS_NAMES = {
     4: "RTX_ERR_NODE_LARGER_THAN_BUDGET", 5: "RTX_ERR_TRACE_OVERFLOW", 6: "RTX_ERR_STACK_OVERFLOW",
     7: "RTX_ERR_HIP", 8: "RTX_ERR_NO_DEVICE", 9: "RTX_ERR_NO_SCENE", 10: "RTX_ERR_NO_PARTITION",
     11: "RTX_ERR_UNSUPPORTED", 12: "RTX_ERR_OUT_OF_MEMORY", 13: "RTX_ERR_TIMEOUT", 14: "RTX_ERR_STALE",
+    15: "RTX_ERR_CYCLE",
 }
 MODE_PLAIN, MODE_TREELET = 0, 1
 PREFETCH_NONE, PREFETCH_ALWAYS, PREFETCH_POPULARITY, PREFETCH_PARTIAL, PREFETCH_LEARNED = 0, 1, 2, 3, 4
