@@ -244,7 +244,12 @@ const char* rtx_version(void);
 rtx_status rtx_device_count(int* count);
 rtx_status rtx_create(int device, rtx_ctx** out);
 rtx_status rtx_destroy(rtx_ctx* ctx);
+/* Waits for the context's stream.  Returns the first error of the traces
+ * queued with rtx_trace_async since the previous wait (and clears it). */
 rtx_status rtx_synchronize(rtx_ctx* ctx);
+/* The context's hipStream_t, for callers that order their own work with it
+ * (hipStreamWaitEvent, hipEventRecord). */
+rtx_status rtx_get_stream(rtx_ctx* ctx, void** hip_stream);
 rtx_status rtx_malloc(rtx_ctx* ctx, size_t bytes, void** out_dev);
 rtx_status rtx_free(rtx_ctx* ctx, void* dev);
 rtx_status rtx_memcpy_h2d(rtx_ctx* ctx, void* dst_dev, const void* src_host, size_t bytes);
@@ -315,6 +320,15 @@ rtx_status rtx_trace(rtx_ctx* ctx, const rtx_ray* rays_dev, uint32_t n, const rt
                      rtx_hit* hits_dev, rtx_record* records_dev, uint64_t record_capacity,
                      uint64_t* record_offsets_dev);
 rtx_status rtx_get_prefetch_stats(rtx_ctx* ctx, rtx_prefetch_stats* out);
+
+/* rtx_trace without the wait, stream-ordered: the traversal, and the level
+ * pass a scene may need, are queued on the context's stream and the call
+ * returns.  Hits only, no staging (records or a prefetch policy:
+ * RTX_ERR_UNSUPPORTED).  Argument errors are returned at once, errors of the
+ * rays themselves by the next rtx_synchronize.  The rays and hits must stay
+ * allocated until then. */
+rtx_status rtx_trace_async(rtx_ctx* ctx, const rtx_ray* rays_dev, uint32_t n, const rtx_trace_options* opts,
+                           rtx_hit* hits_dev);
 
 /* Run-level statistics of a traced batch, the functional RT counters the
  * reference prints at the end of a kernel (src/gpgpu-sim/gpu-sim.cc:2168-2182;

@@ -334,3 +334,31 @@ def test_run_statistics_match_the_oracle(gpu, rtx, oracle, instanced, mode):
     leaves = [int((r_ref["type"] == t).sum()) for t in (4, 5, 6)]
     assert d["access_type"][:4] == s["access_type"][:4] and d["access_type"][4] == sum(leaves)
     assert d["from_records"] == 0 and d["accessed_bytes"] == s["accessed_bytes"]
+
+
+@pytest.mark.parametrize("mode", [0, 1])
+def test_async_traces_queue_and_report_at_the_wait(gpu, rtx, oracle, c1, mode):
+    """rtx_trace_async queues launches and returns; after rtx_synchronize every
+    queued batch holds the hits rtx_trace gives, and a ray error of one queued
+    launch is reported by that wait and not by the next."""
+    scene, rays = c1
+    gpu.set_scene(scene.arena, scene.tlas_offset)
+    if mode:
+        gpu.build_treelets(8192)
+    parts = [rays[:5000], rays[5000:], rays[::3], rays[:40]]  # the last one takes the eight-lane kernel
+    bufs = [(gpu.to_device(r), gpu.buffer(len(r) * rtx.HIT_DTYPE.itemsize)) for r in parts]
+    for r, (rb, hb) in zip(parts, bufs):
+        assert gpu.trace_async_device(rb, len(r), hb, mode=mode) == 0
+    assert gpu.wait() == 0
+    for k, (r, (rb, hb)) in enumerate(zip(parts, bufs)):
+        assert_hits_equal(hb.download(rtx.HIT_DTYPE, len(r)), gpu.trace(r, mode=mode), f"async batch {k}")
+    # a claim order naming a ray outside the batch: reported by the wait
+    bad = gpu.to_device(np.array([0, 1, len(parts[3]) + 7], dtype=np.uint32))
+    rb, hb = bufs[3]
+    assert gpu.trace_async_device(rb, 3, hb, mode=mode, claim_order=bad) == 0
+    assert gpu.trace_async_device(bufs[0][0], len(parts[0]), bufs[0][1], mode=mode) == 0
+    assert gpu.wait() == 3  # RTX_ERR_OFFSET_OUT_OF_ARENA
+    assert gpu.wait() == 0
+    # records and staging are not queued
+    o = rtx._TraceOptions(mode, 0, 0.0, 1, 0, 0, 0, 0, 0, 0.0, 0.0, 0, 0, 0, 0, None)
+    assert rtx._lib.rtx_trace_async(gpu._h, rb.ptr, 3, rtx.ctypes.byref(o), hb.ptr) == 11

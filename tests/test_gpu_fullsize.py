@@ -72,6 +72,31 @@ def test_diffuse_workload_full_batch(gpu, rtx, oracle):
     assert_hits_equal(hits2, hits, "learned staging over the full batch")
 
 
+@pytest.mark.parametrize("budget", [16384, 32768, 65536])
+def test_diffuse_workload_treelet_budgets(gpu, rtx, oracle, budget):
+    """The 1M-triangle scene's partition at the larger budgets equals the
+    oracle's (roots, lists, owner), and treelet order over a seeded sample of
+    the full batch is bit-exact in hits and records."""
+    import bench
+
+    d = bench.c2_desc(rtx)
+    scene = rtx.workload_scene(d)
+    gpu.set_scene(scene.arena, scene.tlas_offset)
+    prim = rtx.primary_rays(d)
+    bounce, _ = scene.secondary_rays(prim, gpu.trace(prim), kind=0, seed=99)
+    batch = np.concatenate([prim, bounce])
+    s = gpu.build_treelets(budget)
+    part = oracle.Partition(scene.arena, scene.tlas_offset, budget)
+    t = gpu.get_treelets()
+    assert s["n_treelets"] == part.n_treelets
+    for k in ("roots", "list_offsets", "list_nodes", "owner"):
+        assert np.array_equal(t[k], getattr(part, k)), k
+    print(f"budget {budget}: {part.n_treelets} treelets", flush=True)
+    hits = gpu.trace(batch, mode=1)
+    _props(batch, hits)
+    _sample_check(gpu, oracle, scene, batch, hits, 1, part, n=3000, seed=budget)
+
+
 def test_large_workload_sample(gpu, rtx, oracle):
     import bench
 

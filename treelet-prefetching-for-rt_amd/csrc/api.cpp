@@ -60,6 +60,10 @@ struct rtx_ctx {
   uint32_t* slots = nullptr;
   uint32_t slot = 0;
   bool slots_clean = true;
+  // rtx_trace_async: the error word its launches share (read and cleared by
+  // rtx_synchronize) and whether any was queued since
+  uint32_t* async_err = nullptr;
+  bool async_pending = false;
   uint32_t* err_host = nullptr;
   uint32_t* err_host_dev = nullptr;
   // workgroups per CU of each trace kernel, from the occupancy API on first use
@@ -315,6 +319,8 @@ rtx_status rtx_create(int device, rtx_ctx** out) {
   if (hipMemset(c->counters, 0, 64 * sizeof(uint32_t)) != hipSuccess) return fail(RTX_ERR_HIP);
   if (hipMalloc(&c->slots, kSlots * kSlotWords * sizeof(uint32_t)) != hipSuccess) return fail(RTX_ERR_OUT_OF_MEMORY);
   if (hipMemset(c->slots, 0, kSlots * kSlotWords * sizeof(uint32_t)) != hipSuccess) return fail(RTX_ERR_HIP);
+  if (hipMalloc(&c->async_err, 256) != hipSuccess) return fail(RTX_ERR_OUT_OF_MEMORY);
+  if (hipMemset(c->async_err, 0, 256) != hipSuccess) return fail(RTX_ERR_HIP);
   if (hipHostMalloc(&c->err_host, 64, hipHostMallocMapped | hipHostMallocCoherent) != hipSuccess)
     return fail(RTX_ERR_OUT_OF_MEMORY);
   if (hipHostGetDevicePointer(reinterpret_cast<void**>(&c->err_host_dev), c->err_host, 0) != hipSuccess)
@@ -342,6 +348,7 @@ rtx_status rtx_destroy(rtx_ctx* c) {
   if (c->spill) hipFree(c->spill);
   if (c->counters) hipFree(c->counters);
   if (c->slots) hipFree(c->slots);
+  if (c->async_err) hipFree(c->async_err);
   if (c->err_host) hipHostFree(c->err_host);
   if (c->scan_tmp) hipFree(c->scan_tmp);
   if (c->slabs) hipFree(c->slabs);
@@ -367,6 +374,18 @@ rtx_status rtx_synchronize(rtx_ctx* c) {
   if (!c) return RTX_ERR_INVALID_ARGUMENT;
   HIP_TRY(hipSetDevice(c->device));
   HIP_TRY(hipStreamSynchronize(c->stream));
+  if (!c->async_pending) return RTX_OK;
+  // the first error of the traces queued by rtx_trace_async since the last wait
+  c->async_pending = false;
+  uint32_t err = 0;
+  HIP_TRY(hipMemcpy(&err, c->async_err, sizeof(uint32_t), hipMemcpyDeviceToHost));
+  if (err) HIP_TRY(hipMemset(c->async_err, 0, sizeof(uint32_t)));
+  return kernel_error_status(err);
+}
+
+rtx_status rtx_get_stream(rtx_ctx* c, void** stream) {
+  if (!c || !stream) return RTX_ERR_INVALID_ARGUMENT;
+  *stream = reinterpret_cast<void*>(c->stream);
   return RTX_OK;
 }
 
@@ -570,9 +589,16 @@ rtx_status rtx_set_treelets(rtx_ctx* c, uint32_t budget, uint32_t n, const uint6
                            c->stream, c->part);
 }
 
-rtx_status rtx_trace(rtx_ctx* c, const rtx_ray* rays, uint32_t n, const rtx_trace_options* opts, rtx_hit* hits,
-                     rtx_record* records, uint64_t capacity, uint64_t* offsets) {
+}  // extern "C"
+
+namespace {
+
+// rtx_trace, and with `async` rtx_trace_async: the launches are queued and
+// the call returns without waiting (hits only, no staging).
+rtx_status trace_impl(rtx_ctx* c, const rtx_ray* rays, uint32_t n, const rtx_trace_options* opts, rtx_hit* hits,
+                      rtx_record* records, uint64_t capacity, uint64_t* offsets, bool async) {
   if (!c || !opts || (n > 0 && (!rays || !hits))) return RTX_ERR_INVALID_ARGUMENT;
+  if (async && (opts->emit_records || opts->prefetch != RTX_PREFETCH_NONE)) return RTX_ERR_UNSUPPORTED;
   if (opts->mode != RTX_MODE_PLAIN && opts->mode != RTX_MODE_TREELET) return RTX_ERR_INVALID_ARGUMENT;
   if (opts->prefetch > RTX_PREFETCH_LATTER) return RTX_ERR_INVALID_ARGUMENT;
   if (opts->emit_records && !offsets) return RTX_ERR_INVALID_ARGUMENT;
@@ -589,7 +615,7 @@ rtx_status rtx_trace(rtx_ctx* c, const rtx_ray* rays, uint32_t n, const rtx_trac
   memset(&c->last_stats, 0, sizeof(c->last_stats));
   if (n == 0) {
     if (opts->emit_records) HIP_TRY(hipMemsetAsync(offsets, 0, sizeof(uint64_t), c->stream));
-    HIP_TRY(hipStreamSynchronize(c->stream));
+    if (!async) HIP_TRY(hipStreamSynchronize(c->stream));
     return RTX_OK;
   }
   // 16 KiB of LDS stack per 256-lane block (8 entries per lane in plain order,
@@ -708,10 +734,10 @@ rtx_status rtx_trace(rtx_ctx* c, const rtx_ray* rays, uint32_t n, const rtx_trac
     uint32_t* cur = c->slots + (size_t)c->slot * kSlotWords;
     c->slot = (c->slot + 1) % kSlots;
     p.ray_counter = cur;
-    p.error = cur + 1;
+    p.error = async ? c->async_err : cur + 1;
     p.stats = reinterpret_cast<unsigned long long*>(cur + 16);
-    p.host_err = c->err_host_dev;
-    *herr = 0;
+    p.host_err = async ? nullptr : c->err_host_dev;
+    if (!async) *herr = 0;
     return RTX_OK;
   };
   auto launch_error = [&]() -> rtx_status {  // after a synchronize
@@ -793,9 +819,11 @@ rtx_status rtx_trace(rtx_ctx* c, const rtx_ray* rays, uint32_t n, const rtx_trac
   if (!c->levels_consistent) {
     // a node is reachable at two levels: the rays again, keeping the
     // reference's tree_level_map, for max_level
-    HIP_TRY(hipStreamSynchronize(c->stream));
-    st = launch_error();
-    if (st != RTX_OK) return st;
+    if (!async) {
+      HIP_TRY(hipStreamSynchronize(c->stream));
+      st = launch_error();
+      if (st != RTX_OK) return st;
+    }
     st = ensure_level_mem(c);
     if (st != RTX_OK) return st;
     TraceParams lp = scene_params;
@@ -818,6 +846,10 @@ rtx_status rtx_trace(rtx_ctx* c, const rtx_ray* rays, uint32_t n, const rtx_trac
     lp.stats = p.stats;
     HIP_TRY(launch_level_pass(lp, opts->mode, lm_grid, c->stream));
   }
+  if (async) {
+    c->async_pending = true;
+    return RTX_OK;
+  }
   HIP_TRY(hipStreamSynchronize(c->stream));
   c->slots_clean = true;
   c->last_stats.node_reads = stats[0];
@@ -833,6 +865,20 @@ rtx_status rtx_trace(rtx_ctx* c, const rtx_ray* rays, uint32_t n, const rtx_trac
   c->last_stats.too_late_bytes = stats[10];
   c->last_stats.late_bytes = 0;
   return launch_error();
+}
+
+}  // namespace
+
+extern "C" {
+
+rtx_status rtx_trace(rtx_ctx* c, const rtx_ray* rays, uint32_t n, const rtx_trace_options* opts, rtx_hit* hits,
+                     rtx_record* records, uint64_t capacity, uint64_t* offsets) {
+  return trace_impl(c, rays, n, opts, hits, records, capacity, offsets, false);
+}
+
+rtx_status rtx_trace_async(rtx_ctx* c, const rtx_ray* rays, uint32_t n, const rtx_trace_options* opts,
+                           rtx_hit* hits) {
+  return trace_impl(c, rays, n, opts, hits, nullptr, 0, nullptr, true);
 }
 
 // rtx_trace without allocation: hits only, the library's static device

@@ -153,6 +153,8 @@ _sig("rtx_get_treelets", _i32, _vp, _vp, _vp, _vp, _vp)
 _sig("rtx_set_treelets", _i32, _vp, _u32, _u32, _vp, _vp, _vp, _vp)
 _sig("rtx_trace", _i32, _vp, _vp, _u32, _P(_TraceOptions), _vp, _vp, _u64, _vp)
 _sig("rtx_trace_static", _i32, _vp, _vp, _u32, _P(_TraceOptions), _vp)
+_sig("rtx_trace_async", _i32, _vp, _vp, _u32, _P(_TraceOptions), _vp)
+_sig("rtx_get_stream", _i32, _vp, _P(_vp))
 _sig("rtx_build_treelet_layout", _i32, _vp, _u32, _P(_LayoutSummary))
 _sig("rtx_coherence_order", _i32, _vp, _vp, _u32, _P(_CoherenceConfig), _vp, _vp)
 _sig("rtx_trace_stats_of", _i32, _vp, _vp, _u32, _vp, _u64, _P(_TraceStats))
@@ -209,7 +211,7 @@ EXPORTED_SYMBOLS = [
     "rtx_status_string", "rtx_version", "rtx_device_count", "rtx_create", "rtx_destroy", "rtx_synchronize",
     "rtx_malloc", "rtx_free", "rtx_memcpy_h2d", "rtx_memcpy_d2h", "rtx_memcpy_d2d", "rtx_memset", "rtx_timer_start",
     "rtx_timer_stop", "rtx_device_info", "rtx_set_scene", "rtx_set_scene_device", "rtx_validate_scene",
-    "rtx_build_treelets", "rtx_get_treelets", "rtx_set_treelets", "rtx_build_treelet_layout", "rtx_get_treelet_layout", "rtx_trace", "rtx_trace_static", "rtx_coherence_order", "rtx_trace_stats_of", "rtx_get_prefetch_stats",
+    "rtx_build_treelets", "rtx_get_treelets", "rtx_set_treelets", "rtx_build_treelet_layout", "rtx_get_treelet_layout", "rtx_trace", "rtx_trace_static", "rtx_trace_async", "rtx_get_stream", "rtx_coherence_order", "rtx_trace_stats_of", "rtx_get_prefetch_stats",
     "rtx_last_kernel_ms", "rtx_add_scene", "rtx_build_scene_treelets", "rtx_trace_batch", "rtx_batch_graph_create",
     "rtx_batch_graph_launch", "rtx_batch_graph_destroy", "rtx_autotune_batch", "rtx_autotune_batch_treelets",
     "rtx_set_prefetch_table", "rtx_mlp_create", "rtx_mlp_destroy", "rtx_mlp_param_count", "rtx_mlp_get_params",
@@ -807,3 +809,19 @@ class Context:
 
     def synchronize(self):
         _check(_lib.rtx_synchronize(self._h), "rtx_synchronize")
+
+    def wait(self):
+        """rtx_synchronize; returns the rtx_status (the first error of queued async traces)."""
+        return _lib.rtx_synchronize(self._h)
+
+    def trace_async_device(self, rays_buf, n, hits_buf, mode=MODE_PLAIN, waves_per_cu=0, lds_stack=0, refill=0,
+                           lanes_per_ray=0, claim_start=0, claim_order=None):
+        """rtx_trace_async (hits only, queued) on device buffers; returns the rtx_status of the call."""
+        o = _TraceOptions(mode, PREFETCH_NONE, 0.0, 0, waves_per_cu, lds_stack, 0, refill, 0, 0.0, 0.0, lanes_per_ray,
+                          claim_start, 0, 0, claim_order.ptr if claim_order is not None else None)
+        return _lib.rtx_trace_async(self._h, rays_buf.ptr, n, ctypes.byref(o), hits_buf.ptr)
+
+    def stream(self):
+        st = ctypes.c_void_p()
+        _check(_lib.rtx_get_stream(self._h, ctypes.byref(st)), "rtx_get_stream")
+        return st.value
