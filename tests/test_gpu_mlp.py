@@ -52,9 +52,13 @@ def mo_sample_dtype():
     return rtx.MLP_SAMPLE_DTYPE
 
 
-def test_loss_and_gradients_match_oracle(gpu):
-    T, C, n = 40, 96, 300
-    rng = np.random.default_rng(5)
+@pytest.mark.parametrize("T,C,n", [(40, 96, 300), (60, 2500, 700), (20, 17, 33)])
+def test_loss_and_gradients_match_oracle(gpu, T, C, n):
+    """Loss and every gradient block against the oracle.  The output layer is
+    fused (logits only in registers): 2,500 classes span three row-statistics
+    chunks and sixteen class splits of the dQ pass; 17 classes and 33 rows
+    leave most of every tile empty."""
+    rng = np.random.default_rng(5 + C)
     mlp = gpu.mlp(T, C, seed=9)
     try:
         vec = mlp.params()
