@@ -185,12 +185,12 @@ typedef struct rtx_trace_options {
                              -remap_to_treelet_layout makes the reference record remapped addresses
                              (vulkan_ray_tracing.cc:2446-2450, 2509-2512); RTX_NO_NODE for a node
                              placed nowhere.  Needs the layout (else RTX_ERR_NO_PARTITION) */
-  uint32_t record_slab;  /* records, one traversal: each ray's first record_slab records go to a
-                            per-ray slab in library memory and are compacted into the caller's CSR
-                            buffer; only rays with more records are traced a second time.
-                            0 = 256, halved until n * record_slab * 16 bytes fit 32 GiB (at least
-                            16); when they do not fit, rtx_trace runs a counting pass and a
-                            writing pass instead */
+  uint32_t record_slab;  /* records, one traversal: each ray's first record_slab records (at most
+                            1024; 0 = 256) go to a per-ray slab in library memory and are compacted
+                            into the caller's CSR buffer; only rays with more records are traced a
+                            second time.  The slabs take at most 32 GiB: a batch that needs more is
+                            traced in consecutive ray ranges that fit; with a claim_order or staging
+                            it takes a counting pass and a writing pass instead */
   const uint32_t* claim_order; /* device array of n ray indices or NULL: claim k traces ray
                                   claim_order[k] (outputs stay at input indices), e.g. the order of
                                   rtx_coherence_order; claim_start must then be 0.  An index >= n is

@@ -75,9 +75,11 @@ def test_slab_records_capacity_and_staging(gpu, rtx, oracle, c1):
     assert_records_equal(o, r, o_ref, r_ref, "staged, slab 12")
 
 
-def test_two_pass_records_when_slabs_exceed_the_limit(gpu, rtx, oracle, c1):
-    """2,359,296 rays with 1,024-record slabs would need 36 GiB of slabs: rtx_trace
-    falls back to a counting pass and a writing pass, with the same records."""
+def test_records_past_the_slab_limit(gpu, rtx, oracle, c1):
+    """2,359,296 rays with 1,024-record slabs would need 36 GiB of slabs, past
+    the 32 GiB cap: rtx_trace traces them in two ray ranges that fit (one
+    traversal), or, with a claim order, in a counting pass and a writing pass.
+    Both give the records of the default 256-record slabs (one range)."""
     rtx_slab_limit = 32 << 30
     assert 2048 * 1152 * 1024 * 16 > rtx_slab_limit
     scene, _ = c1
@@ -85,13 +87,29 @@ def test_two_pass_records_when_slabs_exceed_the_limit(gpu, rtx, oracle, c1):
 
     d = c1_desc(rtx, 2048, 1152)
     rays = rtx.primary_rays(d)
-    assert len(rays) == 2048 * 1152
+    n = len(rays)
+    assert n == 2048 * 1152
     gpu.set_scene(scene.arena, scene.tlas_offset)
-    h2, o2, r2 = gpu.trace(rays, mode=0, records=True, record_slab=1024)
     h1, o1, r1 = gpu.trace(rays, mode=0, records=True)
-    assert_hits_equal(h2, h1, "two-pass vs slab")
-    assert_records_equal(o2, r2, o1, r1, "two-pass vs slab")
-    sample = np.arange(0, len(rays), 293)
+    # two ranges, the claim starting inside the second
+    h2, o2, r2 = gpu.trace(rays, mode=0, records=True, record_slab=1024, claim_start=n - 1000)
+    assert_hits_equal(h2, h1, "ranges vs one slab")
+    assert_records_equal(o2, r2, o1, r1, "ranges vs one slab")
+    # a claim order: counting pass and writing pass
+    order = gpu.to_device(np.arange(n, dtype=np.uint32)[::-1].copy())
+    rb, hb, ob = gpu.to_device(rays), gpu.buffer(n * rtx.HIT_DTYPE.itemsize), gpu.buffer((n + 1) * 8)
+    total = int(o1[-1])
+    recb = gpu.buffer(total * rtx.RECORD_DTYPE.itemsize)
+    assert gpu.trace_device(rb, n, hb, records_buf=recb, record_capacity=total, offsets_buf=ob, record_slab=1024,
+                            claim_order=order) == 0
+    assert_hits_equal(hb.download(rtx.HIT_DTYPE, n), h1, "two-pass vs one slab")
+    assert_records_equal(ob.download(np.uint64, n + 1), recb.download(rtx.RECORD_DTYPE, total), o1, r1,
+                         "two-pass vs one slab")
+    # a capacity one short past the first range: reported, offsets filled
+    assert gpu.trace_device(rb, n, hb, records_buf=recb, record_capacity=total - 1, offsets_buf=ob,
+                            record_slab=1024) == 5
+    assert np.array_equal(ob.download(np.uint64, n + 1), o1)
+    sample = np.arange(0, n, 293)
     h_ref, o_ref, r_ref = oracle.trace(scene.arena, scene.tlas_offset, rays[sample], mode=0)
     assert_hits_equal(h1[sample], h_ref, "slab vs oracle sample")
     got = np.concatenate([r1[o1[i]:o1[i + 1]] for i in sample])

@@ -647,17 +647,18 @@ rtx_status trace_impl(rtx_ctx* c, const rtx_ray* rays, uint32_t n, const rtx_tra
   if (opts->lanes_per_ray != 0 && opts->lanes_per_ray != 1 && opts->lanes_per_ray != kWideRayLanes)
     return RTX_ERR_INVALID_ARGUMENT;
   const bool wide_ok = !opts->emit_records && stage == 0;
-  // records in one traversal: per-ray slabs of `slab` records, compacted into
-  // the caller's CSR buffer; only rays with more records are traced again.
-  // The default slab is 256 records (a 1M-triangle diffuse batch has at most
-  // about 210 per ray), halved until the slabs fit 32 GiB; below 16 records a
-  // counting pass and a writing pass replace them.
-  uint32_t slab = opts->record_slab;
-  if (slab == 0) {
-    slab = 256;
-    while (slab > 16 && (uint64_t)n * slab * sizeof(rtx_record) > kSlabBytesMax) slab /= 2;
-  }
-  const bool use_slab = opts->emit_records && (uint64_t)n * slab * sizeof(rtx_record) <= kSlabBytesMax;
+  // records in one traversal: per-ray slabs of `slab` records (default 256: a
+  // 1M-triangle diffuse batch has at most about 210 per ray), compacted into the
+  // caller's CSR buffer; only rays with more records are traced again.  The
+  // slabs take at most 32 GiB: a larger batch is traced in consecutive ray
+  // ranges that fit, each compacted before the next reuses the slabs.  With a
+  // claim order or staging such a batch takes a counting pass and a writing
+  // pass instead.
+  const uint32_t slab = opts->record_slab ? opts->record_slab : 256;
+  const uint64_t chunk_rays = std::max<uint64_t>(1, kSlabBytesMax / ((uint64_t)slab * sizeof(rtx_record)));
+  const bool use_slab = opts->emit_records && ((uint64_t)n <= chunk_rays || (!opts->claim_order && stage == 0));
+  const uint32_t per_chunk = (uint32_t)std::min<uint64_t>(n, chunk_rays);
+  const uint32_t n_chunks = use_slab ? (n + per_chunk - 1) / per_chunk : 1;
   const int main_rec = !opts->emit_records ? kRecNone : use_slab ? kRecSlab : kRecNone;
   if (opts->lanes_per_ray == kWideRayLanes && !wide_ok) return RTX_ERR_INVALID_ARGUMENT;
   const bool wide = wide_ok && (opts->lanes_per_ray == kWideRayLanes ||
@@ -748,45 +749,76 @@ rtx_status trace_impl(rtx_ctx* c, const rtx_ray* rays, uint32_t n, const rtx_tra
   };
   c->kernel_timed = false;
   if (use_slab) {
-    st = ensure_buffer(c->slabs, c->slab_entries, (size_t)n * slab);
+    st = ensure_buffer(c->slabs, c->slab_entries, (size_t)per_chunk * slab);
     if (st != RTX_OK) return st;
     st = ensure_buffer(c->rec_counts, c->rec_count_entries, (size_t)n);
+    if (st != RTX_OK) return st;
+    st = ensure_buffer(c->overflow_list, c->overflow_entries, (size_t)n);
     if (st != RTX_OK) return st;
     p.records = c->slabs;
     p.record_slab = slab;
     p.rec_counts = c->rec_counts;
   }
+  // scan tiles of the largest scanned range, and one word for a range's base offset
+  const uint32_t scan_n = use_slab ? per_chunk : n;
+  if (opts->emit_records) {
+    st = ensure_scan_tmp(c, (size_t)(scan_n + 1023) / 1024 + 2);
+    if (st != RTX_OK) return st;
+  }
+  uint64_t* const base_word = opts->emit_records ? c->scan_tmp + (scan_n + 1023) / 1024 + 1 : nullptr;
   st = use_slot();
   if (st != RTX_OK) return st;
   uint32_t* main_slot = p.ray_counter;
   const TraceParams scene_params = p;  // for the level pass
+  // One ray range of a slab traversal (all of it when it fits): traced, its
+  // counts scanned into offsets[b..e] (plus the previous ranges' total), its
+  // slabs compacted.  The ranges share the control slot: its ray counter is
+  // cleared between them, its error word and word 2 (the rays whose records
+  // overflowed their slab) collect all of them.
+  auto slab_range = [&](uint32_t b, uint32_t e, bool last) -> rtx_status {
+    TraceParams q = p;
+    q.rays = rays + b;
+    q.hits = hits + b;
+    q.n = q.n_rays = e - b;
+    q.claim_start = opts->claim_start >= b && opts->claim_start < e ? opts->claim_start - b : 0;
+    q.rec_counts = c->rec_counts + b;
+    if (b > 0) HIP_TRY(launch_fill_zero(main_slot, 1, c->stream));
+    const uint32_t gq = std::max(1u, (uint32_t)std::min<uint64_t>(grid, ((uint64_t)(e - b) + bs - 1) / bs));
+    HIP_TRY(launch_trace(q, opts->mode, kRecSlab, lds_stack, stage, gq, c->stream));
+    if (last) HIP_TRY(hipEventRecord(c->evk1, c->stream));
+    if (b == 0)
+      HIP_TRY(launch_fill_zero(reinterpret_cast<uint32_t*>(base_word), 2, c->stream));
+    else
+      HIP_TRY(hipMemcpyAsync(base_word, offsets + b, sizeof(uint64_t), hipMemcpyDeviceToDevice, c->stream));
+    HIP_TRY(launch_scan_records(nullptr, c->rec_counts + b, e - b, offsets + b, c->scan_tmp, c->stream));
+    if (b > 0) HIP_TRY(launch_add_offset(offsets + b, (uint64_t)(e - b) + 1, base_word, c->stream));
+    // the kernel copies nothing when the running total exceeds the capacity
+    HIP_TRY(launch_compact_records(c->rec_counts + b, e - b, c->slabs, slab, offsets + b, records,
+                                   records ? capacity : 0, b, c->overflow_list, main_slot + 2, c->stream));
+    return RTX_OK;
+  };
+  // the timed span: the traversal launch (from the first range's to the end of the last range's)
   HIP_TRY(hipEventRecord(c->evk0, c->stream));
-  if (wide)
+  if (wide) {
     HIP_TRY(launch_trace_wide(p, opts->mode, lds_stack + spill_cap, grid, c->stream));
-  else
+    HIP_TRY(hipEventRecord(c->evk1, c->stream));
+  } else if (use_slab) {
+    for (uint32_t k = 0; k < n_chunks; ++k) {
+      st = slab_range(k * per_chunk, (uint32_t)std::min<uint64_t>(n, (uint64_t)(k + 1) * per_chunk), k + 1 == n_chunks);
+      if (st != RTX_OK) return st;
+    }
+  } else {
     HIP_TRY(launch_trace(p, opts->mode, main_rec, lds_stack, stage, grid, c->stream));
-  HIP_TRY(hipEventRecord(c->evk1, c->stream));
+    HIP_TRY(hipEventRecord(c->evk1, c->stream));
+  }
   c->kernel_timed = true;
   // staging statistics of the main traversal (a second pass traces some rays again)
   unsigned long long stats[11] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
   if (stage) HIP_TRY(hipMemcpyAsync(stats, p.stats, sizeof(stats), hipMemcpyDeviceToHost, c->stream));
   if (opts->emit_records) {
-    const size_t tiles = (n + 1023) / 1024;
-    st = ensure_scan_tmp(c, tiles + 1);
-    if (st != RTX_OK) return st;
-    HIP_TRY(launch_scan_records(hits, use_slab ? c->rec_counts : nullptr, n, offsets, c->scan_tmp, c->stream));
+    if (!use_slab) HIP_TRY(launch_scan_records(hits, nullptr, n, offsets, c->scan_tmp, c->stream));
     uint32_t second = n;  // rays the writing pass traces
-    if (use_slab) {
-      // slab records to their CSR places, queued behind the scan (the kernel
-      // copies nothing when the total exceeds the capacity); word 2 of the
-      // main launch's control slot (cleared with it) counts the rays whose
-      // records overflowed their slab
-      st = ensure_buffer(c->overflow_list, c->overflow_entries, (size_t)n);
-      if (st != RTX_OK) return st;
-      HIP_TRY(launch_compact_records(c->rec_counts, n, c->slabs, slab, offsets, records, records ? capacity : 0,
-                                     c->overflow_list, main_slot + 2, c->stream));
-      HIP_TRY(hipMemcpyAsync(&second, main_slot + 2, sizeof(uint32_t), hipMemcpyDeviceToHost, c->stream));
-    }
+    if (use_slab) HIP_TRY(hipMemcpyAsync(&second, main_slot + 2, sizeof(uint32_t), hipMemcpyDeviceToHost, c->stream));
     uint64_t total = 0;
     HIP_TRY(hipMemcpyAsync(&total, offsets + n, sizeof(uint64_t), hipMemcpyDeviceToHost, c->stream));
     HIP_TRY(hipStreamSynchronize(c->stream));

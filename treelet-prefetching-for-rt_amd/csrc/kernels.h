@@ -148,11 +148,13 @@ hipError_t launch_trace_stats(const rtx_hit* hits, uint32_t n, const rtx_record*
 hipError_t coherence_order(const rtx_ray* rays, uint32_t n, const rtx_coherence_config& cfg, const float wmin[3],
                            const float wmax[3], uint32_t* order, uint32_t* hashes_out, hipStream_t s);
 // Copy each ray's slab records (at most `slab`) to its CSR position; rays with
-// more than `slab` records are appended to overflow_list (count in
-// overflow_count, which must start at 0).  Nothing is copied when the total
-// (offsets[n]) exceeds `capacity`.
+// more than `slab` records are appended to overflow_list as ray_base + their
+// index (count in overflow_count, which must start at 0).  Nothing is copied
+// when the total (offsets[n]) exceeds `capacity`.
 hipError_t launch_compact_records(const uint32_t* counts, uint32_t n, const rtx_record* slabs, uint32_t slab,
-                                  const uint64_t* offsets, rtx_record* records, uint64_t capacity,
+                                  const uint64_t* offsets, rtx_record* records, uint64_t capacity, uint32_t ray_base,
                                   uint32_t* overflow_list, uint32_t* overflow_count, hipStream_t s);
+// v[0..count) += *addend (a device word).
+hipError_t launch_add_offset(uint64_t* v, uint64_t count, const uint64_t* addend, hipStream_t s);
 
 }  // namespace rtxk

@@ -1657,7 +1657,7 @@ TraceFn pick_trace(uint32_t mode, int records, uint32_t lds_stack, uint32_t stag
 // slab are left to the second pass, which lists the ray.
 __global__ __launch_bounds__(256) void compact_records(const uint32_t* counts, uint32_t n, const uint4* slabs,
                                                        uint32_t slab, const uint64_t* offsets, uint4* records,
-                                                       uint64_t capacity, uint32_t* overflow_list,
+                                                       uint64_t capacity, uint32_t ray_base, uint32_t* overflow_list,
                                                        uint32_t* overflow_count) {
   if (offsets[n] > capacity) return;  // the caller reports RTX_ERR_TRACE_OVERFLOW
   const uint32_t lane = threadIdx.x & 63u;
@@ -1669,7 +1669,7 @@ __global__ __launch_bounds__(256) void compact_records(const uint32_t* counts, u
     uint32_t cnt = 0;
     if (r < n) {
       cnt = counts[r];
-      if (cnt > slab) overflow_list[atomicAdd(overflow_count, 1u)] = r;
+      if (cnt > slab) overflow_list[atomicAdd(overflow_count, 1u)] = ray_base + r;
     }
     uint32_t incl = cnt;
 #pragma unroll
@@ -1730,6 +1730,12 @@ uint32_t blocks_per_cu(TraceFn f, uint32_t bs) {
                 hipSuccess)
     return 0;
   return nb > 0 ? (uint32_t)nb : 0u;
+}
+
+__global__ void add_offset_kernel(uint64_t* v, uint64_t count, const uint64_t* addend) {
+  const uint64_t a = *addend;
+  for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < count; i += (uint64_t)gridDim.x * blockDim.x)
+    v[i] += a;
 }
 
 __global__ void fill_zero_kernel(uint32_t* dst, uint32_t words) {
@@ -1826,12 +1832,20 @@ hipError_t launch_trace_stats(const rtx_hit* hits, uint32_t n, const rtx_record*
 }
 
 hipError_t launch_compact_records(const uint32_t* counts, uint32_t n, const rtx_record* slabs, uint32_t slab,
-                                  const uint64_t* offsets, rtx_record* records, uint64_t capacity,
+                                  const uint64_t* offsets, rtx_record* records, uint64_t capacity, uint32_t ray_base,
                                   uint32_t* overflow_list, uint32_t* overflow_count, hipStream_t s) {
   if (n == 0 || slab == 0) return hipSuccess;
   const uint32_t grid = std::max(1u, std::min<uint32_t>(4096, (n + 255) / 256));
   hipLaunchKernelGGL(compact_records, dim3(grid), dim3(256), 0, s, counts, n, reinterpret_cast<const uint4*>(slabs),
-                     slab, offsets, reinterpret_cast<uint4*>(records), capacity, overflow_list, overflow_count);
+                     slab, offsets, reinterpret_cast<uint4*>(records), capacity, ray_base, overflow_list,
+                     overflow_count);
+  return hipGetLastError();
+}
+
+hipError_t launch_add_offset(uint64_t* v, uint64_t count, const uint64_t* addend, hipStream_t s) {
+  if (count == 0) return hipSuccess;
+  const uint32_t grid = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(4096, (count + 255) / 256));
+  hipLaunchKernelGGL(add_offset_kernel, dim3(grid), dim3(256), 0, s, v, count, addend);
   return hipGetLastError();
 }
 
