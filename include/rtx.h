@@ -188,9 +188,10 @@ typedef struct rtx_trace_options {
   uint32_t record_slab;  /* records, one traversal: each ray's first record_slab records (at most
                             1024; 0 = 256) go to a per-ray slab in library memory and are compacted
                             into the caller's CSR buffer; only rays with more records are traced a
-                            second time.  The slabs take at most 32 GiB: a batch that needs more is
-                            traced in consecutive ray ranges that fit; with a claim_order or staging
-                            it takes a counting pass and a writing pass instead */
+                            second time.  The slabs (8 bytes per record) take at most 16 GiB: a
+                            batch that needs more is traced in consecutive ray ranges that fit; with
+                            a claim_order or staging it takes a counting pass and a writing pass
+                            instead */
   const uint32_t* claim_order; /* device array of n ray indices or NULL: claim k traces ray
                                   claim_order[k] (outputs stay at input indices), e.g. the order of
                                   rtx_coherence_order; claim_start must then be 0.  An index >= n is

@@ -76,12 +76,12 @@ def test_slab_records_capacity_and_staging(gpu, rtx, oracle, c1):
 
 
 def test_records_past_the_slab_limit(gpu, rtx, oracle, c1):
-    """2,359,296 rays with 1,024-record slabs would need 36 GiB of slabs, past
-    the 32 GiB cap: rtx_trace traces them in two ray ranges that fit (one
+    """2,359,296 rays with 1,024-record slabs would need 18 GiB of slabs (8
+    bytes per record), past the 16 GiB cap: rtx_trace traces them in two ray ranges that fit (one
     traversal), or, with a claim order, in a counting pass and a writing pass.
     Both give the records of the default 256-record slabs (one range)."""
-    rtx_slab_limit = 32 << 30
-    assert 2048 * 1152 * 1024 * 16 > rtx_slab_limit
+    rtx_slab_limit = 16 << 30
+    assert 2048 * 1152 * 1024 * 8 > rtx_slab_limit
     scene, _ = c1
     from conftest import c1_desc
 

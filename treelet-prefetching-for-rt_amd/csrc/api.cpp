@@ -23,8 +23,8 @@ using namespace rtxk;
 
 // bytes of the control block cleared before and read after a traversal
 static constexpr uint32_t kSlots = 64;
-// most device memory the per-ray record slabs may take
-static constexpr uint64_t kSlabBytesMax = 32ull << 30;
+// most device memory the per-ray record slabs (8 bytes per record) may take
+static constexpr uint64_t kSlabBytesMax = 16ull << 30;
 
 struct rtx_ctx {
   int device = 0;
@@ -72,7 +72,7 @@ struct rtx_ctx {
   size_t scan_tmp_entries = 0;
   // slab records (one traversal with records): per-ray slabs and the list of
   // rays whose records did not fit
-  rtx_record* slabs = nullptr;
+  uint64_t* slabs = nullptr;
   size_t slab_entries = 0;
   uint32_t* overflow_list = nullptr;
   size_t overflow_entries = 0;
@@ -650,12 +650,12 @@ rtx_status trace_impl(rtx_ctx* c, const rtx_ray* rays, uint32_t n, const rtx_tra
   // records in one traversal: per-ray slabs of `slab` records (default 256: a
   // 1M-triangle diffuse batch has at most about 210 per ray), compacted into the
   // caller's CSR buffer; only rays with more records are traced again.  The
-  // slabs take at most 32 GiB: a larger batch is traced in consecutive ray
+  // slabs take at most 16 GiB: a larger batch is traced in consecutive ray
   // ranges that fit, each compacted before the next reuses the slabs.  With a
   // claim order or staging such a batch takes a counting pass and a writing
   // pass instead.
   const uint32_t slab = opts->record_slab ? opts->record_slab : 256;
-  const uint64_t chunk_rays = std::max<uint64_t>(1, kSlabBytesMax / ((uint64_t)slab * sizeof(rtx_record)));
+  const uint64_t chunk_rays = std::max<uint64_t>(1, kSlabBytesMax / ((uint64_t)slab * sizeof(uint64_t)));
   const bool use_slab = opts->emit_records && ((uint64_t)n <= chunk_rays || (!opts->claim_order && stage == 0));
   const uint32_t per_chunk = (uint32_t)std::min<uint64_t>(n, chunk_rays);
   const uint32_t n_chunks = use_slab ? (n + per_chunk - 1) / per_chunk : 1;
@@ -755,7 +755,7 @@ rtx_status trace_impl(rtx_ctx* c, const rtx_ray* rays, uint32_t n, const rtx_tra
     if (st != RTX_OK) return st;
     st = ensure_buffer(c->overflow_list, c->overflow_entries, (size_t)n);
     if (st != RTX_OK) return st;
-    p.records = c->slabs;
+    p.slabs = c->slabs;
     p.record_slab = slab;
     p.rec_counts = c->rec_counts;
   }
@@ -793,8 +793,8 @@ rtx_status trace_impl(rtx_ctx* c, const rtx_ray* rays, uint32_t n, const rtx_tra
     HIP_TRY(launch_scan_records(nullptr, c->rec_counts + b, e - b, offsets + b, c->scan_tmp, c->stream));
     if (b > 0) HIP_TRY(launch_add_offset(offsets + b, (uint64_t)(e - b) + 1, base_word, c->stream));
     // the kernel copies nothing when the running total exceeds the capacity
-    HIP_TRY(launch_compact_records(c->rec_counts + b, e - b, c->slabs, slab, offsets + b, records,
-                                   records ? capacity : 0, b, c->overflow_list, main_slot + 2, c->stream));
+    HIP_TRY(launch_compact_records(c->rec_counts + b, e - b, c->slabs, slab, p.owner, p.layout_remap, offsets + b,
+                                   records, records ? capacity : 0, b, c->overflow_list, main_slot + 2, c->stream));
     return RTX_OK;
   };
   // the timed span: the traversal launch (from the first range's to the end of the last range's)
@@ -864,6 +864,7 @@ rtx_status trace_impl(rtx_ctx* c, const rtx_ray* rays, uint32_t n, const rtx_tra
     if (st != RTX_OK) return st;
     lp.spill = c->spill;
     lp.records = nullptr;
+    lp.slabs = nullptr;
     lp.record_slab = 0;
     lp.rec_counts = nullptr;
     lp.prefetch = RTX_PREFETCH_NONE;

@@ -56,7 +56,10 @@ struct TraceParams {
   rtx_record* records;
   uint64_t record_capacity;
   const uint64_t* record_offsets;
-  uint32_t record_slab;    // slab records: ray r's first record_slab records at records[r * record_slab]
+  // slab records: ray r's first record_slab records as 8-byte slab entries at
+  // slabs[r * record_slab] (`records` unused)
+  uint64_t* slabs;
+  uint32_t record_slab;
   uint32_t* rec_counts;    // slab records: ray r's record count at rec_counts[r]
   // level pass: per lane a tree_level_map of lm_cap u64 entries (zero between
   // rays) at lm_table + lane * lm_cap, and lm_cap / 2 positions at lm_list
@@ -147,13 +150,21 @@ hipError_t launch_trace_stats(const rtx_hit* hits, uint32_t n, const rtx_record*
 // The coherence engine's claim order of n rays (coherence.hip).
 hipError_t coherence_order(const rtx_ray* rays, uint32_t n, const rtx_coherence_config& cfg, const float wmin[3],
                            const float wmax[3], uint32_t* order, uint32_t* hashes_out, hipStream_t s);
-// Copy each ray's slab records (at most `slab`) to its CSR position; rays with
+// A slab entry: the node's arena slot (offset / 64) in the low word, the
+// record type and level in the high word.  The size follows from the type;
+// the treelet and the layout offset are looked up when the entry is expanded.
+__host__ __device__ inline uint64_t slab_entry(uint32_t slot, uint32_t type, uint32_t level) {
+  return (uint64_t)slot | ((uint64_t)(type | ((level > 255u ? 255u : level) << 8)) << 32);
+}
+// Expand each ray's slab entries (at most `slab`) into rtx_records at its CSR
+// position (owner and layout_remap nullable, as in TraceParams); rays with
 // more than `slab` records are appended to overflow_list as ray_base + their
 // index (count in overflow_count, which must start at 0).  Nothing is copied
 // when the total (offsets[n]) exceeds `capacity`.
-hipError_t launch_compact_records(const uint32_t* counts, uint32_t n, const rtx_record* slabs, uint32_t slab,
-                                  const uint64_t* offsets, rtx_record* records, uint64_t capacity, uint32_t ray_base,
-                                  uint32_t* overflow_list, uint32_t* overflow_count, hipStream_t s);
+hipError_t launch_compact_records(const uint32_t* counts, uint32_t n, const uint64_t* slabs, uint32_t slab,
+                                  const uint32_t* owner, const uint64_t* layout_remap, const uint64_t* offsets,
+                                  rtx_record* records, uint64_t capacity, uint32_t ray_base, uint32_t* overflow_list,
+                                  uint32_t* overflow_count, hipStream_t s);
 // v[0..count) += *addend (a device word).
 hipError_t launch_add_offset(uint64_t* v, uint64_t count, const uint64_t* addend, hipStream_t s);
 

@@ -261,12 +261,17 @@ void trace_kernel(TraceParams p) {
 
   auto emit = [&](uint64_t off, uint32_t size, uint32_t type, uint32_t lvl) {
     if (type != RTX_TX_BVH_PRIMITIVE_LEAF_DESCRIPTOR) fetch_bytes += size;
-    if (REC) {
-      // a slab's base follows from the ray index (no register held for it)
-      uint64_t idx = (REC == kRecSlab ? (uint64_t)local_id * p.record_slab : rec_base) + n_rec;
-      // a slab keeps the ray's first record_slab records; the rest are written
-      // by a second pass over the rays that had more
-      if (REC == kRecSlab ? n_rec < p.record_slab : idx < p.record_capacity) {
+    if constexpr (REC == kRecSlab) {
+      // a slab's base follows from the ray index (no register held for it); a
+      // slab keeps the ray's first record_slab records, the rest are written
+      // by a second pass over the rays that had more.  The entry holds what
+      // the traversal knows; the owner and layout lookups wait for the
+      // compaction, off the traversal's dependent chain.
+      if (n_rec < p.record_slab)
+        p.slabs[(uint64_t)local_id * p.record_slab + n_rec] = slab_entry((uint32_t)(off >> 6), type, lvl);
+    } else if constexpr (REC == kRecCsr) {
+      const uint64_t idx = rec_base + n_rec;
+      if (idx < p.record_capacity) {
         uint32_t tl = owner ? owner[off >> 6] : RTX_NO_TREELET;
         const uint64_t roff = p.layout_remap ? p.layout_remap[off >> 6] : off;
         uint4 v;
@@ -275,7 +280,7 @@ void trace_kernel(TraceParams p) {
         v.z = tl;
         v.w = size | (type << 16) | ((lvl > 255u ? 255u : lvl) << 24);
         reinterpret_cast<uint4*>(p.records)[idx] = v;
-      } else if (REC == kRecCsr) {
+      } else {
         err |= kErrRecords;
       }
     }
@@ -1655,10 +1660,11 @@ TraceFn pick_trace(uint32_t mode, int records, uint32_t lds_stack, uint32_t stag
 // wave's prefixes with shuffles).  Loads and stores are 16 bytes, the stores
 // contiguous, and each lane has four loads in flight.  Records past a ray's
 // slab are left to the second pass, which lists the ray.
-__global__ __launch_bounds__(256) void compact_records(const uint32_t* counts, uint32_t n, const uint4* slabs,
-                                                       uint32_t slab, const uint64_t* offsets, uint4* records,
-                                                       uint64_t capacity, uint32_t ray_base, uint32_t* overflow_list,
-                                                       uint32_t* overflow_count) {
+__global__ __launch_bounds__(256) void compact_records(const uint32_t* counts, uint32_t n, const uint64_t* slabs,
+                                                       uint32_t slab, const uint32_t* owner,
+                                                       const uint64_t* layout_remap, const uint64_t* offsets,
+                                                       uint4* records, uint64_t capacity, uint32_t ray_base,
+                                                       uint32_t* overflow_list, uint32_t* overflow_count) {
   if (offsets[n] > capacity) return;  // the caller reports RTX_ERR_TRACE_OVERFLOW
   const uint32_t lane = threadIdx.x & 63u;
   const uint32_t wave = blockIdx.x * 4 + (threadIdx.x >> 6);
@@ -1681,7 +1687,7 @@ __global__ __launch_bounds__(256) void compact_records(const uint32_t* counts, u
     const uint64_t out = offsets[base];
     // wave-uniform loop: every lane takes part in the shuffles
     for (uint32_t f0 = 0; f0 < total; f0 += 64 * kUnroll) {
-      uint4 v[kUnroll];
+      uint64_t v[kUnroll];
       uint32_t fs[kUnroll];
       bool ok[kUnroll];
 #pragma unroll
@@ -1700,9 +1706,25 @@ __global__ __launch_bounds__(256) void compact_records(const uint32_t* counts, u
         ok[u] = f < total && k < slab;
         if (ok[u]) v[u] = slabs[(uint64_t)(base + lo) * slab + k];
       }
+      uint32_t tl[kUnroll];
+      uint64_t roff[kUnroll];
 #pragma unroll
-      for (uint32_t u = 0; u < kUnroll; ++u)
-        if (ok[u]) records[out + fs[u]] = v[u];
+      for (uint32_t u = 0; u < kUnroll; ++u) {
+        if (!ok[u]) continue;
+        const uint32_t slot = (uint32_t)v[u];
+        tl[u] = owner ? owner[slot] : RTX_NO_TREELET;
+        roff[u] = layout_remap ? layout_remap[slot] : (uint64_t)slot << 6;
+      }
+#pragma unroll
+      for (uint32_t u = 0; u < kUnroll; ++u) {
+        if (!ok[u]) continue;
+        const uint32_t hw = (uint32_t)(v[u] >> 32);
+        const uint32_t type = hw & 0xffu;
+        const uint32_t size = type == RTX_TX_BVH_INSTANCE_LEAF ? 128u
+                              : type == RTX_TX_BVH_PRIMITIVE_LEAF_DESCRIPTOR ? 8u : 64u;
+        records[out + fs[u]] = make_uint4((uint32_t)roff[u], (uint32_t)(roff[u] >> 32), tl[u],
+                                          size | (type << 16) | ((hw >> 8) << 24));
+      }
     }
   }
 }
@@ -1831,14 +1853,14 @@ hipError_t launch_trace_stats(const rtx_hit* hits, uint32_t n, const rtx_record*
   return hipGetLastError();
 }
 
-hipError_t launch_compact_records(const uint32_t* counts, uint32_t n, const rtx_record* slabs, uint32_t slab,
-                                  const uint64_t* offsets, rtx_record* records, uint64_t capacity, uint32_t ray_base,
-                                  uint32_t* overflow_list, uint32_t* overflow_count, hipStream_t s) {
+hipError_t launch_compact_records(const uint32_t* counts, uint32_t n, const uint64_t* slabs, uint32_t slab,
+                                  const uint32_t* owner, const uint64_t* layout_remap, const uint64_t* offsets,
+                                  rtx_record* records, uint64_t capacity, uint32_t ray_base, uint32_t* overflow_list,
+                                  uint32_t* overflow_count, hipStream_t s) {
   if (n == 0 || slab == 0) return hipSuccess;
   const uint32_t grid = std::max(1u, std::min<uint32_t>(4096, (n + 255) / 256));
-  hipLaunchKernelGGL(compact_records, dim3(grid), dim3(256), 0, s, counts, n, reinterpret_cast<const uint4*>(slabs),
-                     slab, offsets, reinterpret_cast<uint4*>(records), capacity, ray_base, overflow_list,
-                     overflow_count);
+  hipLaunchKernelGGL(compact_records, dim3(grid), dim3(256), 0, s, counts, n, slabs, slab, owner, layout_remap, offsets,
+                     reinterpret_cast<uint4*>(records), capacity, ray_base, overflow_list, overflow_count);
   return hipGetLastError();
 }
 
