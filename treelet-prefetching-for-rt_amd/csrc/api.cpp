@@ -655,7 +655,7 @@ rtx_status trace_impl(rtx_ctx* c, const rtx_ray* rays, uint32_t n, const rtx_tra
   // claim order or staging such a batch takes a counting pass and a writing
   // pass instead.
   const uint32_t slab = opts->record_slab ? opts->record_slab : 256;
-  const uint64_t chunk_rays = std::max<uint64_t>(1, kSlabBytesMax / ((uint64_t)slab * sizeof(uint64_t)));
+  const uint64_t chunk_rays = std::max<uint64_t>(1, kSlabBytesMax / ((uint64_t)slab_stride(slab) * sizeof(uint64_t)));
   const bool use_slab = opts->emit_records && ((uint64_t)n <= chunk_rays || (!opts->claim_order && stage == 0));
   const uint32_t per_chunk = (uint32_t)std::min<uint64_t>(n, chunk_rays);
   const uint32_t n_chunks = use_slab ? (n + per_chunk - 1) / per_chunk : 1;
@@ -749,7 +749,7 @@ rtx_status trace_impl(rtx_ctx* c, const rtx_ray* rays, uint32_t n, const rtx_tra
   };
   c->kernel_timed = false;
   if (use_slab) {
-    st = ensure_buffer(c->slabs, c->slab_entries, (size_t)per_chunk * slab);
+    st = ensure_buffer(c->slabs, c->slab_entries, (size_t)per_chunk * slab_stride(slab));
     if (st != RTX_OK) return st;
     st = ensure_buffer(c->rec_counts, c->rec_count_entries, (size_t)n);
     if (st != RTX_OK) return st;

@@ -23,7 +23,7 @@ enum : uint32_t {
 enum : int {
   kRecNone = 0,  // hits only
   kRecCsr = 1,   // records at record_offsets[ray]
-  kRecSlab = 2   // a ray's first record_slab records at ray * record_slab
+  kRecSlab = 2   // a ray's first record_slab records at ray * slab_stride(record_slab)
 };
 
 constexpr uint32_t kBlock = 256;        // threads per block (4 waves of 64)
@@ -57,7 +57,7 @@ struct TraceParams {
   uint64_t record_capacity;
   const uint64_t* record_offsets;
   // slab records: ray r's first record_slab records as 8-byte slab entries at
-  // slabs[r * record_slab] (`records` unused)
+  // slabs[r * slab_stride(record_slab)] (`records` unused)
   uint64_t* slabs;
   uint32_t record_slab;
   uint32_t* rec_counts;    // slab records: ray r's record count at rec_counts[r]
@@ -150,6 +150,11 @@ hipError_t launch_trace_stats(const rtx_hit* hits, uint32_t n, const rtx_record*
 // The coherence engine's claim order of n rays (coherence.hip).
 hipError_t coherence_order(const rtx_ray* rays, uint32_t n, const rtx_coherence_config& cfg, const float wmin[3],
                            const float wmax[3], uint32_t* order, uint32_t* hashes_out, hipStream_t s);
+// Slab records: each lane holds its last kRecRing entries in LDS and writes
+// them to the slab in aligned groups of four (32 bytes), so slab rows are
+// padded to a multiple of four entries.
+constexpr uint32_t kRecRing = 8;
+__host__ __device__ inline uint32_t slab_stride(uint32_t record_slab) { return (record_slab + 3u) & ~3u; }
 // A slab entry: the node's arena slot (offset / 64) in the low word, the
 // record type and level in the high word.  The size follows from the type;
 // the treelet and the layout offset are looked up when the entry is expanded.

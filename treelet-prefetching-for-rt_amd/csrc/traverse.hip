@@ -187,11 +187,15 @@ void trace_kernel(TraceParams p) {
   // direct-mapped table (a collision forgets the older slot)
   constexpr uint32_t EV = STAGE ? 128 : 1;
   __shared__ uint32_t ev_key[EV], dm_key[EV];
+  // slab records: each lane's last kRecRing slab entries (entry k at ring
+  // position k % kRecRing), written to the slab four at a time
+  __shared__ uint64_t rec_ring[REC == kRecSlab ? kRecRing * BS : 1];
 
   const uint32_t tid = threadIdx.x;
   const uint32_t lane = tid & 63u;
   lds_u64* ldsA = (lds_u64*)(lds_stack + tid);
   lds_u64* ldsB = (lds_u64*)(lds_stack + (NS - 1) * K * BS + tid);
+  lds_u64* ring = (lds_u64*)(rec_ring + (REC == kRecSlab ? tid : 0u));
   const uint64_t lane_slot = (uint64_t)blockIdx.x * BS + tid;
   glb_u64* gA = (glb_u64*)(p.spill + lane_slot * (uint64_t)p.spill_cap * NS);
   glb_u64* gB = gA + (NS - 1) * (uint64_t)p.spill_cap;
@@ -226,6 +230,7 @@ void trace_kernel(TraceParams p) {
   bool ray_ok = false, cur_ok = false;
   uint32_t n_rec = 0, n_nodes = 0, fetch_bytes = 0, n_any = 0, max_level = 0;
   uint64_t rec_base = 0;
+  uint32_t rec_flushed = 0;  // slab records: entries of the ray already in its slab
   bool phase2 = false;
 
   // The entry a lane popped this iteration and processes below.
@@ -262,13 +267,12 @@ void trace_kernel(TraceParams p) {
   auto emit = [&](uint64_t off, uint32_t size, uint32_t type, uint32_t lvl) {
     if (type != RTX_TX_BVH_PRIMITIVE_LEAF_DESCRIPTOR) fetch_bytes += size;
     if constexpr (REC == kRecSlab) {
-      // a slab's base follows from the ray index (no register held for it); a
-      // slab keeps the ray's first record_slab records, the rest are written
+      // a slab keeps the ray's first record_slab records, the rest are written
       // by a second pass over the rays that had more.  The entry holds what
       // the traversal knows; the owner and layout lookups wait for the
-      // compaction, off the traversal's dependent chain.
-      if (n_rec < p.record_slab)
-        p.slabs[(uint64_t)local_id * p.record_slab + n_rec] = slab_entry((uint32_t)(off >> 6), type, lvl);
+      // compaction, off the traversal's dependent chain.  It goes to the
+      // lane's LDS ring first (flush_slab_group writes it out).
+      if (n_rec < p.record_slab) ring[(n_rec & (kRecRing - 1)) * BS] = slab_entry((uint32_t)(off >> 6), type, lvl);
     } else if constexpr (REC == kRecCsr) {
       const uint64_t idx = rec_base + n_rec;
       if (idx < p.record_capacity) {
@@ -285,6 +289,34 @@ void trace_kernel(TraceParams p) {
       }
     }
     n_rec++;
+  };
+
+  // Slab records: entries [rec_flushed, rec_flushed + 4) of the lane's ray go
+  // from the LDS ring to the slab once all four exist, as two 16-byte stores
+  // that fill one aligned 32-byte sector (8-byte stores scattered over the
+  // slabs cost partial-sector writes and fills).  Called at the top of every
+  // iteration, which adds at most three entries, so at most six are ever
+  // waiting and the ring of eight never overwrites one.
+  auto flush_slab_group = [&]() {
+    if constexpr (REC == kRecSlab) {
+      if (rec_flushed + 4u <= min(n_rec, p.record_slab)) {
+        const uint32_t q = rec_flushed & (kRecRing - 1);
+        const uint64_t a0 = ring[q * BS], a1 = ring[(q + 1) * BS], a2 = ring[(q + 2) * BS], a3 = ring[(q + 3) * BS];
+        uint4* dst = reinterpret_cast<uint4*>(p.slabs + (uint64_t)local_id * slab_stride(p.record_slab) + rec_flushed);
+        dst[0] = make_uint4((uint32_t)a0, (uint32_t)(a0 >> 32), (uint32_t)a1, (uint32_t)(a1 >> 32));
+        dst[1] = make_uint4((uint32_t)a2, (uint32_t)(a2 >> 32), (uint32_t)a3, (uint32_t)(a3 >> 32));
+        rec_flushed += 4u;
+      }
+    }
+  };
+  // the entries still in the ring when the ray's hit is written
+  auto flush_slab_rest = [&]() {
+    if constexpr (REC == kRecSlab) {
+      const uint32_t end = min(n_rec, p.record_slab);
+      uint64_t* row = p.slabs + (uint64_t)local_id * slab_stride(p.record_slab);
+      for (uint32_t k = rec_flushed; k < end; ++k) row[k] = ring[(k & (kRecRing - 1)) * BS];
+      rec_flushed = end;
+    }
   };
 
   auto push_entry = [&](uint64_t e, bool to_a) {
@@ -567,6 +599,7 @@ void trace_kernel(TraceParams p) {
       }
       return;
     }
+    flush_slab_rest();
     uint4 o0 = make_uint4(0, 0, 0, 0), o1 = make_uint4(0, 0, 0, 0), o2 = make_uint4(0, 0, 0, 0);
     uint4 o4 = make_uint4(0xffffffffu, 0xffffffffu, 0, 0);
     if (err == 0 && min_thit < rtmax && closest_leaf != kNoSlot) {
@@ -617,6 +650,7 @@ void trace_kernel(TraceParams p) {
     tl_h1 = hq[1];
   }
   while (true) {
+    flush_slab_group();
     // ---- refill idle lanes from the global ray counter (wave converged) ----
     const uint64_t idle = __ballot(!active);
     const uint32_t nidle = (uint32_t)__popcll(idle);
@@ -676,6 +710,7 @@ void trace_kernel(TraceParams p) {
           closest_inst = kNoSlot;
           cur_inst = kNoSlot;
           n_rec = n_nodes = fetch_bytes = n_any = 0;
+          rec_flushed = 0;
           max_level = 1;  // tree_level_map[topRootAddr] = 1
           if constexpr (REC == kRecCsr) rec_base = p.record_offsets[r];
           sa = Stack{0, 0, 0};
@@ -1685,33 +1720,39 @@ __global__ __launch_bounds__(256) void compact_records(const uint32_t* counts, u
     }
     const uint32_t total = __shfl(incl, 63, 64);
     const uint64_t out = offsets[base];
-    // wave-uniform loop: every lane takes part in the shuffles
+    // wave-uniform loop: every lane takes part in the shuffles.  The kUnroll
+    // searches run interleaved and every load is issued unconditionally (an
+    // index past the end reads entry 0 and is not stored), so the slab loads
+    // and then the owner loads of a pass are in flight together.
+    const uint32_t stride = slab_stride(slab);
     for (uint32_t f0 = 0; f0 < total; f0 += 64 * kUnroll) {
+      uint32_t lo[kUnroll];
+#pragma unroll
+      for (uint32_t u = 0; u < kUnroll; ++u) lo[u] = 0;
+      // first ray i with prefix(i) > f
+#pragma unroll
+      for (uint32_t step = 32; step > 0; step >>= 1) {
+#pragma unroll
+        for (uint32_t u = 0; u < kUnroll; ++u) {
+          const uint32_t pre = __shfl(incl, (int)(lo[u] + step - 1), 64);
+          if (pre <= f0 + u * 64 + lane) lo[u] += step;
+        }
+      }
       uint64_t v[kUnroll];
-      uint32_t fs[kUnroll];
       bool ok[kUnroll];
 #pragma unroll
       for (uint32_t u = 0; u < kUnroll; ++u) {
         const uint32_t f = f0 + u * 64 + lane;
-        // first ray i with prefix(i) > f
-        uint32_t lo = 0;
-#pragma unroll
-        for (uint32_t step = 32; step > 0; step >>= 1) {
-          const uint32_t pre = __shfl(incl, (int)(lo + step - 1), 64);
-          if (pre <= f) lo += step;
-        }
-        const uint32_t before = __shfl(incl, (int)(lo ? lo - 1 : 0), 64);
-        const uint32_t k = f - (lo ? before : 0u);
-        fs[u] = f;
+        const uint32_t before = __shfl(incl, (int)(lo[u] ? lo[u] - 1 : 0), 64);
+        const uint32_t k = f - (lo[u] ? before : 0u);
         ok[u] = f < total && k < slab;
-        if (ok[u]) v[u] = slabs[(uint64_t)(base + lo) * slab + k];
+        v[u] = slabs[ok[u] ? (uint64_t)(base + lo[u]) * stride + k : 0u];
       }
       uint32_t tl[kUnroll];
       uint64_t roff[kUnroll];
 #pragma unroll
       for (uint32_t u = 0; u < kUnroll; ++u) {
-        if (!ok[u]) continue;
-        const uint32_t slot = (uint32_t)v[u];
+        const uint32_t slot = ok[u] ? (uint32_t)v[u] : 0u;
         tl[u] = owner ? owner[slot] : RTX_NO_TREELET;
         roff[u] = layout_remap ? layout_remap[slot] : (uint64_t)slot << 6;
       }
@@ -1722,8 +1763,8 @@ __global__ __launch_bounds__(256) void compact_records(const uint32_t* counts, u
         const uint32_t type = hw & 0xffu;
         const uint32_t size = type == RTX_TX_BVH_INSTANCE_LEAF ? 128u
                               : type == RTX_TX_BVH_PRIMITIVE_LEAF_DESCRIPTOR ? 8u : 64u;
-        records[out + fs[u]] = make_uint4((uint32_t)roff[u], (uint32_t)(roff[u] >> 32), tl[u],
-                                          size | (type << 16) | ((hw >> 8) << 24));
+        records[out + f0 + u * 64 + lane] = make_uint4((uint32_t)roff[u], (uint32_t)(roff[u] >> 32), tl[u],
+                                                       size | (type << 16) | ((hw >> 8) << 24));
       }
     }
   }
