@@ -23,6 +23,9 @@ using namespace rtxk;
 
 // bytes of the control block cleared before and read after a traversal
 static constexpr uint32_t kSlots = 64;
+// words of the mapped host block (err_host, 16 words): word 0 is the ray-error
+// flag, words kReportWord.. receive launch_report_words
+static constexpr uint32_t kReportWord = 8;
 // most device memory the per-ray record slabs (8 bytes per record) may take
 static constexpr uint64_t kSlabBytesMax = 16ull << 30;
 
@@ -817,13 +820,16 @@ rtx_status trace_impl(rtx_ctx* c, const rtx_ray* rays, uint32_t n, const rtx_tra
   if (stage) HIP_TRY(hipMemcpyAsync(stats, p.stats, sizeof(stats), hipMemcpyDeviceToHost, c->stream));
   if (opts->emit_records) {
     if (!use_slab) HIP_TRY(launch_scan_records(hits, nullptr, n, offsets, c->scan_tmp, c->stream));
-    uint32_t second = n;  // rays the writing pass traces
-    if (use_slab) HIP_TRY(hipMemcpyAsync(&second, main_slot + 2, sizeof(uint32_t), hipMemcpyDeviceToHost, c->stream));
-    uint64_t total = 0;
-    HIP_TRY(hipMemcpyAsync(&total, offsets + n, sizeof(uint64_t), hipMemcpyDeviceToHost, c->stream));
+    // the rays the writing pass traces (slab overflows) and the records'
+    // total, through mapped host words
+    HIP_TRY(launch_report_words(use_slab ? main_slot + 2 : nullptr, offsets + n, c->err_host_dev + kReportWord,
+                                c->stream));
     HIP_TRY(hipStreamSynchronize(c->stream));
     st = launch_error();
     if (st != RTX_OK) return st;
+    const volatile uint32_t* rep = c->err_host + kReportWord;
+    const uint32_t second = use_slab ? rep[0] : n;
+    const uint64_t total = (uint64_t)rep[2] | ((uint64_t)rep[3] << 32);
     if (total > capacity || (total > 0 && !records)) {
       c->slots_clean = true;
       return RTX_ERR_TRACE_OVERFLOW;

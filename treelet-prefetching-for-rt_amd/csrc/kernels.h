@@ -138,6 +138,9 @@ constexpr uint64_t level_pass_lane_bytes(uint32_t lm_cap) { return (uint64_t)lm_
 constexpr uint32_t kLevelMapCap = 4096;  // map entries per lane: at most 2,048 nodes assigned per ray
 // Clears `words` 32-bit words with a kernel (usable inside graph capture).
 hipError_t launch_fill_zero(uint32_t* dst, uint32_t words, hipStream_t s);
+// host[0] = *word, host[2..3] = *dword (host: mapped host memory), so that the
+// caller reads both after a stream synchronisation without a copy each.
+hipError_t launch_report_words(const uint32_t* word, const uint64_t* dword, uint32_t* host, hipStream_t s);
 // CSR offsets[0..n] of the rays' record counts: hits[i].n_records, or
 // counts[i] when `counts` is given.
 hipError_t launch_scan_records(const rtx_hit* hits, const uint32_t* counts, uint32_t n, uint64_t* offsets,

@@ -1695,6 +1695,16 @@ TraceFn pick_trace(uint32_t mode, int records, uint32_t lds_stack, uint32_t stag
 // wave's prefixes with shuffles).  Loads and stores are 16 bytes, the stores
 // contiguous, and each lane has four loads in flight.  Records past a ray's
 // slab are left to the second pass, which lists the ray.
+__global__ void report_words_kernel(const uint32_t* word, const uint64_t* dword, uint32_t* host) {
+  if (threadIdx.x == 0) {
+    volatile uint32_t* h = host;
+    const uint64_t d = *dword;
+    h[0] = word ? *word : 0u;
+    h[2] = (uint32_t)d;
+    h[3] = (uint32_t)(d >> 32);
+  }
+}
+
 __global__ __launch_bounds__(256) void compact_records(const uint32_t* counts, uint32_t n, const uint64_t* slabs,
                                                        uint32_t slab, const uint32_t* owner,
                                                        const uint64_t* layout_remap, const uint64_t* offsets,
@@ -1815,6 +1825,11 @@ hipError_t launch_trace_batch(const TraceParams& p, uint32_t mode, uint32_t lds_
 
 uint32_t trace_batch_blocks_per_cu(uint32_t mode, uint32_t lds_stack) {
   return blocks_per_cu(pick_batch(mode, lds_stack), 256);
+}
+
+hipError_t launch_report_words(const uint32_t* word, const uint64_t* dword, uint32_t* host, hipStream_t s) {
+  hipLaunchKernelGGL(report_words_kernel, dim3(1), dim3(64), 0, s, word, dword, host);
+  return hipGetLastError();
 }
 
 hipError_t launch_fill_zero(uint32_t* dst, uint32_t words, hipStream_t s) {
