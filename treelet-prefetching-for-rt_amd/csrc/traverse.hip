@@ -1092,12 +1092,64 @@ void trace_kernel(TraceParams p) {
               const uint32_t lm_child = LM ? lm_get(slot) + 1 : 0u;
               const uint32_t inst = top ? 0u : e_inst_rel(e);
               const uint32_t base_flags = top ? kFlTop : 0u;
+              bool pushed = false;
+              if constexpr (MODE == 0 && !LM && K >= 8) {
+                // Plain order without the level map, for a node whose children
+                // lie in the arena and (under the TLAS) are internal nodes or
+                // instance leaves: the pushes below give the same stack as one
+                // push per child.  The ring's oldest entries are spilled first,
+                // as many as pushing one by one would spill (up to six new
+                // entries never spill each other when K >= 8), then each hit
+                // child is stored at its place: children in slot order, the
+                // first internal one last.
+                const uint32_t iw0 = __builtin_amdgcn_alignbyte(nd.w[6], nd.w[5], 2);  // info bytes of children 0-3
+                const uint32_t iw1 = nd.w[6] >> 16;                                    // children 4-5
+                uint32_t pre[6], tot = 0, leafm = 0;
+#pragma unroll
+                for (int i = 0; i < 6; ++i) {
+                  const uint32_t info = (i < 4 ? iw0 >> (8 * i) : iw1 >> (8 * (i - 4))) & 0x3fu;
+                  pre[i] = tot;
+                  tot += info & 3u;
+                  leafm |= (info >> 2) != 0u ? 1u << i : 0u;
+                }
+                const uint32_t n_push = (uint32_t)__builtin_popcount(hitmask);
+                const uint32_t need = sa.cnt + n_push;
+                const uint32_t sp = need > (uint32_t)K ? need - (uint32_t)K : 0u;
+                const bool ok = first_child < bytes && first_child + (uint64_t)tot * 64 <= bytes &&
+                                !(top && ((iw0 & 0x38383838u) | (iw1 & 0x3838u)) != 0u) && sa.g + sp <= p.spill_cap;
+                if (ok) {
+                  pushed = true;
+                  for (uint32_t j = 0; j < sp; ++j) {
+                    gA[sa.g++] = ldsA[sa.bot * BS];
+                    sa.bot = (sa.bot + 1) & (K - 1);
+                    sa.cnt--;
+                  }
+                  const uint32_t im = hitmask & ~leafm;
+                  const uint32_t dbit = im & (0u - im);  // the first internal hit child
+                  const uint32_t pm = hitmask & ~dbit;
+                  const uint32_t first_slot = (uint32_t)(first_child >> 6);
+                  const uint32_t hi = base_flags | ((lvl > 255u ? 255u : lvl) << 2) | ((inst & 0x3fffffu) << 10);
+                  const uint32_t top_pos = sa.bot + sa.cnt;
+                  uint32_t dslot = 0;
+#pragma unroll
+                  for (int i = 0; i < 6; ++i) {
+                    if ((dbit >> i) & 1u) dslot = first_slot + pre[i];
+                    if ((pm >> i) & 1u) {
+                      const uint32_t r = (uint32_t)__builtin_popcount(pm & ((1u << i) - 1u));
+                      ldsA[((top_pos + r) & (K - 1)) * BS] =
+                          (uint64_t)(first_slot + pre[i]) | ((uint64_t)(hi | (((leafm >> i) & 1u) ? kFlLeaf : 0u)) << 32);
+                    }
+                  }
+                  if (dbit) ldsA[((top_pos + (uint32_t)__builtin_popcount(pm)) & (K - 1)) * BS] = (uint64_t)dslot | ((uint64_t)hi << 32);
+                  sa.cnt += n_push;
+                }
+              }
               int deferred = -1;
               uint32_t deferred_cs = 0;
               const uint32_t cur_bits = fbits;
               uint64_t child = first_child;
 #pragma unroll
-              for (int i = 0; i < 6; ++i) {
+              for (int i = 0; i < 6 && !pushed; ++i) {
                 const uint32_t info = byte_of(nd, 22 + i) & 0x3fu;
                 const uint64_t c = child;
                 child += (uint64_t)(info & 3u) * 64;
@@ -1118,7 +1170,7 @@ void trace_kernel(TraceParams p) {
                 }
                 push_entry(make_entry(cs, base_flags | (leaf ? kFlLeaf : 0u), lvl, inst), MODE == 0 || ((cur_bits >> i) & 1u));
               }
-              if (MODE == 0 && deferred >= 0) {
+              if (MODE == 0 && deferred >= 0 && !pushed) {
                 push_entry(make_entry(deferred_cs, base_flags, lvl, inst), true);
               }
             }
