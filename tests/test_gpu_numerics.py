@@ -142,6 +142,40 @@ def test_ray_intervals(gpu, rtx, oracle, c1):
     _check_all(gpu, oracle, scene, r, "ray intervals")
 
 
+def test_cull_distances_at_float_boundaries(gpu, rtx, oracle, c1):
+    """The closest-hit cull (a child box is skipped when its entry distance is
+    at or beyond the closest hit so far, vulkan_ray_tracing.cc:2537, 2725) at
+    the edges of float order: tmax equal to the ray's own hit distance (the
+    ground quad's leaf box is entered exactly there), one float either side,
+    signed zeros, denormals, -FLT_MAX, -inf, FLT_MAX, +inf, with tmin 0, -1 or
+    -inf and opaque and non-opaque rays."""
+    scene, rays = c1
+    hall = oracle.trace(scene.arena, scene.tlas_offset, rays, mode=0, records=False)
+    pick = np.concatenate([np.flatnonzero(hall["hit"] != 0)[::4][:1536], np.flatnonzero(hall["hit"] == 0)[:512]])
+    base = rays[pick].copy()
+    h0 = hall[pick]
+    hit = h0["hit"] != 0
+    t = np.where(hit, h0["t"], np.float32(10.0)).astype(np.float32)
+    f32 = np.float32
+    tiny = np.finfo(np.float32).smallest_subnormal
+    big = np.finfo(np.float32).max
+    fixed = np.array([-np.inf, -big, -1.0, -0.0, 0.0, tiny, -tiny, big, np.inf], np.float32)
+    groups = []
+    for k, tm in enumerate((t, np.nextafter(t, f32(np.inf)), np.nextafter(t, f32(-np.inf)))):
+        r = base.copy()
+        r["tmax"] = tm
+        r["tmin"] = np.array([0.0, -1.0, -np.inf], np.float32)[(np.arange(len(r)) + k) % 3]
+        groups.append(r)
+    r = base.copy()
+    r["tmax"] = fixed[np.arange(len(r)) % len(fixed)]
+    r["tmin"] = np.where(np.arange(len(r)) % 2 == 0, f32(-np.inf), f32(0.0))
+    groups.append(r)
+    rr = np.concatenate(groups)
+    rr["flags"] = np.where(np.arange(len(rr)) % 3 == 0, 0, 1)  # some rays non-opaque
+    h = _check_all(gpu, oracle, scene, rr, "cull boundaries")
+    assert h["hit"].sum() > 500
+
+
 def _overflowing_tlas_arena():
     """The hand-assembled tiny arena with the TLAS root's x origin at 3e38 and
     x exponent 127: its child box decodes to x in [3e38, +inf] (q 128 * 2^119

@@ -1026,7 +1026,8 @@ void trace_kernel(TraceParams p) {
             // v_min/v_max forms differ from the reference's compare-selects
             // only in the sign of a zero, which only enters comparisons.
             // Otherwise the reference's compare-selects run.
-            const bool fast = (top ? ray_ok : cur_ok) && !__builtin_isnan(f_of(nd, 0) + f_of(nd, 1) + f_of(nd, 2));
+            const bool fast = (top ? ray_ok : cur_ok) && !__builtin_isnan(f_of(nd, 0) + f_of(nd, 1) + f_of(nd, 2)) &&
+                              cull != -__builtin_inff();
             auto test_children = [&](auto fast_tag) {
               constexpr bool FAST = decltype(fast_tag)::value;
               if constexpr (FAST) {
@@ -1045,24 +1046,36 @@ void trace_kernel(TraceParams p) {
                   fw[a][1] = neg ? l1 : u1;
                 }
                 const float ox = f_of(nd, 0), oy = f_of(nd, 1), oz = f_of(nd, 2);
+                // The cull test folded into the far bound: for a non-NaN tn
+                // and a cull above -inf, !(tn >= cull) is tn < cull, which is
+                // tn <= the largest float below cull; a NaN cull culls
+                // nothing.  (A cull of -inf takes the compare-select path.)
+                float tfc = tmax;
+                if (!__builtin_isnan(cull)) {
+                  const uint32_t cu = __float_as_uint(cull);
+                  const uint32_t below = (int32_t)cu > 0 ? cu - 1u : ((cu << 1) == 0u ? 0x80000001u : cu + 1u);
+                  tfc = __builtin_fminf(tmax, __uint_as_float(below));
+                }
+                // every child slot is tested (no branch per child) and the
+                // empty slots are masked out: with a finite ray and a non-NaN
+                // node origin an empty slot's bytes give finite or infinite
+                // distances, never a NaN, and its result is discarded
 #pragma unroll
                 for (int i = 0; i < 6; ++i) {
                   const uint32_t info = byte_of(nd, 22 + i) & 0x3fu;
-                  if ((info & 3u) > 0) {
-                    const int wi = i >> 2, sh = 8 * (i & 3);
-                    const float nx = __builtin_fmaf((float)((nw[0][wi] >> sh) & 0xffu), sc.x, ox);
-                    const float ny = __builtin_fmaf((float)((nw[1][wi] >> sh) & 0xffu), sc.y, oy);
-                    const float nz = __builtin_fmaf((float)((nw[2][wi] >> sh) & 0xffu), sc.z, oz);
-                    const float fx = __builtin_fmaf((float)((fw[0][wi] >> sh) & 0xffu), sc.x, ox);
-                    const float fy = __builtin_fmaf((float)((fw[1][wi] >> sh) & 0xffu), sc.y, oy);
-                    const float fz = __builtin_fmaf((float)((fw[2][wi] >> sh) & 0xffu), sc.z, oz);
-                    const float tn = __builtin_fmaxf(__builtin_fmaxf(__builtin_fmaxf((nx - o.x) * idir.x, (ny - o.y) * idir.y),
-                                                                     (nz - o.z) * idir.z), tmin);
-                    const float tf = __builtin_fminf(__builtin_fminf(__builtin_fminf((fx - o.x) * idir.x, (fy - o.y) * idir.y),
-                                                                     (fz - o.z) * idir.z), tmax);
-                    const bool h = tn <= tf && !(tn >= cull);
-                    if (h) hitmask |= 1u << i;
-                  }
+                  const int wi = i >> 2, sh = 8 * (i & 3);
+                  const float nx = __builtin_fmaf((float)((nw[0][wi] >> sh) & 0xffu), sc.x, ox);
+                  const float ny = __builtin_fmaf((float)((nw[1][wi] >> sh) & 0xffu), sc.y, oy);
+                  const float nz = __builtin_fmaf((float)((nw[2][wi] >> sh) & 0xffu), sc.z, oz);
+                  const float fx = __builtin_fmaf((float)((fw[0][wi] >> sh) & 0xffu), sc.x, ox);
+                  const float fy = __builtin_fmaf((float)((fw[1][wi] >> sh) & 0xffu), sc.y, oy);
+                  const float fz = __builtin_fmaf((float)((fw[2][wi] >> sh) & 0xffu), sc.z, oz);
+                  const float tn = __builtin_fmaxf(__builtin_fmaxf(__builtin_fmaxf((nx - o.x) * idir.x, (ny - o.y) * idir.y),
+                                                                   (nz - o.z) * idir.z), tmin);
+                  const float tf = __builtin_fminf(__builtin_fminf(__builtin_fminf((fx - o.x) * idir.x, (fy - o.y) * idir.y),
+                                                                   (fz - o.z) * idir.z), tfc);
+                  const bool h = (info & 3u) != 0u && tn <= tf;
+                  hitmask |= h ? 1u << i : 0u;
                 }
               } else {
 #pragma unroll
