@@ -1156,6 +1156,54 @@ void trace_kernel(TraceParams p) {
                   if (dbit) ldsA[((top_pos + (uint32_t)__builtin_popcount(pm)) & (K - 1)) * BS] = (uint64_t)dslot | ((uint64_t)hi << 32);
                   sa.cnt += n_push;
                 }
+              } else if constexpr (MODE == 1 && !LM) {
+                // Treelet order, same conditions: the children go in slot
+                // order, a child owned by the first treelet to the current
+                // list (A), the others to list B.  Each list first spills its
+                // ring's oldest entries, as many as pushing one by one would
+                // (at most K new entries per list, so new ones never spill
+                // each other), then stores each child at its place.
+                const uint32_t iw0 = __builtin_amdgcn_alignbyte(nd.w[6], nd.w[5], 2);
+                const uint32_t iw1 = nd.w[6] >> 16;
+                uint32_t pre[6], tot = 0, leafm = 0;
+#pragma unroll
+                for (int i = 0; i < 6; ++i) {
+                  const uint32_t info = (i < 4 ? iw0 >> (8 * i) : iw1 >> (8 * (i - 4))) & 0x3fu;
+                  pre[i] = tot;
+                  tot += info & 3u;
+                  leafm |= (info >> 2) != 0u ? 1u << i : 0u;
+                }
+                const uint32_t pmA = hitmask & fbits, pmB = hitmask & ~fbits;
+                const uint32_t nA = (uint32_t)__builtin_popcount(pmA), nB = (uint32_t)__builtin_popcount(pmB);
+                const uint32_t spA = sa.cnt + nA > (uint32_t)K ? sa.cnt + nA - (uint32_t)K : 0u;
+                const uint32_t spB = sb.cnt + nB > (uint32_t)K ? sb.cnt + nB - (uint32_t)K : 0u;
+                const bool ok = first_child < bytes && first_child + (uint64_t)tot * 64 <= bytes &&
+                                !(top && ((iw0 & 0x38383838u) | (iw1 & 0x3838u)) != 0u) && nA <= (uint32_t)K &&
+                                nB <= (uint32_t)K && sa.g + spA <= p.spill_cap && sb.g + spB <= p.spill_cap;
+                if (ok) {
+                  pushed = true;
+                  const uint32_t first_slot = (uint32_t)(first_child >> 6);
+                  const uint32_t hi = base_flags | ((lvl > 255u ? 255u : lvl) << 2) | ((inst & 0x3fffffu) << 10);
+                  auto fill_list = [&](Stack& st, lds_u64* lds, glb_u64* g, uint32_t sp, uint32_t pm, uint32_t n) {
+                    for (uint32_t j = 0; j < sp; ++j) {
+                      g[st.g++] = lds[st.bot * BS];
+                      st.bot = (st.bot + 1) & (K - 1);
+                      st.cnt--;
+                    }
+                    const uint32_t top_pos = st.bot + st.cnt;
+#pragma unroll
+                    for (int i = 0; i < 6; ++i) {
+                      if ((pm >> i) & 1u) {
+                        const uint32_t r = (uint32_t)__builtin_popcount(pm & ((1u << i) - 1u));
+                        lds[((top_pos + r) & (K - 1)) * BS] =
+                            (uint64_t)(first_slot + pre[i]) | ((uint64_t)(hi | (((leafm >> i) & 1u) ? kFlLeaf : 0u)) << 32);
+                      }
+                    }
+                    st.cnt += n;
+                  };
+                  fill_list(sa, ldsA, gA, spA, pmA, nA);
+                  fill_list(sb, ldsB, gB, spB, pmB, nB);
+                }
               }
               int deferred = -1;
               uint32_t deferred_cs = 0;
