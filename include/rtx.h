@@ -158,7 +158,7 @@ typedef struct rtx_trace_options {
   float threshold;       /* POPULARITY share threshold (reference default 0.5, tested 0.75) */
   uint32_t emit_records; /* 1: fill the record buffer (see record_slab) */
   uint32_t waves_per_cu; /* persistent waves per CU; 0 = as many as fit on a CU at once */
-  uint32_t lds_stack;    /* stack entries per lane held in LDS (4, 8, 16); 0 = 8 for PLAIN, 4 for TREELET */
+  uint32_t lds_stack;    /* stack entries per lane (per list in TREELET order) held in LDS (4, 8, 16); 0 = 8 */
   uint32_t stage_bytes;  /* LDS bytes per wave for a staged treelet (4-64 KiB, power of two);
                             0 = the treelet budget, capped at 16 KiB */
   uint32_t refill;       /* a wave fetches new rays once this many lanes are idle (1-64); 0 = 40

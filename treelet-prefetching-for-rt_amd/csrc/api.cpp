@@ -624,7 +624,7 @@ rtx_status trace_impl(rtx_ctx* c, const rtx_ray* rays, uint32_t n, const rtx_tra
   // 16 KiB of LDS stack per 256-lane block (8 entries per lane in plain order,
   // 4 per list in treelet order), so the register budget, not LDS, bounds how
   // many waves share a CU
-  uint32_t lds_stack = opts->lds_stack ? opts->lds_stack : (opts->mode == RTX_MODE_PLAIN ? 8 : 4);
+  uint32_t lds_stack = opts->lds_stack ? opts->lds_stack : 8;
   if (lds_stack != 4 && lds_stack != 8 && lds_stack != 16) return RTX_ERR_INVALID_ARGUMENT;
   // treelet staging: one wave per workgroup with stage_bytes of LDS for the image
   uint32_t stage = 0;
@@ -935,7 +935,7 @@ rtx_status rtx_trace_static(rtx_ctx* c, const rtx_ray* rays, uint32_t n, const r
   if (opts->mode == RTX_MODE_TREELET && !c->part.valid) return RTX_ERR_NO_PARTITION;
   if (((uintptr_t)rays & 15) || ((uintptr_t)hits & 15)) return RTX_ERR_INVALID_ARGUMENT;
   if (!c->static_scratch) return RTX_ERR_HIP;
-  const uint32_t lds_stack = opts->lds_stack ? opts->lds_stack : (opts->mode == RTX_MODE_PLAIN ? 8 : 4);
+  const uint32_t lds_stack = opts->lds_stack ? opts->lds_stack : 8;
   const int ki = lds_stack == 4 ? 0 : lds_stack == 8 ? 1 : lds_stack == 16 ? 2 : -1;
   if (ki < 0) return RTX_ERR_INVALID_ARGUMENT;
   if (opts->lanes_per_ray != 0 && opts->lanes_per_ray != 1 && opts->lanes_per_ray != kWideRayLanes)

@@ -164,7 +164,7 @@ __device__ __forceinline__ uint32_t wave_sum(uint32_t v) {
 // in which a node is reachable at two levels; elsewhere every assignment of a
 // node carries the same value and the traversal's running maximum is exact.
 template <int MODE, int REC, int K, int STAGE, bool MULTI = false, bool LM = false>
-__global__ __launch_bounds__(STAGE ? 64 : 256) __attribute__((amdgpu_waves_per_eu(STAGE ? 1 : (REC == kRecCsr || (MODE == 1 && REC) || LM) ? 4 : 5)))
+__global__ __launch_bounds__(STAGE ? 64 : 256) __attribute__((amdgpu_waves_per_eu(STAGE ? 1 : (REC == kRecCsr || MODE == 1 || LM) ? 4 : 5)))
 void trace_kernel(TraceParams p) {
   static_assert(!MULTI || (!REC && STAGE == 0), "batch kernels write hits only");
   static_assert(!LM || (!REC && STAGE == 0 && !MULTI), "the level pass writes max_level only");
