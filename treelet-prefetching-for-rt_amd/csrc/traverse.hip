@@ -309,12 +309,19 @@ void trace_kernel(TraceParams p) {
       }
     }
   };
-  // the entries still in the ring when the ray's hit is written
+  // The entries still in the ring when the ray's hit is written, as whole
+  // groups too: the row is padded to a multiple of four entries, and the
+  // entries past the ray's count (stale ring words) are never read.
   auto flush_slab_rest = [&]() {
     if constexpr (REC == kRecSlab) {
       const uint32_t end = min(n_rec, p.record_slab);
-      uint64_t* row = p.slabs + (uint64_t)local_id * slab_stride(p.record_slab);
-      for (uint32_t k = rec_flushed; k < end; ++k) row[k] = ring[(k & (kRecRing - 1)) * BS];
+      uint4* row = reinterpret_cast<uint4*>(p.slabs + (uint64_t)local_id * slab_stride(p.record_slab));
+      for (; rec_flushed < end; rec_flushed += 4u) {
+        const uint32_t q = rec_flushed & (kRecRing - 1);
+        const uint64_t a0 = ring[q * BS], a1 = ring[(q + 1) * BS], a2 = ring[(q + 2) * BS], a3 = ring[(q + 3) * BS];
+        row[rec_flushed / 2] = make_uint4((uint32_t)a0, (uint32_t)(a0 >> 32), (uint32_t)a1, (uint32_t)(a1 >> 32));
+        row[rec_flushed / 2 + 1] = make_uint4((uint32_t)a2, (uint32_t)(a2 >> 32), (uint32_t)a3, (uint32_t)(a3 >> 32));
+      }
       rec_flushed = end;
     }
   };
