@@ -146,3 +146,52 @@ def tiny_ray(ray_dtype, flags=1):
 def expected():
     with open(os.path.join(HERE, "tiny_expected.json")) as f:
         return json.load(f)
+
+
+def deep_chain_arena(depth):
+    """A BLAS that is a chain of `depth` + 1 internal nodes, each with five quad
+    leaves and the next node as children (the last has six leaves), under the
+    tiny arena's TLAS and instance leaf; every child box spans x, y in [-1, 3]
+    and z in [0, 4], every triangle lies at x, y in [2, 3] (off the ray of
+    deep_chain_ray).  Plain order keeps the five leaves of every level on the
+    stack while it descends: 5 * depth + 6 entries at the deepest node.
+        0   TLAS header, 64 TLAS root, 128 instance leaf (BVHAddress -> 256)
+      256   BLAS header (root at 320)
+      320 + 384 j   chain node j, its children from 384 + 384 j: five leaves,
+                    then node j + 1 at 704 + 384 j
+    """
+    src = tiny_arena()
+    size = 320 + 384 * depth + 64 + 6 * 64
+    A = bytearray(size)
+    A[0:64] = _header(64, (-1, -1, -1), (3, 3, 6))
+    full = (0, 128, 0, 128, 0, 128)
+    A[64:128] = _internal((-1, -1, 0), (3, 3, 3), 1, [(2, 1, full)])
+    A[128:256] = src[128:256].tobytes()
+    A[256:320] = _header(64, (-1, -1, 0), (3, 3, 4))
+
+    def leaf(prim):
+        b = bytearray(_quad(prim, 3, 2.0))
+        struct.pack_into("<12f", b, 16, 2, 2, 2, 3, 2, 2, 2, 3, 2, 2, 3, 2)
+        return b
+
+    prim = 0
+    for j in range(depth + 1):
+        at = 320 + 384 * j
+        last = j == depth
+        kids = [(1, 4, full)] * 5 + [(1, 4 if last else 0, full)]
+        A[at:at + 64] = _internal((-1, -1, 0), (3, 3, 3), 1, kids)
+        for k in range(6 if last else 5):
+            A[at + 64 + 64 * k:at + 128 + 64 * k] = leaf(prim)
+            prim += 1
+    return np.frombuffer(bytes(A), dtype=np.uint8).copy()
+
+
+def deep_chain_ray(ray_dtype, n=1):
+    r = np.zeros(n, dtype=ray_dtype)
+    r["org"] = (0.25, 0.25, -1.0)
+    r["dir"] = (0.0, 0.0, 1.0)
+    r["tmin"] = 0.0
+    r["tmax"] = 100.0
+    r["flags"] = 1
+    r["cull_mask"] = 0xFF
+    return r

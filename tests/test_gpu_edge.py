@@ -9,6 +9,7 @@ import pytest
 
 sys.path.insert(0, os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden"))
 import tiny_scene  # noqa: E402
+from conftest import assert_hits_equal, assert_records_equal  # noqa: E402
 
 pytestmark = pytest.mark.gpu
 
@@ -174,3 +175,53 @@ def test_inverted_child_boxes_match_the_oracle(gpu, rtx, oracle, c1, mode):
     h_orig = oracle.trace(scene.arena, scene.tlas_offset, rays, mode=mode, partition=None if mode == 0 else
                           oracle.Partition(scene.arena, scene.tlas_offset, 8192), records=False)
     assert np.array_equal(h_orig["n_nodes"], h_ref["n_nodes"]) and h_ref["n_nodes"].sum() > 0
+
+
+@pytest.mark.parametrize("lds_stack", [4, 8, 16])
+def test_stack_capacity_edge(gpu, rtx, oracle, lds_stack):
+    """A chain BLAS whose plain-order stack peaks at 5 * depth + 6 entries: at
+    the deepest depth that fits the lane's K LDS entries plus 128 spilled ones
+    the hits and records equal the oracle's, one level deeper the call reports
+    RTX_ERR_STACK_OVERFLOW, with one lane per ray (the per-node pushes, their
+    spills and the per-child path's overflow check) and with eight."""
+    cap = lds_stack + 128
+    fits = (cap - 6) // 5
+    assert 5 * fits + 6 <= cap < 5 * (fits + 1) + 6
+    ray = tiny_scene.deep_chain_ray(rtx.RAY_DTYPE, n=64)
+    A = tiny_scene.deep_chain_arena(fits)
+    gpu.set_scene(A, 0)
+    h_ref, o_ref, r_ref = oracle.trace(A, 0, ray, mode=0)
+    for lanes in (1, 8):
+        assert_hits_equal(gpu.trace(ray, mode=0, lds_stack=lds_stack, lanes_per_ray=lanes), h_ref,
+                          f"depth {fits}, stack {lds_stack}, {lanes} lanes")
+    h, o, r = gpu.trace(ray, mode=0, records=True, lds_stack=lds_stack)
+    assert_hits_equal(h, h_ref, "records")
+    assert_records_equal(o, r, o_ref, r_ref, f"depth {fits}, stack {lds_stack}")
+    gpu.set_scene(tiny_scene.deep_chain_arena(fits + 1), 0)
+    for lanes in (1, 8):
+        with pytest.raises(rtx.RtxError) as e:
+            gpu.trace(ray, mode=0, lds_stack=lds_stack, lanes_per_ray=lanes)
+        assert e.value.status == 6, (lanes, e.value.status)
+
+
+def test_stack_capacity_treelet_order(gpu, rtx, oracle):
+    """Treelet order on the chain BLAS: at depth 20 the two lists hold 106
+    entries at most, however they split, and fit; at depth 60 the first
+    treelet (1 KiB: 16 nodes) is left early and the other list alone has to
+    take some 290 entries, past its 136."""
+    ray = tiny_scene.deep_chain_ray(rtx.RAY_DTYPE, n=64)
+    A = tiny_scene.deep_chain_arena(20)
+    gpu.set_scene(A, 0)
+    gpu.build_treelets(1024)
+    part = oracle.Partition(A, 0, 1024)
+    h_ref, o_ref, r_ref = oracle.trace(A, 0, ray, mode=1, partition=part)
+    for lanes in (1, 8):
+        assert_hits_equal(gpu.trace(ray, mode=1, lanes_per_ray=lanes), h_ref, f"treelet, {lanes} lanes")
+    h, o, r = gpu.trace(ray, mode=1, records=True)
+    assert_records_equal(o, r, o_ref, r_ref, "treelet records")
+    gpu.set_scene(tiny_scene.deep_chain_arena(60), 0)
+    gpu.build_treelets(1024)
+    for lanes in (1, 8):
+        with pytest.raises(rtx.RtxError) as e:
+            gpu.trace(ray, mode=1, lanes_per_ray=lanes)
+        assert e.value.status == 6, (lanes, e.value.status)
