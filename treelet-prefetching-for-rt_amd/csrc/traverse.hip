@@ -1202,25 +1202,28 @@ void trace_kernel(TraceParams p) {
                   pushed = true;
                   const uint32_t first_slot = (uint32_t)(first_child >> 6);
                   const uint32_t hi = base_flags | ((lvl > 255u ? 255u : lvl) << 2) | ((inst & 0x3fffffu) << 10);
-                  auto fill_list = [&](Stack& st, lds_u64* lds, glb_u64* g, uint32_t sp, uint32_t pm, uint32_t n) {
+                  auto spill_list = [&](Stack& st, lds_u64* lds, glb_u64* g, uint32_t sp) {
                     for (uint32_t j = 0; j < sp; ++j) {
                       g[st.g++] = lds[st.bot * BS];
                       st.bot = (st.bot + 1) & (K - 1);
                       st.cnt--;
                     }
-                    const uint32_t top_pos = st.bot + st.cnt;
-#pragma unroll
-                    for (int i = 0; i < 6; ++i) {
-                      if ((pm >> i) & 1u) {
-                        const uint32_t r = (uint32_t)__builtin_popcount(pm & ((1u << i) - 1u));
-                        lds[((top_pos + r) & (K - 1)) * BS] =
-                            (uint64_t)(first_slot + pre[i]) | ((uint64_t)(hi | (((leafm >> i) & 1u) ? kFlLeaf : 0u)) << 32);
-                      }
-                    }
-                    st.cnt += n;
                   };
-                  fill_list(sa, ldsA, gA, spA, pmA, nA);
-                  fill_list(sb, ldsB, gB, spB, pmB, nB);
+                  spill_list(sa, ldsA, gA, spA);
+                  spill_list(sb, ldsB, gB, spB);
+                  // one store per hit child, into the list it goes to
+                  const uint32_t topA = sa.bot + sa.cnt, topB = sb.bot + sb.cnt;
+#pragma unroll
+                  for (int i = 0; i < 6; ++i) {
+                    if ((hitmask >> i) & 1u) {
+                      const bool a = (fbits >> i) & 1u;
+                      const uint32_t r = (uint32_t)__builtin_popcount((a ? pmA : pmB) & ((1u << i) - 1u));
+                      (a ? ldsA : ldsB)[(((a ? topA : topB) + r) & (K - 1)) * BS] =
+                          (uint64_t)(first_slot + pre[i]) | ((uint64_t)(hi | (((leafm >> i) & 1u) ? kFlLeaf : 0u)) << 32);
+                    }
+                  }
+                  sa.cnt += nA;
+                  sb.cnt += nB;
                 }
               }
               int deferred = -1;
