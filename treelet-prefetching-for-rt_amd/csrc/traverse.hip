@@ -1161,17 +1161,16 @@ void trace_kernel(TraceParams p) {
                   const uint32_t first_slot = (uint32_t)(first_child >> 6);
                   const uint32_t hi = base_flags | ((lvl > 255u ? 255u : lvl) << 2) | ((inst & 0x3fffffu) << 10);
                   const uint32_t top_pos = sa.bot + sa.cnt;
-                  uint32_t dslot = 0;
 #pragma unroll
                   for (int i = 0; i < 6; ++i) {
-                    if ((dbit >> i) & 1u) dslot = first_slot + pre[i];
-                    if ((pm >> i) & 1u) {
-                      const uint32_t r = (uint32_t)__builtin_popcount(pm & ((1u << i) - 1u));
+                    if ((hitmask >> i) & 1u) {
+                      // the deferred child takes the last place
+                      const uint32_t r = ((dbit >> i) & 1u) ? n_push - 1u
+                                                             : (uint32_t)__builtin_popcount(pm & ((1u << i) - 1u));
                       ldsA[((top_pos + r) & (K - 1)) * BS] =
                           (uint64_t)(first_slot + pre[i]) | ((uint64_t)(hi | (((leafm >> i) & 1u) ? kFlLeaf : 0u)) << 32);
                     }
                   }
-                  if (dbit) ldsA[((top_pos + (uint32_t)__builtin_popcount(pm)) & (K - 1)) * BS] = (uint64_t)dslot | ((uint64_t)hi << 32);
                   sa.cnt += n_push;
                 }
               } else if constexpr (MODE == 1 && !LM) {
