@@ -55,7 +55,11 @@ typedef enum rtx_status {
   RTX_ERR_CYCLE = 15                   /* rtx_validate_scene: a node the TLAS reaches reaches itself */
 } rtx_status;
 
-/* Traversal order. PLAIN follows traceRay, TREELET follows traceRayWithTreelets. */
+/* Traversal order. PLAIN follows traceRay, TREELET follows traceRayWithTreelets.
+ * TREELET keeps each stack entry's instance leaf as a signed 22-bit slot distance
+ * from the TLAS header: a ray that reaches an instance leaf 128 MiB or more away
+ * from the header, on either side, reports RTX_ERR_UNSUPPORTED.  PLAIN has no
+ * such limit. */
 enum { RTX_MODE_PLAIN = 0, RTX_MODE_TREELET = 1 };
 
 /* Treelet staging (prefetch into LDS).  Never changes hits or traces. */
