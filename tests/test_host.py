@@ -224,3 +224,32 @@ def test_numpy_records_match_the_headers(rtx, name, dtype):
     assert list(dt.names) == [f[0] for f in fields]
     assert [dt.fields[f[0]][1] for f in fields] == [f[1] for f in fields]
     assert dt.itemsize == size
+
+
+def test_smoke_comparison_skips_only_the_oracles_path_level(rtx, oracle, c1):
+    """smoke() compares the library's hit records with the oracle's field by
+    field: the oracle's last word (path_level) sits where the library keeps a
+    reserved word, and a NaN equals any NaN; every other difference counts.
+    The library-side records here are the oracle's own, in the library's
+    layout (reserved word 0), with single mistakes planted."""
+    import __graft_entry__ as g
+
+    scene, rays = c1
+    ref = oracle.trace(scene.arena, scene.tlas_offset, rays[:2048], records=False)
+    assert (ref["path_level"] != 0).any()
+    lib = np.zeros(len(ref), rtx.HIT_DTYPE)
+    for f in rtx.HIT_DTYPE.names:
+        if f in ref.dtype.names:
+            lib[f] = ref[f]
+    assert lib.tobytes() != ref.tobytes() and g._same_hits(lib, ref)
+    hit = int(np.nonzero(ref["hit"])[0][0])
+    nan = lib.copy()
+    nan["bary"][hit, 0] = np.float32(np.nan)
+    nref = ref.copy()
+    nref["bary"][hit, 0] = -np.float32(np.nan)
+    assert g._same_hits(nan, nref)
+    for field, change in (("t", lambda a: np.nextafter(a, np.float32(np.inf))), ("n_nodes", lambda a: a + 1),
+                          ("instance_leaf", lambda a: a ^ 64), ("p", lambda a: -a)):
+        bad = lib.copy()
+        bad[field][hit] = change(bad[field][hit])
+        assert not g._same_hits(bad, ref), field
