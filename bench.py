@@ -235,7 +235,8 @@ def setup_single(rtx, ctx, args, info):
     builder = rtx.workload_builder(desc)
     mesh_s = time.perf_counter() - t0
     t0 = time.perf_counter()
-    scene = builder.finish(ctx=ctx if args.build == "device" else None)
+    scene = builder.finish(ctx=ctx if args.build == "device" else None,
+                           quality=rtx.BUILD_SAH_OPTIMAL if args.tree == "sah" else rtx.BUILD_DEFAULT)
     build_s = time.perf_counter() - t0
     log(f"scene: {scene.n_triangles} tris, {scene.n_blas} BLAS, {scene.arena.nbytes / 2**20:.1f} MiB arena, "
         f"meshes in {mesh_s:.2f} s, BVH built ({args.build}) in {build_s:.2f} s")
@@ -281,7 +282,7 @@ def setup_single(rtx, ctx, args, info):
                     f"roulette) ({sum(len(x) for x in rays)} rays/step)")
     del hits
     segs = [Segment(rtx, ctx, scene, r, claim_start=claim if k == 0 else 0) for k, r in enumerate(rays)]
-    meta = {"arena_mib": round(scene.arena.nbytes / 2**20, 2), "bvh_build": args.build,
+    meta = {"arena_mib": round(scene.arena.nbytes / 2**20, 2), "bvh_build": args.build, "tree": args.tree,
             "bvh_build_s": round(build_s, 3), "mesh_gen_s": round(mesh_s, 3)}
     return workload, segs, train_rays, scene, meta
 
@@ -320,6 +321,9 @@ def main():
     ap.add_argument("--workload", default="diffuse", choices=["diffuse", "large", "c3", "c1"])
     ap.add_argument("--build", default="host", choices=["host", "device"],
                     help="BLAS builder: host SAH (default) or the device radix-tree builder")
+    ap.add_argument("--tree", default="default", choices=["default", "sah"],
+                    help="host tree quality of the diffuse, large and c1 scenes: default, or sah "
+                         "(three-axis SAH, SAH-optimal 6-wide collapse)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="copy MIB MiB device to device once before tracing: a dispatch of known size "
                          "that calibrates the FETCH_SIZE / WRITE_SIZE counters")
@@ -509,7 +513,9 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": round(head["node_fetch_gbs"] / HBM_PEAK_GBS, 4),
-            "traffic": load_traffic(args.workload, head_mode),
+            # the committed counters were taken on the default host-built trees
+            "traffic": (load_traffic(args.workload, head_mode)
+                        if args.build == "host" and args.tree == "default" else None),
             "kernel": single_kernel,
             "kernel_ms": round(head["kernel_ms"], 4),
             "bytes_per_launch": head["fetch_bytes"] / (len(segs) if MODES[head_mode]["launch"] == "trace" else 1),

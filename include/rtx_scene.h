@@ -30,6 +30,17 @@ typedef struct rtx_builder rtx_builder;
 
 rtx_status rtx_builder_create(rtx_builder** out);
 rtx_status rtx_builder_destroy(rtx_builder* b);
+
+/* Tree quality of the host build (the reference takes its trees from Embree, so
+ * there is no counterpart in it).  DEFAULT: binned SAH on the axis of the
+ * largest centroid extent, collapsed to 6-wide nodes by opening the child with
+ * the largest surface area.  SAH_OPTIMAL: binned SAH over all three axes, and
+ * the collapse that minimises the surface-area cost of the 6-wide tree (a
+ * dynamic programme over the binary tree); fewer nodes per ray, a slower build.
+ * Set before rtx_builder_finish (BLASes and TLAS) or rtx_builder_finish_device
+ * (the TLAS only; the device builds its BLASes one way). */
+enum { RTX_BUILD_DEFAULT = 0, RTX_BUILD_SAH_OPTIMAL = 1 };
+rtx_status rtx_builder_set_quality(rtx_builder* b, uint32_t quality);
 /* Add a triangle BLAS.  tris holds n_tris * 9 floats (v0 v1 v2, xyz each). */
 rtx_status rtx_builder_add_blas(rtx_builder* b, const float* tris, uint32_t n_tris,
                                 uint32_t geometry_index, uint32_t* blas_id);

@@ -62,8 +62,9 @@ struct BNode {
   int32_t item = -1;  // >= 0: leaf holding item
 };
 
-// Binned SAH over item boxes; one item per leaf.
-void build_binary(const std::vector<Box>& items, std::vector<BNode>& nodes) {
+// Binned SAH over item boxes; one item per leaf.  The split is searched on the
+// axis of the largest centroid extent, or with all_axes on each of the three.
+void build_binary(const std::vector<Box>& items, std::vector<BNode>& nodes, bool all_axes) {
   const int n = (int)items.size();
   nodes.clear();
   if (n == 0) return;
@@ -106,17 +107,18 @@ void build_binary(const std::vector<Box>& items, std::vector<BNode>& nodes) {
         axis = a;
       }
     int mid = -1;
-    if (ext > 0.0f) {
+    double best = INFINITY;
+    int best_k = -1, best_a = -1;
+    for (int a = all_axes ? 0 : axis; a < (all_axes ? 3 : axis + 1); ++a) {
+      const float ea = cb.hi[a] - cb.lo[a];
+      if (!(ea > 0.0f)) continue;
       Box bin_box[kBins];
       int bin_cnt[kBins] = {0};
       for (int k = 0; k < kBins; ++k) bin_box[k].reset();
-      const double scale = kBins / (double)ext;
-      auto bin_of = [&](int item) {
-        int k = (int)((cen[3 * (size_t)item + axis] - cb.lo[axis]) * scale);
-        return std::min(kBins - 1, std::max(0, k));
-      };
+      const double scale = kBins / (double)ea;
       for (int i = j.begin; i < j.end; ++i) {
-        int k = bin_of(idx[i]);
+        int k = (int)((cen[3 * (size_t)idx[i] + a] - cb.lo[a]) * scale);
+        k = std::min(kBins - 1, std::max(0, k));
         bin_cnt[k]++;
         bin_box[k].grow(items[idx[i]]);
       }
@@ -133,8 +135,6 @@ void build_binary(const std::vector<Box>& items, std::vector<BNode>& nodes) {
       }
       acc.reset();
       c = 0;
-      double best = INFINITY;
-      int best_k = -1;
       for (int k = kBins - 1; k >= 1; --k) {
         acc.grow(bin_box[k]);
         c += bin_cnt[k];
@@ -143,14 +143,19 @@ void build_binary(const std::vector<Box>& items, std::vector<BNode>& nodes) {
         if (cost < best) {
           best = cost;
           best_k = k;
+          best_a = a;
         }
       }
-      if (best_k > 0) {
-        auto it = std::partition(idx.begin() + j.begin, idx.begin() + j.end,
-                                 [&](int32_t item) { return bin_of(item) < best_k; });
-        mid = (int)(it - idx.begin());
-        if (mid == j.begin || mid == j.end) mid = -1;
-      }
+    }
+    if (best_k > 0) {
+      const double scale = kBins / (double)(cb.hi[best_a] - cb.lo[best_a]);
+      const float lo_a = cb.lo[best_a];
+      auto it = std::partition(idx.begin() + j.begin, idx.begin() + j.end, [&](int32_t item) {
+        int k = (int)((cen[3 * (size_t)item + best_a] - lo_a) * scale);
+        return std::min(kBins - 1, std::max(0, k)) < best_k;
+      });
+      mid = (int)(it - idx.begin());
+      if (mid == j.begin || mid == j.end) mid = -1;
     }
     if (mid < 0) {
       // Coincident centroids: split by a stable median on the axis.
@@ -172,11 +177,83 @@ void build_binary(const std::vector<Box>& items, std::vector<BNode>& nodes) {
   }
 }
 
-// Up to six binary-tree nodes that become the children of 6-wide node `id`.
-int collapse_children(const std::vector<BNode>& nodes, int32_t id, int32_t out[6]) {
+// SAH-optimal collapse of a binary tree to 6-wide nodes.  D(n, j) is the least
+// cost of covering binary node n's subtree with at most j roots, where a root
+// costs its box area (a ray's chance of testing what lies under it) plus, for
+// an internal root, the cost of covering its own subtree with at most six
+// roots.  Leaves are one item each and every leaf is a root exactly once, so
+// only internal nodes change the cost.
+struct CollapsePlan {
+  std::vector<int8_t> inner;  // per node: roots given to the left child when it is a 6-wide node
+  std::vector<int8_t> split;  // per node and j = 2..6: 0 keeps the node as one root, else roots to the left
+};
+
+void plan_collapse(const std::vector<BNode>& nodes, CollapsePlan& p) {
+  const size_t n = nodes.size();
+  p.inner.assign(n, 0);
+  p.split.assign(7 * n, 0);
+  std::vector<double> D(7 * n, 0.0);
+  // children are created after their parent, so a backward sweep is bottom-up
+  for (size_t q = n; q-- > 0;) {
+    const BNode& b = nodes[q];
+    const double a = b.box.area();
+    if (b.item >= 0) {
+      for (int j = 1; j <= 6; ++j) D[7 * q + j] = a;
+      continue;
+    }
+    const size_t l = (size_t)b.left, r = (size_t)b.right;
+    double best = INFINITY;
+    int bk = 1;
+    for (int k = 1; k <= 5; ++k) {
+      const double c = D[7 * l + k] + D[7 * r + 6 - k];
+      if (c < best) {
+        best = c;
+        bk = k;
+      }
+    }
+    p.inner[q] = (int8_t)bk;
+    D[7 * q + 1] = a + best;
+    for (int j = 2; j <= 6; ++j) {
+      double bj = D[7 * q + 1];
+      int sk = 0;
+      for (int k = 1; k < j; ++k) {
+        const double c = D[7 * l + k] + D[7 * r + j - k];
+        if (c < bj) {
+          bj = c;
+          sk = k;
+        }
+      }
+      D[7 * q + j] = bj;
+      p.split[7 * q + j] = (int8_t)sk;
+    }
+  }
+}
+
+// The roots that cover node x's subtree with at most j of them, in tree order.
+void plan_roots(const std::vector<BNode>& nodes, const CollapsePlan& p, int32_t x, int j, int32_t* out, int& n) {
+  const int k = (j >= 2 && nodes[x].item < 0) ? p.split[7 * (size_t)x + j] : 0;
+  if (k == 0) {
+    out[n++] = x;
+    return;
+  }
+  plan_roots(nodes, p, nodes[x].left, k, out, n);
+  plan_roots(nodes, p, nodes[x].right, j - k, out, n);
+}
+
+// Up to six binary-tree nodes that become the children of 6-wide node `id`:
+// from the plan when there is one, else by opening the child with the largest
+// surface area until six are open.
+int collapse_children(const std::vector<BNode>& nodes, const CollapsePlan* plan, int32_t id, int32_t out[6]) {
   if (nodes[id].item >= 0) {
     out[0] = id;
     return 1;
+  }
+  if (plan) {
+    int n = 0;
+    const int k = plan->inner[id];
+    plan_roots(nodes, *plan, nodes[id].left, k, out, n);
+    plan_roots(nodes, *plan, nodes[id].right, 6 - k, out, n);
+    return n;
   }
   int n = 2;
   out[0] = nodes[id].left;
@@ -280,12 +357,16 @@ void write_header(uint8_t* dst, uint64_t root_offset, const Box& b) {
 
 // Lays out a BVH (header + nodes + leaves) into a position-independent blob.
 // leaf_bytes: 64 (quad) or 128 (instance).  write_leaf(dst, item) fills a leaf.
+// sah_collapse: 6-wide nodes from the SAH-optimal plan, else the largest-area opening.
 template <typename LeafWriter>
 bool emit_bvh(const std::vector<BNode>& nodes, uint32_t leaf_type, uint32_t leaf_bytes, LeafWriter write_leaf,
-              std::vector<uint8_t>& blob, uint64_t* n_internal) {
+              std::vector<uint8_t>& blob, uint64_t* n_internal, bool sah_collapse) {
   blob.clear();
   *n_internal = 0;
   if (nodes.empty()) return false;
+  CollapsePlan plan_store;
+  if (sah_collapse) plan_collapse(nodes, plan_store);
+  const CollapsePlan* plan = sah_collapse ? &plan_store : nullptr;
   // Pass 1: assign offsets breadth first.
   struct Item {
     int32_t bnode;
@@ -297,7 +378,7 @@ bool emit_bvh(const std::vector<BNode>& nodes, uint32_t leaf_type, uint32_t leaf
   std::vector<uint64_t> block_of;  // children block offset per queue entry
   for (size_t qi = 0; qi < queue.size(); ++qi) {
     int32_t ch[6];
-    int nc = collapse_children(nodes, queue[qi].bnode, ch);
+    int nc = collapse_children(nodes, plan, queue[qi].bnode, ch);
     block_of.push_back(cursor);
     for (int k = 0; k < nc; ++k) {
       bool leaf = nodes[ch[k]].item >= 0;
@@ -309,7 +390,7 @@ bool emit_bvh(const std::vector<BNode>& nodes, uint32_t leaf_type, uint32_t leaf
   write_header(blob.data(), kHeaderBytes, nodes[0].box);
   for (size_t qi = 0; qi < queue.size(); ++qi) {
     int32_t ch[6];
-    int nc = collapse_children(nodes, queue[qi].bnode, ch);
+    int nc = collapse_children(nodes, plan, queue[qi].bnode, ch);
     Box boxes[6];
     uint32_t types[6], sizes[6];
     uint64_t coff = block_of[qi];
@@ -357,12 +438,13 @@ struct rtx_builder {
   std::vector<uint8_t> arena;
   uint64_t tlas_offset = 0;
   uint64_t tlas_internal = 0;
+  uint32_t quality = RTX_BUILD_DEFAULT;
   bool finished = false;
 };
 
 namespace {
 
-void build_blas(Blas& b) {
+void build_blas(Blas& b, uint32_t quality) {
   const uint32_t n = (uint32_t)(b.tris.size() / 9);
   std::vector<Box> boxes(n);
   for (uint32_t i = 0; i < n; ++i) {
@@ -370,7 +452,8 @@ void build_blas(Blas& b) {
     for (int v = 0; v < 3; ++v) boxes[i].grow(&b.tris[9 * (size_t)i + 3 * v]);
   }
   std::vector<BNode> nodes;
-  build_binary(boxes, nodes);
+  const bool sah = quality == RTX_BUILD_SAH_OPTIMAL;
+  build_binary(boxes, nodes, sah);
   const float* tris = b.tris.data();
   const uint32_t geom = b.geometry_index;
   auto leaf = [tris, geom](uint8_t* dst, int32_t item) {
@@ -383,7 +466,7 @@ void build_blas(Blas& b) {
     memcpy(dst + kQlV0, t, 36);
     memcpy(dst + kQlV0 + 36, t + 6, 12);  // vertex 3 repeats vertex 2 (one triangle per leaf)
   };
-  b.ok = emit_bvh(nodes, kTypeQuad, kQuadLeafBytes, leaf, b.blob, &b.n_internal);
+  b.ok = emit_bvh(nodes, kTypeQuad, kQuadLeafBytes, leaf, b.blob, &b.n_internal, sah);
   if (!nodes.empty()) b.box = nodes[0].box;
 }
 
@@ -423,6 +506,13 @@ rtx_status rtx_builder_create(rtx_builder** out) {
 
 rtx_status rtx_builder_destroy(rtx_builder* b) {
   delete b;
+  return RTX_OK;
+}
+
+rtx_status rtx_builder_set_quality(rtx_builder* b, uint32_t quality) {
+  if (!b || b->finished || (quality != RTX_BUILD_DEFAULT && quality != RTX_BUILD_SAH_OPTIMAL))
+    return RTX_ERR_INVALID_ARGUMENT;
+  b->quality = quality;
   return RTX_OK;
 }
 
@@ -493,7 +583,8 @@ rtx_status assemble(rtx_builder* b, const uint8_t* pool_dev, uint64_t pool_bytes
     b->inst[k].world_box = wb;
   }
   std::vector<BNode> nodes;
-  build_binary(boxes, nodes);
+  const bool sah = b->quality == RTX_BUILD_SAH_OPTIMAL;
+  build_binary(boxes, nodes, sah);
   std::vector<uint8_t> tlas;
   std::vector<uint64_t> leaf_pos(b->inst.size(), 0);
   const std::vector<Instance>& insts = b->inst;
@@ -507,7 +598,7 @@ rtx_status assemble(rtx_builder* b, const uint8_t* pool_dev, uint64_t pool_bytes
     store_as<uint32_t>(dst + kIlInstanceIndex, (uint32_t)item);
     memcpy(dst + kIlObjectToWorld, in.o2w_rows, 48);
   };
-  if (!emit_bvh(nodes, kTypeInstance, kInstanceLeafBytes, leaf, tlas, &b->tlas_internal))
+  if (!emit_bvh(nodes, kTypeInstance, kInstanceLeafBytes, leaf, tlas, &b->tlas_internal, sah))
     return RTX_ERR_INVALID_ARGUMENT;
   // Locate each instance leaf by walking the TLAS (InstanceIndex names the instance).
   {
@@ -577,7 +668,7 @@ rtx_status rtx_builder_finish(rtx_builder* b, uint32_t threads, uint64_t* arena_
   nt = std::min<unsigned>(nt, (unsigned)b->blas.size());
   std::atomic<size_t> next{0};
   auto worker = [&]() {
-    for (size_t i = next++; i < b->blas.size(); i = next++) build_blas(b->blas[i]);
+    for (size_t i = next++; i < b->blas.size(); i = next++) build_blas(b->blas[i], b->quality);
   };
   std::vector<std::thread> pool;
   for (unsigned t = 1; t < nt; ++t) pool.emplace_back(worker);

@@ -39,6 +39,7 @@ TX_BVH_STRUCTURE, TX_BVH_INTERNAL_NODE, TX_BVH_INSTANCE_LEAF = 0, 1, 2
 TX_BVH_PRIMITIVE_LEAF_DESCRIPTOR, TX_BVH_QUAD_LEAF, TX_BVH_QUAD_LEAF_HIT, TX_BVH_PROCEDURAL_LEAF = 3, 4, 5, 6
 NO_TREELET = 0xFFFFFFFF
 WORKLOAD_SPHERE_AND_QUAD, WORKLOAD_INSTANCED_SPHERES = 0, 1
+BUILD_DEFAULT, BUILD_SAH_OPTIMAL = 0, 1
 
 RAY_DTYPE = np.dtype([("org", "<f4", 3), ("tmin", "<f4"), ("dir", "<f4", 3), ("tmax", "<f4"),
                       ("flags", "<u4"), ("cull_mask", "<u4"), ("sbt_record_offset", "<u4"),
@@ -191,6 +192,7 @@ _sig("rtx_as_dump_info", _i32, ctypes.c_char_p, _P(_u64), _P(_u64))
 _sig("rtx_as_dump_read", _i32, ctypes.c_char_p, _vp, _u64)
 _sig("rtx_builder_create", _i32, _P(_vp))
 _sig("rtx_builder_destroy", _i32, _vp)
+_sig("rtx_builder_set_quality", _i32, _vp, _u32)
 _sig("rtx_builder_add_blas", _i32, _vp, _vp, _u32, _u32, _P(_u32))
 _sig("rtx_builder_add_instance", _i32, _vp, _u32, _vp, _u32, _u32)
 _sig("rtx_builder_finish", _i32, _vp, _u32, _P(_u64), _P(_u64))
@@ -216,7 +218,7 @@ EXPORTED_SYMBOLS = [
     "rtx_batch_graph_launch", "rtx_batch_graph_destroy", "rtx_autotune_batch", "rtx_autotune_batch_treelets",
     "rtx_set_prefetch_table", "rtx_mlp_create", "rtx_mlp_destroy", "rtx_mlp_param_count", "rtx_mlp_get_params",
     "rtx_mlp_set_params", "rtx_mlp_loss_grad", "rtx_mlp_train", "rtx_mlp_predict", "rtx_mlp_gemm",
-    "rtx_builder_create", "rtx_builder_destroy", "rtx_builder_add_blas", "rtx_builder_add_instance",
+    "rtx_builder_create", "rtx_builder_destroy", "rtx_builder_set_quality", "rtx_builder_add_blas", "rtx_builder_add_instance",
     "rtx_builder_finish", "rtx_builder_finish_device", "rtx_workload_scene_device", "rtx_workload_builder", "rtx_builder_copy_arena", "rtx_builder_arena_info", "rtx_builder_counts", "rtx_builder_blas_offset",
     "rtx_mesh_displaced_sphere", "rtx_workload_scene", "rtx_workload_primary_rays", "rtx_workload_secondary_rays",
     "rtx_format_treelet_trace", "rtx_as_dump_write", "rtx_as_dump_info", "rtx_as_dump_read",
@@ -319,9 +321,12 @@ class SceneBuilder:
         _check(_lib.rtx_builder_add_instance(self._h, blas_id, _ptr(m), instance_id, hit_group),
                "rtx_builder_add_instance")
 
-    def finish(self, threads=0, ctx=None):
-        """Build on the host (SAH), or with `ctx` the BLASes on that context's device."""
+    def finish(self, threads=0, ctx=None, quality=BUILD_DEFAULT):
+        """Build on the host (SAH), or with `ctx` the BLASes on that context's device.
+        quality: BUILD_DEFAULT or BUILD_SAH_OPTIMAL (rtx_builder_set_quality; on the
+        device path it applies to the TLAS only)."""
         nb, tl = ctypes.c_uint64(0), ctypes.c_uint64(0)
+        _check(_lib.rtx_builder_set_quality(self._h, quality), "rtx_builder_set_quality")
         if ctx is None:
             _check(_lib.rtx_builder_finish(self._h, threads, ctypes.byref(nb), ctypes.byref(tl)), "rtx_builder_finish")
         else:
