@@ -162,6 +162,29 @@ def test_sah_optimal_scene_traversal(gpu, rtx, oracle, which):
             assert h["n_nodes"].mean() < h_base["n_nodes"].mean()
 
 
+def test_sah_optimal_tlas_over_device_built_blas(gpu, rtx, oracle):
+    """The device build with the SAH-optimal setting (it applies to the host-built
+    TLAS there): bit-exact traversal against the oracle, and the same hits and
+    distances as the default device build."""
+    d = small_instanced_desc(rtx, n_instances=64)
+    s = rtx.workload_builder(d).finish(ctx=gpu, quality=rtx.BUILD_SAH_OPTIMAL)
+    base = rtx.workload_builder(d).finish(ctx=gpu)
+    rays = rtx.primary_rays(d)
+    gpu.set_scene(s.arena, s.tlas_offset)
+    for mode in (0, 1):
+        part = None
+        if mode == 1:
+            gpu.build_treelets(4096)
+            part = oracle.Partition(s.arena, s.tlas_offset, 4096)
+        h, o, r = gpu.trace(rays, mode=mode, records=True)
+        h_ref, o_ref, r_ref = oracle.trace(s.arena, s.tlas_offset, rays, mode=mode, partition=part)
+        assert_hits_equal(h, h_ref, f"SAH-optimal TLAS over device BLAS, mode {mode}")
+        assert_records_equal(o, r, o_ref, r_ref, f"SAH-optimal TLAS over device BLAS, mode {mode}")
+    h_base = oracle.trace(base.arena, base.tlas_offset, rays, records=False)
+    assert np.array_equal(h_ref["hit"], h_base["hit"])
+    assert np.array_equal(h_ref["t"][h_ref["hit"] == 1], h_base["t"][h_base["hit"] == 1])
+
+
 def test_device_builder_speed_and_quality(gpu, rtx, oracle):
     import bench
 
